@@ -1,0 +1,289 @@
+"""Oracle FP64 sea-level time step.  Test infrastructure only (see ``oracle/__init__.py``).
+
+Restates the reference's exact-memory solver:
+
+* ``SeaLevelSolver.__init__`` step-0 set-up (``python/slevolve/solver.py:521-605``);
+* ``SeaLevelSolver.step`` with ``ramp`` off, ``axial_rotation`` off, no rates / horizontal
+  read-out (solver.py:814-1144): modal advance, grounded-ice projection on the previous converged
+  topography, eustatic first guess, the inner fixed point (rotation, eq. B18, synthesis, rO,
+  analysis, conservation constant, ocean-load update, convergence ratio, ocean-function and
+  grounded-ice re-guess), radial displacement, commit and the degree-0 diagnostics;
+* ``inner="extrap"``: the extrapolated inner start of ``OptSeaLevelSolver``
+  (``solver_opt.py:148-165,504-527,817-822``).
+
+Constants are those of ``solver.py:31-39,98-110``.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .memory import ExactMemory
+
+PI = 3.1415926535898
+EPS0 = float(np.finfo(np.float64).eps)
+GR_MAXACC = 3
+MAX_INNER = 9999
+
+
+class Planet:
+    def __init__(self):
+        self.radius = 6.371e6
+        self.mass = 5.976e24
+        self.rhoi = 920.0
+        self.rhow = 1000.0
+        self.gacc = 9.80665
+        self.omega = 7.292e-5
+        self.moiA = 0.3296145 * self.mass * self.radius**2
+        self.moiC = 0.3307007 * self.mass * self.radius**2
+        self.kf = 0.9342 + 0.008
+
+
+def extrap_guess(h1, h2, dt0, dt1, dt2):
+    """dS0_j = dt_j [Sdot_{j-1} + (Sdot_{j-1} - Sdot_{j-2}) (dt_j + dt_{j-1})/(dt_{j-1} + dt_{j-2})],
+    Sdot_i = dS_i/dt_i; equal steps give exactly 2 h1 - h2 (solver_opt.py:148-165)."""
+    if dt0 == dt1 == dt2:
+        return h1 * 2.0 - h2
+    w = (dt0 + dt1) / (dt1 + dt2)
+    return h1 * (dt0 / dt1 * (1.0 + w)) - (dt0 / dt2 * w) * h2
+
+
+def convergence_ratio(sa, so):
+    """xi of solver.py:1025-1030 with its EPS0 guards."""
+    if abs(sa - so) < EPS0 and abs(so) < EPS0:
+        return 0.0
+    if abs(so) < EPS0:
+        return abs((sa - so) / (EPS0 * 10))
+    return abs((sa - so) / so)
+
+
+class SeaLevelSolver:
+    def __init__(self, sht, love, topo0, ice0, *, norder, dt=1.0, tpw=True, checkmarine=True,
+                 calcRG=True, epsilon1=1.0e-5, inner="off", planet=None):
+        if inner not in ("off", "extrap"):
+            raise ValueError("inner must be 'off' or 'extrap'")
+        self.sht, self.love = sht, love
+        self.L = norder
+        self.nl = norder + 1
+        self.dt = float(dt)
+        self.tpw, self.checkmarine, self.calcRG = tpw, checkmarine, calcRG
+        self.epsilon1 = epsilon1
+        self.inner = inner
+        self.p = planet or Planet()
+        p = self.p
+        self.tinit = np.array(topo0, dtype=np.float64)
+        self.cxy0 = (self.tinit < 0).astype(np.float64)
+        g0 = self._grounded(ice0, self.tinit) if checkmarine else np.asarray(ice0, np.float64)
+        self.beta0 = (g0 < EPS0).astype(np.float64)
+        self.cstar0 = self.cxy0 * self.beta0
+        self.cxy = self.cxy0.copy()
+        self.topo_prev = self.tinit.copy()
+        area = 4 * PI * p.radius**2
+        self.ocean_area0 = sht.analysis(self.cxy0)[0, 0].real * area
+        self.oceanBeta_area0 = sht.analysis(self.cstar0)[0, 0].real * area
+        self.ice_volume0 = sht.analysis(g0)[0, 0].real * area
+        self.state = ExactMemory(love, norder, self.dt)
+        nl = self.nl
+        ell = np.arange(nl, dtype=np.float64)
+        ttl = np.zeros(nl)
+        ttl[1:] = (4.0 * PI * p.radius**3) / (p.mass * (2 * ell[1:] + 1.0))
+        ekhl = np.zeros(nl)
+        ekhl[1:] = 1 + love.ke[1:nl] - love.he[1:nl]
+        he = np.zeros(nl)
+        he[1:] = love.he[1:nl]
+        self.ttl = ttl[:, None]
+        self.ttlekhl = (ttl * ekhl)[:, None]
+        self.ttlhe = (ttl * he)[:, None]
+        self.il = np.zeros((3, 3))
+        self.mm = np.zeros(3)
+        self.lam = np.zeros((3, 3), dtype=np.complex128)
+        self.beta = np.zeros_like(self.tinit)
+        self.icestarxy = np.zeros_like(self.tinit)
+        self.ice_prev = np.array(ice0, dtype=np.float64)
+        self.iter = 0
+        self.hist = []  # converged load_inc of the last two steps (newest first), with dt
+        self.gr_counts = {"reguess": 0, "ice_analyses": 0}
+
+    def _grounded(self, ice, topo):
+        """I* = ice where (topo > 0) or ice > |topo| rho_w / rho_i, else 0 (solver.py:655-660)."""
+        p = self.p
+        ice = np.asarray(ice, dtype=np.float64)
+        keep = (topo > 0) | (ice > np.abs(topo) * p.rhow / p.rhoi)
+        return ice * keep
+
+    def set_step_dt(self, dt):
+        if dt is not None:
+            self.dt = float(dt)
+            self.state.set_dt(self.dt)
+
+    # ------------------------------------------------------------------------------------
+    def step(self, ice_cur, dt=None):
+        self.set_step_dt(dt)
+        p, sht, st = self.p, self.sht, self.state
+        self.iter += 1
+        ice_cur = np.asarray(ice_cur, dtype=np.float64)
+        visc, rad, sum_il, sum_m, sum_tsl, sum_trr = st.advance_and_read()
+        dS_prev = st.p_ocean.copy()
+        old_il, old_m, lam_prev2 = st.p_il.copy(), st.p_m.copy(), st.p_lambda.copy()
+
+        # ---- ice: grounded projection on the previous converged topography
+        if self.checkmarine:
+            self.icestarxy = self._grounded(ice_cur, self.topo_prev)
+        else:
+            self.icestarxy = ice_cur.copy()
+        icestar_lm = sht.analysis(self.icestarxy)
+        if self.iter == 1:
+            src = self._grounded(self.ice_prev, self.tinit) if self.checkmarine else self.ice_prev
+            oldice = sht.analysis(src)
+        else:
+            oldice = st.prev_ice.copy()
+        dice = icestar_lm - oldice
+        dI = st.p_ice + dice
+
+        # ---- ocean function on the last converged mask
+        self.beta = (self.icestarxy < EPS0).astype(np.float64)
+        cstar = self.cxy * self.beta
+        cstar_lm = sht.analysis(cstar)
+        tO = self.tinit * (cstar - self.cstar0)
+        tO_lm = sht.analysis(tO)
+
+        # ---- first guess
+        load_inc = (cstar_lm / cstar_lm[0, 0]) * ((-p.rhoi / p.rhow) * dice[0, 0])
+        dS_cur = np.zeros_like(load_inc)
+        if self.inner == "extrap" and len(self.hist) >= 2:
+            (h1, dt1), (h2, dt2) = self.hist[0], self.hist[1]
+            load_inc = extrap_guess(h1, h2, self.dt, dt1, dt2)
+            dS_cur = dS_prev + load_inc
+
+        kTE2, hTE2, kE2 = self.love.kTE[2], self.love.hTE[2], self.love.ke[2]
+        r2w2 = p.radius**2 * p.omega**2
+        dil = np.zeros((3, 3))
+        dm = np.zeros(3)
+        dlam = np.zeros(3, dtype=np.complex128)
+        dsl_rot = np.zeros(3, dtype=np.complex128)
+        rr_rot = np.zeros(3, dtype=np.complex128)
+        n_acc = 0
+        numiter = 1
+        while True:
+            if self.tpw:
+                # products of inertia from the degree-2 load (solver.py:912-921)
+                z20 = p.rhow * dS_cur[2, 0] + p.rhoi * dI[2, 0]
+                z21 = p.rhow * dS_cur[2, 1] + p.rhoi * dI[2, 1]
+                c4 = 8 * PI * p.radius**4
+                self.il[2, 2] = (-c4 / (3 * math.sqrt(5.0))) * z20.real
+                self.il[0, 2] = (-c4 / math.sqrt(30.0)) * z21.real
+                self.il[1, 2] = (c4 / math.sqrt(30.0)) * (-1j * z21).real
+                dil = self.il - old_il
+                # Liouville equations (solver.py:940-946)
+                for i in (0, 1):
+                    self.mm[i] = (1 / (1 - kTE2 / p.kf)) * (
+                        (1 / (p.moiC - p.moiA)) * ((1 + kE2) * self.il[i, 2] + sum_il[i, 2])
+                        + (1 / p.kf) * sum_m[i])
+                self.mm[2] = -(1.0 / p.moiC) * ((1 + kE2) * self.il[2, 2] + sum_il[2, 2])
+                dm = self.mm - old_m
+                m1, m2, m3 = self.mm
+                self.lam[0, 0] = (r2w2 / 3.0) * ((m1**2 + m2**2 + m3**2) + 2.0 * m3)
+                self.lam[2, 0] = (r2w2 / (6.0 * math.sqrt(5.0))) * (m1**2 + m2**2 - 2.0 * m3**2 - 4.0 * m3)
+                self.lam[2, 1] = -(r2w2 / math.sqrt(30.0)) * ((m1 - 1j * m2) * (1.0 + m3))
+                self.lam[2, 2] = (r2w2 / math.sqrt(5.0 * 24.0)) * (m2**2 - m1**2 + 2.0 * 1j * m1 * m2)
+                dlam = self.lam[2, 0:3] - lam_prev2
+                tot = lam_prev2 + dlam
+                dsl_rot = ((1 + kTE2 - hTE2) * tot + sum_tsl) / p.gacc
+                rr_rot = (hTE2 * tot + sum_trr) / p.gacc if self.calcRG else np.zeros(3, complex)
+
+            # ---- eq. B18 and the grid round trip
+            A = (p.rhoi * dI + p.rhow * dS_prev) + p.rhow * load_inc
+            sl_inc = A * self.ttlekhl + visc * self.ttl
+            if self.tpw:
+                sl_inc[2, 0:3] += dsl_rot
+            dslxy = sht.synthesis(sl_inc)
+            rO = dslxy * cstar
+            rO_lm = sht.analysis(rO)
+            conserv = ((1 / cstar_lm[0, 0])
+                       * (-(p.rhoi / p.rhow) * dI[0, 0] - rO_lm[0, 0] + tO_lm[0, 0])).real
+            old = load_inc
+            load_inc = ((-dS_prev + rO_lm) + cstar_lm * conserv) - tO_lm
+            dS_cur = dS_prev + load_inc
+            sl_tot = sl_inc.copy()
+            sl_tot[0, 0] += conserv
+            deltasl = sht.synthesis(sl_tot)
+
+            xi = convergence_ratio(np.abs(load_inc).sum(), np.abs(old).sum())
+            if xi > self.epsilon1:
+                topo = self.tinit - deltasl
+                self.cxy = (topo < 0.0).astype(np.float64)
+                if self.checkmarine and n_acc < GR_MAXACC:
+                    # grounded-ice re-guess on the topography iterate (solver.py:661-684)
+                    cand = self._grounded(ice_cur, topo)
+                    bnew = cand < EPS0
+                    self.gr_counts["reguess"] += 1
+                    if np.count_nonzero(bnew != (self.beta != 0.0)):
+                        self.icestarxy = cand
+                        self.beta = bnew.astype(np.float64)
+                        icestar_lm = sht.analysis(self.icestarxy)
+                        dice = icestar_lm - oldice
+                        dI = st.p_ice + dice
+                        self.gr_counts["ice_analyses"] += 1
+                        n_acc += 1
+                cstar = self.cxy * self.beta
+                cstar_lm = sht.analysis(cstar)
+                tO = self.tinit * (cstar - self.cstar0)
+                tO_lm = sht.analysis(tO)
+            else:
+                break
+            if numiter == MAX_INNER:
+                raise RuntimeError(f"inner loop failed to converge, xi={xi}")
+            numiter += 1
+
+        if self.inner == "extrap":
+            self.hist = [(load_inc.copy(), self.dt)] + self.hist[:1]
+
+        rrxy = None
+        if self.calcRG:
+            A = (p.rhoi * dI + p.rhow * dS_prev) + p.rhow * load_inc
+            rr_lm = A * self.ttlhe + rad * self.ttl
+            if self.tpw:
+                rr_lm[2, 0:3] += rr_rot
+            rrxy = sht.synthesis(rr_lm)
+
+        # ---- commit (row l = 0 of the load excluded, solver.py:1080-1095)
+        cur = p.rhoi * dice + p.rhow * load_inc
+        cur[0, :] = 0.0
+        if self.tpw:
+            st.commit(cur, dil, dm, dlam)
+        else:
+            st.commit(cur)
+        st.p_ice = dI.copy()
+        st.p_ocean = dS_cur.copy()
+        st.prev_ice = icestar_lm.copy()
+
+        topo = self.tinit - deltasl
+        self.cxy = (topo < 0.0).astype(np.float64)
+        self.topo_prev = topo.copy()
+        area = 4 * PI * p.radius**2
+        slc = deltasl * self.cxy
+        gmslc_ocn = (sht.analysis(slc)[0, 0] / sht.analysis(self.cxy)[0, 0]).real
+        clm00 = sht.analysis(self.cxy)[0, 0].real
+        cst = self.cxy * self.beta
+        cstar00 = sht.analysis(cst)[0, 0].real
+        gmslc_beta = (sht.analysis(slc * self.beta)[0, 0].real) / cstar00
+        self.ice_prev = ice_cur
+        return {
+            "iter": self.iter,
+            "numiter": numiter,
+            "tgrid": topo,
+            "ocean": self.cxy.copy(),
+            "beta": self.beta.copy(),
+            "R": rrxy,
+            "G": (deltasl + rrxy) if rrxy is not None else None,
+            "dS_converged": dS_cur,
+            "ocean_area": clm00 * area,
+            "oceanBeta_area": cstar00 * area,
+            "gmslc_ocn": gmslc_ocn,
+            "gmslc_ocnBeta": gmslc_beta,
+            "ice_volume": icestar_lm[0, 0].real * area,
+            "il": self.il.copy(),
+            "mm": self.mm.copy(),
+            "lambda": self.lam.copy(),
+        }
