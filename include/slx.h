@@ -1,0 +1,134 @@
+/* libslx: MI355X (gfx950) HIP kernels behind the SLEvolve sea-level time-step API.
+ *
+ * C-ABI only: plain pointers and sizes.  Pointers documented "device" are HIP device pointers
+ * (torch tensors' data_ptr() in the Python layer); "host" pointers are ordinary host memory.
+ * Every function returns 0 on success, a HIP error code (> 0) or a negative code for a bad
+ * argument; slx_last_error() returns a thread-local message.  `stream` is a hipStream_t
+ * (NULL = the default stream); launches are asynchronous unless stated otherwise.
+ *
+ * Spectral arrays on the device are packed, order-major triangles: coefficient (l, m),
+ * 0 <= m <= l <= lmax, at index moff(m) + (l - m), moff(m) = m (lmax+1) - m (m-1)/2,
+ * ncoef = (lmax+1)(lmax+2)/2, complex double (re, im).  Grids are row-major double
+ * [nlat][nlon], Gauss-Legendre rows north -> south.  Conventions are those of the reference's
+ * python/slevolve/sht.py:6-31 (legacy spharmt: orthonormal P~ without Condon-Shortley phase,
+ * the truncated 1.41421356 literal).
+ */
+#ifndef SLX_H
+#define SLX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLX_ABI_VERSION 1
+
+typedef struct slx_sht slx_sht;
+typedef struct slx_solver slx_solver;
+
+const char* slx_last_error(void);
+int slx_abi_version(void);
+
+/* ------------------------------------------------------------------ transform ------------
+ * Replaces the constructor of python/slevolve/sht.py:151-255 (SHT(nlat, nlon, lmax)) and of
+ * sht_ducc.py:70-132 (DuccSHT), reached through make_sht (sht_ducc.py:206-217).  Builds the
+ * Gauss-Legendre nodes/weights (gaulw, sht.py:65-101), the sectoral P~_mm and recurrence
+ * coefficients (legend, sht.py:105-144) and FFT twiddles, and work buffers for up to
+ * max_fields (1..4) fields per call.  nlat even, nlon a power of two, lmax <= nlon/2. */
+int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_sht** out);
+int slx_sht_destroy(slx_sht* h);
+/* dims (host, 5 ints or NULL) = {nlat, nlon, lmax, ncoef, max_fields}; gaulats/weights (host,
+ * nlat doubles each, or NULL) = the nodes mu_j north -> south and weights (SHT.gaulats). */
+int slx_sht_info(const slx_sht* h, int* dims, double* gaulats, double* weights);
+/* Replaces SHT.analysis (sht.py:274-302) / DuccSHT.analysis (sht_ducc.py:134-154) for nf fields
+ * at once: grid (device) nf x [nlat][nlon] -> coef (device) nf x [ncoef] packed. */
+int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* coef, void* stream);
+/* Replaces SHT.synthesis (sht.py:305-343) / DuccSHT.synthesis (sht_ducc.py:156-187): coef
+ * (device) nf x [ncoef] -> grid (device) nf x [nlat][nlon].  Im of the m = 0 (and Nyquist)
+ * coefficients is ignored, as irfft does. */
+int slx_sht_synthesis(slx_sht* h, const void* coef, int nf, double* grid, void* stream);
+/* Layout conversion between packed triangles and the dense [l][m] (lmax+1)^2 complex arrays
+ * the reference's Python API exchanges (zeros above the diagonal). */
+int slx_sht_tri_to_dense(const slx_sht* h, const void* tri, void* dense, int nf, void* stream);
+int slx_sht_dense_to_tri(const slx_sht* h, const void* dense, void* tri, int nf, void* stream);
+
+/* ---------------------------------------------------------------- modal state ------------
+ * Replaces ExactMemory.advance_and_read (python/slevolve/solver.py:386-422),
+ * kernels.advance_and_read (kernels.py:28-67) and Fortran state_advance_and_read
+ * (modal_state_mod.f90:286-336): for every packed (l, m) and mode k < nmod[l]
+ *   R_k <- decay[k][l] R_k + omd[k][l] P ;  visc = sum_k a_sl[k][l] R_k ;  rad = sum_k a_rr[k][l] R_k
+ * in k order, without FMA contraction (bit-identical to the NumPy statements).
+ * R (device) [kmax][ncoef] complex; P, visc, rad (device) [ncoef] complex; nmod (device, int)
+ * [lmax+1]; decay, omd, a_sl, a_rr (device) [kmax][lmax+1]. */
+int slx_modal_advance_read(const slx_sht* h, int kmax, const int* nmod, const double* decay,
+                           const double* omd, const double* a_sl, const double* a_rr, void* R,
+                           const void* P, void* visc, void* rad, void* stream);
+
+/* ------------------------------------------------------------------- the step ------------
+ * Replaces SeaLevelSolver (solver.py:521-1144) and OptSeaLevelSolver with inner = "off" or
+ * "extrap" (solver_opt.py:369-979), default step placement, axial rotation "off".  All state
+ * is device-resident; the inner fixed point runs as predicated launches that stop on a
+ * device convergence flag. */
+typedef struct slx_solver_params {
+  int lmax, kmax, nmod2;
+  int tpw, checkmarine, calcRG, extrap; /* booleans */
+  double epsilon1;
+  double radius, mass, rhoi, rhow, gacc, omega, moiA, moiC, kf; /* Planet (solver.py:98-110) */
+  double kTE2, hTE2, kE2;                                      /* degree-2 Love numbers */
+  const int* nmod;                                  /* host [lmax+1] modes per degree */
+  const double* s;                                  /* host [kmax][lmax+1] 1/kyr, 0 = no mode */
+  const double* a_sl;                               /* host [kmax][lmax+1] (r' - r)/s */
+  const double* a_rr;                               /* host [kmax][lmax+1] r/s */
+  const double* a_il, *a_m, *a_tsl, *a_trr;         /* host [kmax] degree-2 rotational weights */
+  const double* ttl, *ttlekhl, *ttlhe;              /* host [lmax+1] T_l, T_l E_l, T_l h_l */
+} slx_solver_params;
+
+typedef struct slx_status {
+  int numiter, done, failed, nacc, reguess, ice_analyses, step;
+  int finished;
+  double xi, conserv;
+  double il[9], mm[3], lam[18]; /* lam: 3x3 complex, row-major, (re, im) */
+  double ocean_area, oceanBeta_area, gmslc_ocn, gmslc_ocnBeta, ice_volume;
+  double ocean_area0, oceanBeta_area0, ice_volume0;
+} slx_status;
+
+typedef struct slx_fields { /* device pointers owned by the solver */
+  double* icestar; /* grounded ice I* */
+  double* tinit, *cstar0, *cxy0, *beta0;
+  void* dS_converged;  /* packed [ncoef] complex */
+  void* p_load, *p_ice, *p_ocean, *prev_ice, *load_inc;
+  void* modal_R;       /* packed [kmax][ncoef] complex */
+} slx_fields;
+
+typedef struct slx_outputs { /* caller-owned device grids [nlat][nlon] the solver writes in place */
+  double* tgrid;   /* topography tinit - deltasl after the step ("tgrid") */
+  double* ocean;   /* ocean function C, 0/1 ("ocean") */
+  double* beta;    /* grounded-ice complement beta, 0/1 ("beta") */
+  double* R;       /* radial displacement ("R") */
+  double* G;       /* deltasl + R ("G") */
+} slx_outputs;
+
+int slx_solver_create(slx_sht* h, const slx_solver_params* p, const slx_outputs* o, slx_solver** out);
+int slx_solver_destroy(slx_solver* s);
+/* Step-0 set-up (solver.py:554-605): tinit, C0, marine-checked I*0, beta0, C*0 and the three
+ * degree-0 areas.  topo0, ice0: device grids. */
+int slx_solver_init(slx_solver* s, const double* topo0, const double* ice0, void* stream);
+/* Step length (years) of the following steps: decay = exp(-s dt / 1000) (solver.py:331-346). */
+int slx_solver_set_dt(slx_solver* s, double dt_years);
+/* Start a step with the ice grid ice_cur (device): modal advance, grounded-ice projection,
+ * ice / ocean analyses and the first guess (eustatic, or extrapolated with the step lengths
+ * dt0 = this step, dt1, dt2 = the two previous ones when the solver extrapolates). */
+int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, double dt1, double dt2, void* stream);
+/* Enqueue n inner iterations; those after convergence return at once on the device. */
+int slx_step_iterate(slx_solver* s, int n, void* stream);
+/* Radial displacement, commit, topography / ocean function and degree-0 diagnostics.  These
+ * launches run only once the inner loop has converged and only once per step, so the host may
+ * enqueue them right after a batch of iterations and test status.finished afterwards. */
+int slx_step_finish(slx_solver* s, void* stream);
+/* Synchronous: copies the status block to *out (host) after the stream's pending work. */
+int slx_solver_status(slx_solver* s, slx_status* out, void* stream);
+int slx_solver_fields(slx_solver* s, slx_fields* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

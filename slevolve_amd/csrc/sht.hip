@@ -1,0 +1,235 @@
+// Gauss-Legendre transform handle and its C-ABI (declared in include/slx.h).
+#include <cmath>
+#include <cstring>
+
+#include "sht_launch.h"
+#include "slx.h"
+
+namespace {
+thread_local std::string g_err;
+}
+
+void slx_set_error(const std::string& msg) { g_err = msg; }
+extern "C" const char* slx_last_error(void) { return g_err.c_str(); }
+extern "C" int slx_abi_version(void) { return SLX_ABI_VERSION; }
+
+
+using namespace slx;
+
+// ------------------------------------------------------------------------------------ tables
+// Gauss-Legendre nodes/weights: Newton on P_nlat from cos(pi (j - 1/4)/(nlat + 1/2)), at most 10
+// sweeps, |dz| < 1e-15, node and derivative kept at first convergence (python/slevolve/sht.py:65-101).
+static int gauss_legendre(int nlat, std::vector<double>& z_out, std::vector<double>& w) {
+  const double pi = 4.0 * std::atan(1.0);
+  z_out.assign(nlat, 0.0);
+  w.assign(nlat, 0.0);
+  for (int i = 0; i < nlat; ++i) {
+    double z = std::cos(pi * ((double)(i + 1) - 0.25) / ((double)nlat + 0.5));
+    bool done = false;
+    for (int it = 0; it < 10 && !done; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int n = 1; n <= nlat; ++n) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * n - 1.0) * z * p2 - (n - 1.0) * p3) / n;
+      }
+      double pp = nlat * (z * p1 - p2) / (z * z - 1.0);
+      double z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1e-15) {
+        done = true;
+        z_out[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * pp * pp);
+      }
+    }
+    if (!done) return -1;
+  }
+  return 0;
+}
+
+extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_sht** out) {
+  SLX_CHECK(out != nullptr, "slx_sht_create: null handle pointer");
+  SLX_CHECK(nlat >= 2 && nlat % 2 == 0, "nlat must be even");
+  SLX_CHECK(nlon >= 8 && (nlon & (nlon - 1)) == 0, "nlon must be a power of two >= 8");
+  SLX_CHECK(lmax >= 0 && lmax <= nlon / 2, "lmax must be <= nlon/2");
+  SLX_CHECK(lmax <= nlat * 2, "lmax too large for the grid");
+  SLX_CHECK(max_fields >= 1 && max_fields <= 3, "max_fields must be 1..3");
+  SLX_CHECK(nlat / 2 <= 512, "nlat must be <= 1024");
+  auto* h = new slx_sht();
+  h->nlat = nlat;
+  h->nlon = nlon;
+  h->lmax = lmax;
+  h->nh = nlat / 2;
+  h->nm = lmax + 1;
+  h->ncoef = (lmax + 1) * (lmax + 2) / 2;
+  h->max_fields = max_fields;
+  h->ana_waves = 2;
+  int lg = 0;
+  while ((1 << lg) < nlon) ++lg;
+  h->log2n = lg;
+  if (gauss_legendre(nlat, h->h_x, h->h_w) != 0) {
+    delete h;
+    slx_set_error("Gauss-Legendre nodes did not converge");
+    return -22;
+  }
+  const int nh = h->nh, nm = h->nm, L = lmax;
+  std::vector<double> x(nh), wn(nh), ws(nh), pmm((size_t)nm * nh), s2m3(nm);
+  for (int j = 0; j < nh; ++j) {
+    x[j] = h->h_x[j];
+    wn[j] = h->h_w[j] / nlon;
+    ws[j] = h->h_w[nlat - 1 - j] / nlon;
+  }
+  // sectoral start P~_mm = sqrt(0.5 prod_m), prod_m = prod (1 - x^2) A/B (sht.py:118-126)
+  for (int j = 0; j < nh; ++j) {
+    double sinsq = 1.0 - x[j] * x[j], A = 1.0, B = 0.0, prod = 1.0;
+    for (int m = 0; m < nm; ++m) {
+      if (m != 0) {
+        A += 2.0;
+        B += 2.0;
+        prod = prod * sinsq * A / B;
+      }
+      pmm[(size_t)m * nh + j] = std::sqrt(0.5 * prod);
+    }
+  }
+  h->h_moff.resize(nm + 1);
+  for (int m = 0; m <= nm; ++m) h->h_moff[m] = slx_moff(m, L);
+  std::vector<double2> rec(h->ncoef);
+  std::vector<int> l_of(h->ncoef), jb(nm, 0);
+  for (int m = 0; m < nm; ++m) {
+    s2m3[m] = std::sqrt((double)(2 * m + 3));
+    for (int l = m; l <= L; ++l) {
+      const int idx = h->h_moff[m] + (l - m);
+      l_of[idx] = l;
+      if (l >= m + 2) {
+        double e = std::sqrt((double)(l * l - m * m) / (double)(4 * l * l - 1));
+        double ep = std::sqrt((double)((l - 1) * (l - 1) - m * m) / (double)(4 * (l - 1) * (l - 1) - 1));
+        rec[idx] = make_double2(ep, 1.0 / e);
+      } else {
+        rec[idx] = make_double2(0.0, 0.0);
+      }
+    }
+  }
+  std::vector<double2> tw(nlon);
+  const double pi2 = 8.0 * std::atan(1.0);
+  for (int t = 0; t < nlon; ++t) {
+    double a = -pi2 * (double)t / (double)nlon;
+    tw[t] = make_double2(std::cos(a), std::sin(a));
+  }
+  size_t wsz = (size_t)max_fields * nm * nh * sizeof(double2);
+#define ALLOC_COPY(ptr, vec)                                                                   \
+  SLX_HIP(hipMalloc(&ptr, vec.size() * sizeof(vec[0])));                                       \
+  SLX_HIP(hipMemcpy(ptr, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice));
+  ALLOC_COPY(h->x, x);
+  ALLOC_COPY(h->wn, wn);
+  ALLOC_COPY(h->ws, ws);
+  ALLOC_COPY(h->pmm, pmm);
+  ALLOC_COPY(h->rec, rec);
+  ALLOC_COPY(h->s2m3, s2m3);
+  ALLOC_COPY(h->moff, h->h_moff);
+  ALLOC_COPY(h->l_of, l_of);
+  ALLOC_COPY(h->jb, jb);
+  ALLOC_COPY(h->tw, tw);
+#undef ALLOC_COPY
+  SLX_HIP(hipMalloc(&h->WS, wsz));
+  SLX_HIP(hipMalloc(&h->WD, wsz));
+  SLX_HIP(hipMalloc(&h->FN, wsz));
+  SLX_HIP(hipMalloc(&h->FSo, wsz));
+  *out = h;
+  return 0;
+}
+
+extern "C" int slx_sht_destroy(slx_sht* h) {
+  if (!h) return 0;
+  void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->tw,
+                  h->WS, h->WD, h->FN, h->FSo};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete h;
+  return 0;
+}
+
+extern "C" int slx_sht_info(const slx_sht* h, int* dims, double* gaulats, double* weights) {
+  SLX_CHECK(h != nullptr, "null handle");
+  if (dims) {
+    dims[0] = h->nlat;
+    dims[1] = h->nlon;
+    dims[2] = h->lmax;
+    dims[3] = h->ncoef;
+    dims[4] = h->max_fields;
+  }
+  if (gaulats) std::memcpy(gaulats, h->h_x.data(), h->nlat * sizeof(double));
+  if (weights) std::memcpy(weights, h->h_w.data(), h->nlat * sizeof(double));
+  return 0;
+}
+
+
+extern "C" int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* coef_v, void* stream) {
+  double2* coef = static_cast<double2*>(coef_v);
+  SLX_CHECK(h && grid && coef, "slx_sht_analysis: null argument");
+  SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_analysis: nf out of range");
+  hipStream_t s = (hipStream_t)stream;
+  const long gstride = (long)h->nlat * h->nlon;
+  int rc = launch_fft_ana<256>(h, LoadPlain{grid, gstride}, nf, Pred{}, s);
+  if (rc) return rc;
+  switch (nf) {
+    case 1: return launch_leg_ana<1>(h, h->WS, h->WD, OutPtrs<1>{{coef}}, Pred{}, s);
+    case 2: return launch_leg_ana<2>(h, h->WS, h->WD, OutPtrs<2>{{coef, coef + h->ncoef}}, Pred{}, s);
+    default:
+      return launch_leg_ana<3>(h, h->WS, h->WD, OutPtrs<3>{{coef, coef + h->ncoef, coef + 2 * (long)h->ncoef}},
+                               Pred{}, s);
+  }
+}
+
+extern "C" int slx_sht_synthesis(slx_sht* h, const void* coef_v, int nf, double* grid, void* stream) {
+  const double2* coef = static_cast<const double2*>(coef_v);
+  SLX_CHECK(h && grid && coef, "slx_sht_synthesis: null argument");
+  SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_synthesis: nf out of range");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  CoefPlain cp{coef, h->ncoef};
+  switch (nf) {
+    case 1: rc = launch_leg_syn<1>(h, cp, Pred{}, s); break;
+    case 2: rc = launch_leg_syn<2>(h, cp, Pred{}, s); break;
+    default: rc = launch_leg_syn<3>(h, cp, Pred{}, s); break;
+  }
+  if (rc) return rc;
+  return launch_fft_syn<256>(h, StorePlain{grid, (long)h->nlat * h->nlon}, nf, Pred{}, s);
+}
+
+// packed <-> dense [l][m] (L+1)x(L+1) complex, zeros above the diagonal
+__global__ void k_tri_to_dense(const double2* __restrict__ tri, double2* __restrict__ dense, int L,
+                               const int* __restrict__ moff) {
+  const int l = blockIdx.x;
+  for (int m = threadIdx.x; m <= L; m += blockDim.x)
+    dense[(long)l * (L + 1) + m] = (m <= l) ? tri[moff[m] + (l - m)] : make_double2(0.0, 0.0);
+}
+__global__ void k_dense_to_tri(const double2* __restrict__ dense, double2* __restrict__ tri, int L,
+                               const int* __restrict__ moff) {
+  const int l = blockIdx.x;
+  for (int m = threadIdx.x; m <= l; m += blockDim.x) tri[moff[m] + (l - m)] = dense[(long)l * (L + 1) + m];
+}
+
+extern "C" int slx_sht_tri_to_dense(const slx_sht* h, const void* tri_v, void* dense_v, int nf, void* stream) {
+  const double2* tri = static_cast<const double2*>(tri_v);
+  double2* dense = static_cast<double2*>(dense_v);
+  SLX_CHECK(h && tri && dense && nf >= 1, "slx_sht_tri_to_dense: bad argument");
+  const int L = h->lmax;
+  for (int f = 0; f < nf; ++f) {
+    k_tri_to_dense<<<L + 1, 256, 0, (hipStream_t)stream>>>(tri + (long)f * h->ncoef,
+                                                           dense + (long)f * (L + 1) * (L + 1), L, h->moff);
+    SLX_LAUNCH_CHECK();
+  }
+  return 0;
+}
+extern "C" int slx_sht_dense_to_tri(const slx_sht* h, const void* dense_v, void* tri_v, int nf, void* stream) {
+  const double2* dense = static_cast<const double2*>(dense_v);
+  double2* tri = static_cast<double2*>(tri_v);
+  SLX_CHECK(h && tri && dense && nf >= 1, "slx_sht_dense_to_tri: bad argument");
+  const int L = h->lmax;
+  for (int f = 0; f < nf; ++f) {
+    k_dense_to_tri<<<L + 1, 256, 0, (hipStream_t)stream>>>(dense + (long)f * (L + 1) * (L + 1),
+                                                           tri + (long)f * h->ncoef, L, h->moff);
+    SLX_LAUNCH_CHECK();
+  }
+  return 0;
+}

@@ -1,0 +1,92 @@
+// Host-side launch helpers for the transform kernels (shared by the SHT and solver units).
+#pragma once
+#include "sht_kernels.h"
+
+namespace slx {
+
+inline ShtDev dev_of(const slx_sht* h) {
+  ShtDev d;
+  d.nlat = h->nlat;
+  d.nlon = h->nlon;
+  d.lmax = h->lmax;
+  d.nh = h->nh;
+  d.nm = h->nm;
+  d.log2n = h->log2n;
+  d.x = h->x;
+  d.wn = h->wn;
+  d.ws = h->ws;
+  d.pmm = h->pmm;
+  d.rec = h->rec;
+  d.s2m3 = h->s2m3;
+  d.moff = h->moff;
+  d.jb = h->jb;
+  d.tw = h->tw;
+  return d;
+}
+
+struct LoadPlain {
+  const double* g;
+  long fstride;
+  __device__ double operator()(int f, long i) const { return g[f * fstride + i]; }
+};
+struct CoefPlain {
+  const double2* c;
+  long fstride;
+  __device__ double2 operator()(int f, int idx, int, int) const { return c[f * fstride + idx]; }
+};
+struct StorePlain {
+  double* g;
+  long fstride;
+  __device__ void operator()(int f, long i, double v) const { g[f * fstride + i] = v; }
+};
+
+// Launch helpers shared with the solver ------------------------------------------------------
+inline int ana_rings_per_lane(const slx_sht* h) {
+  int r = (h->nh + 64 * h->ana_waves - 1) / (64 * h->ana_waves);
+  return r <= 1 ? 1 : r <= 2 ? 2 : 4;
+}
+
+template <int NF>
+int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out,
+                   Pred pr, hipStream_t s) {
+  ShtDev d = dev_of(h);
+  const int R = ana_rings_per_lane(h);
+  dim3 grid(h->nm);
+  switch (R) {
+    case 1: k_leg_ana<NF, 1, 2><<<grid, 128, 0, s>>>(d, WS, WD, out, pr); break;
+    case 2: k_leg_ana<NF, 2, 2><<<grid, 128, 0, s>>>(d, WS, WD, out, pr); break;
+    default: k_leg_ana<NF, 4, 2><<<grid, 128, 0, s>>>(d, WS, WD, out, pr); break;
+  }
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace slx
+
+namespace slx {
+template <int NT, class Loader>
+int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {
+  dim3 grid(h->nh, nf);
+  size_t lds = 2 * (size_t)h->nlon * sizeof(double2);
+  k_fft_ana<NT, Loader><<<grid, NT, lds, s>>>(dev_of(h), ld, h->WS, h->WD, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+template <int NT, class Store>
+int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
+  dim3 grid(h->nh, nf);
+  size_t lds = 2 * (size_t)h->nlon * sizeof(double2);
+  k_fft_syn<NT, Store><<<grid, NT, lds, s>>>(dev_of(h), h->FN, h->FSo, st, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+template <int NF, class Coef>
+int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s) {
+  constexpr int NT = 128;
+  dim3 grid(h->nm, (h->nh + NT - 1) / NT);
+  size_t lds = (size_t)NF * (h->lmax + 1) * sizeof(double2);
+  k_leg_syn<NF, NT, Coef><<<grid, NT, lds, s>>>(dev_of(h), cf, h->FN, h->FSo, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+}  // namespace slx

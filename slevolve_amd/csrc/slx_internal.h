@@ -1,0 +1,61 @@
+// Internal definitions shared by the libslx translation units (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#define SLX_SQRT2_LEGACY 1.41421356
+
+// ------------------------------------------------------------------ error plumbing
+void slx_set_error(const std::string& msg);
+#define SLX_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      slx_set_error(std::string(#call) + ": " + hipGetErrorString(e_));                      \
+      return (int)e_ == 0 ? -1 : (int)e_;                                                    \
+    }                                                                                        \
+  } while (0)
+#define SLX_CHECK(cond, msg)                                                                 \
+  do {                                                                                       \
+    if (!(cond)) {                                                                           \
+      slx_set_error(msg);                                                                    \
+      return -22;                                                                            \
+    }                                                                                        \
+  } while (0)
+#define SLX_LAUNCH_CHECK()                                                                   \
+  do {                                                                                       \
+    hipError_t e_ = hipGetLastError();                                                       \
+    if (e_ != hipSuccess) {                                                                  \
+      slx_set_error(std::string("kernel launch: ") + hipGetErrorString(e_));                 \
+      return (int)e_;                                                                        \
+    }                                                                                        \
+  } while (0)
+
+// ------------------------------------------------------------------ the transform handle
+// Spectral arrays on the device are "packed": order-major triangles, coefficient (l, m) at
+// moff[m] + (l - m), moff[m] = m (L+1) - m (m-1)/2, complex as double2.
+struct slx_sht {
+  int nlat = 0, nlon = 0, lmax = 0, nh = 0, nm = 0, ncoef = 0, log2n = 0, max_fields = 0;
+  int ana_waves = 1;  // waves per order in the analysis contraction
+  double* x = nullptr;    // [nh] mu_j of the northern rings (north -> equator)
+  double* wn = nullptr;   // [nh] w_j / nlon
+  double* ws = nullptr;   // [nh] w_{nlat-1-j} / nlon
+  double* pmm = nullptr;  // [nm][nh] sectoral P~_mm(mu_j)
+  double2* rec = nullptr; // [ncoef] (eps_{l-1,m}, 1/eps_{l,m}) at (l, m), l >= m+2
+  double* s2m3 = nullptr; // [nm] sqrt(2m+3)
+  int* moff = nullptr;    // [nm+1]
+  int* l_of = nullptr;    // [ncoef] degree of each packed index
+  int* jb = nullptr;      // [nm] first ring (from the pole) with non-negligible P~_lm, multiple of 64
+  double2* tw = nullptr;  // [nlon] exp(-2 pi i t / nlon)
+  double2* WS = nullptr;  // [max_fields][nm][nh] analysis: folded even parts
+  double2* WD = nullptr;  // [max_fields][nm][nh] analysis: folded odd parts
+  double2* FN = nullptr;  // [max_fields][nm][nh] synthesis: northern Fourier coefficients
+  double2* FSo = nullptr; // [max_fields][nm][nh] synthesis: southern Fourier coefficients
+  std::vector<double> h_x, h_w;  // host copies: all nlat nodes / weights (north -> south)
+  std::vector<int> h_moff;
+};
+
+static inline int slx_moff(int m, int L) { return m * (L + 1) - m * (m - 1) / 2; }

@@ -1,0 +1,907 @@
+// Device-resident sea-level time step (FP64).  Restates python/slevolve/solver.py:521-1144 and
+// the extrapolated start of solver_opt.py:148-165,504-527 with the grid and spectral algebra fused
+// into the transform passes:
+//   * rO - tO is analysed as one field (its analysis is all the update uses, solver.py:1002-1015);
+//   * the ocean function C* is analysed in the same sweep as rO - tO of the next iteration;
+//   * deltasl = dslxy + conserv K00 (one synthesis less per iteration, solver_opt.py:28-32);
+//   * eq. B18 is assembled while the synthesis stages its coefficients.
+// The inner iterations are launched ahead and predicated on a device convergence flag.
+#include <cmath>
+#include <cstring>
+
+#include "sht_launch.h"
+#include "slx.h"
+
+namespace slx {
+int launch_modal(const slx_sht* h, int kmax, const int* nmod, const double* decay, const double* omd,
+                 const double* a_sl, const double* a_rr, double2* R, const double2* P, double2* visc,
+                 double2* rad, hipStream_t s);
+}
+
+using namespace slx;
+
+static constexpr double EPS0 = 2.220446049250313e-16;
+static constexpr int GR_MAXACC = 3;
+static constexpr int MAX_INNER = 9999;
+static constexpr double PI_LEGACY = 3.1415926535898;
+
+// Scalar state of a run.  The status block comes first so it can be copied out as is.
+struct DevState {
+  slx_status st;
+  int ice_changed, nflip, counter, pad0;
+  double sa, so;
+  double p_il[9], p_m[3], p_lam[6];
+  double dil[9], dm[3], dlam[6];
+  double sum_il[9], sum_m[3], sum_tsl[6], sum_trr[6];
+  double dsl_rot[6], rr_rot[6];
+};
+
+// Constants evaluated on the host once, in the reference's operation order.
+struct Consts {
+  double rhoi, rhow, gacc, K00, eps1, area;  // area = 4 PI a^2
+  double cI33, cI13, cI23;                   // inertia factors (solver.py:915-920)
+  double alpha, inv_CA, one_kE2, inv_kf, inv_C;
+  double K0, K20, K21, K22;                  // Lambda factors (solver.py:955-959)
+  double ekhTE, hTE2;
+  int L, nl, ncoef, npts, nlat, nlon, kmax, nmod2, tpw, checkmarine, calcRG;
+  int idx20, idx21, idx22;
+};
+
+struct slx_solver {
+  slx_sht* h = nullptr;
+  Consts c;
+  int extrap = 0;
+  int nhist = 0;         // converged increments stored for the extrapolated start
+  int step = 0;
+  int iters_enqueued = 0;
+  // device arrays --------------------------------------------------------------------------
+  DevState* st = nullptr;
+  double* partial = nullptr;  // block partial sums
+  double* rows = nullptr;     // [4][nlat] row sums
+  double* w00 = nullptr;      // [nlat] degree-0 quadrature weights
+  int* nmod = nullptr;
+  double *s = nullptr, *decay = nullptr, *omd = nullptr, *a_sl = nullptr, *a_rr = nullptr;
+  double *a2 = nullptr;       // [4][kmax] a_il, a_m, a_tsl, a_trr
+  double *r2 = nullptr;       // [kmax][18] degree-2 states r_il(9), r_m(3), r_lambda(6)
+  double *ttl = nullptr, *ttlekhl = nullptr, *ttlhe = nullptr;
+  // grids
+  double *tinit, *cxy0, *beta0, *cstar0, *cxy, *beta, *cstar, *tO, *icestar, *dsl, *rr, *G, *topo, *topo_prev,
+      *ice_prev, *tmp;
+  // spectra (packed)
+  double2 *R, *p_load, *p_ice, *p_ocean, *prev_ice, *icestar_lm, *oldice_lm, *dice, *dI, *cstar_lm, *rt_lm,
+      *load_inc, *dS_cur, *visc, *rad, *h1, *h2, *scratch_lm;
+  std::vector<double> h_s;  // host copy of s [kmax][nl]
+  slx_status* pinned = nullptr;
+  double dt = -1.0;
+  const double* ice_cur = nullptr;  // ice grid of the step in progress (device)
+  int closed = 1;                   // host bookkeeping of the last step done
+};
+
+// ================================================================================== kernels
+namespace {
+
+struct Pts {  // all device pointers a kernel may need, passed by value
+  DevState* st;
+  double* partial;
+  const double *tinit, *cxy0, *beta0, *cstar0;
+  double *cxy, *beta, *cstar, *tO, *icestar, *dsl, *rr, *G, *topo, *topo_prev, *ice_prev, *tmp, *rows;
+  const double* w00;
+  const double* ice_cur;
+  double2 *p_load, *p_ice, *p_ocean, *prev_ice, *icestar_lm, *oldice, *dice, *dI, *cstar_lm, *rt_lm, *load_inc,
+      *dS_cur, *visc, *rad, *h1, *h2;
+  const double *ttl, *ttlekhl, *ttlhe, *decay, *a2;
+  double* r2;
+  const int* l_of;
+};
+
+__device__ __forceinline__ double keep_factor(double ice, double topo, double rhow, double rhoi) {
+  // (topo > 0) | (ice > |topo| rho_w / rho_i), as a 0/1 multiplier (solver.py:655-660)
+  return (topo > 0.0 || ice > fabs(topo) * rhow / rhoi) ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ double cabs_d(double2 z) { return hypot(z.x, z.y); }
+
+// Block sum of v (blockDim 256) -> lane 0; fixed order.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__device__ __forceinline__ double load_relaxed(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Last-block-done: every block posts its partial; the block that arrives last returns true
+// (and has all partials visible).  counter is reset by that block.
+__device__ bool post_partial(double v, double* partial, int* counter) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partial[blockIdx.x], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    int t = atomicAdd(counter, 1);
+    last = (t == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ double sum_partials(const double* partial, int n, double* sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += load_relaxed(&partial[i]);
+  return block_sum(v, sh);
+}
+
+// ---------------------------------------------------------------------- set-up (step 0)
+__global__ void k_init_grid(Pts p, Consts c, const double* topo0, const double* ice0) {
+  const long n = c.npts;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double t = topo0[i], ice = ice0[i];
+    double cx = t < 0.0 ? 1.0 : 0.0;
+    double g0 = c.checkmarine ? ice * keep_factor(ice, t, c.rhow, c.rhoi) : ice;
+    double b0 = g0 < EPS0 ? 1.0 : 0.0;
+    const_cast<double*>(p.tinit)[i] = t;
+    const_cast<double*>(p.cxy0)[i] = cx;
+    const_cast<double*>(p.beta0)[i] = b0;
+    const_cast<double*>(p.cstar0)[i] = cx * b0;
+    p.cxy[i] = cx;
+    p.topo_prev[i] = t;
+    p.topo[i] = t;
+    p.ice_prev[i] = ice;
+    p.tmp[i] = g0;
+    p.beta[i] = b0;
+    p.icestar[i] = g0;
+  }
+}
+
+// ---------------------------------------------------------------------- step begin
+// degree-2 rotational states (solver.py:407-421) and the per-step resets
+__global__ void k_begin(Pts p, Consts c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DevState* s = p.st;
+  double sil[9] = {0}, sm[3] = {0}, stsl[6] = {0}, strr[6] = {0};
+  if (c.L >= 2) {
+    for (int k = 0; k < c.nmod2; ++k) {
+      const double d = p.decay[k * c.nl + 2], o = 1.0 - d;
+      double* r = p.r2 + 18 * k;
+      const double ail = p.a2[k], am = p.a2[c.kmax + k], atsl = p.a2[2 * c.kmax + k], atrr = p.a2[3 * c.kmax + k];
+      for (int q = 0; q < 9; ++q) {
+        r[q] = d * r[q] + o * s->p_il[q];
+        sil[q] += ail * r[q];
+      }
+      for (int q = 0; q < 3; ++q) {
+        r[9 + q] = d * r[9 + q] + o * s->p_m[q];
+        sm[q] += am * r[9 + q];
+      }
+      for (int q = 0; q < 6; ++q) {
+        r[12 + q] = d * r[12 + q] + o * s->p_lam[q];
+        stsl[q] += atsl * r[12 + q];
+        strr[q] += atrr * r[12 + q];
+      }
+    }
+  }
+  for (int q = 0; q < 9; ++q) s->sum_il[q] = sil[q];
+  for (int q = 0; q < 3; ++q) s->sum_m[q] = sm[q];
+  for (int q = 0; q < 6; ++q) {
+    s->sum_tsl[q] = stsl[q];
+    s->sum_trr[q] = strr[q];
+    s->dsl_rot[q] = 0.0;
+    s->rr_rot[q] = 0.0;
+  }
+  for (int q = 0; q < 9; ++q) s->dil[q] = 0.0;
+  for (int q = 0; q < 3; ++q) s->dm[q] = 0.0;
+  for (int q = 0; q < 6; ++q) s->dlam[q] = 0.0;
+  s->st.numiter = 0;
+  s->st.done = 0;
+  s->st.failed = 0;
+  s->st.nacc = 0;
+  s->st.finished = 0;
+  s->st.step += 1;
+  s->ice_changed = 0;
+  s->nflip = 0;
+  s->counter = 0;
+}
+
+// grounded ice on the previous converged topography, beta, C*, tO (solver.py:853-887)
+__global__ void k_ice_grid(Pts p, Consts c) {
+  const long n = c.npts;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double ice = p.ice_cur[i];
+    double ist = c.checkmarine ? ice * keep_factor(ice, p.topo_prev[i], c.rhow, c.rhoi) : ice;
+    double b = ist < EPS0 ? 1.0 : 0.0;
+    double cs = p.cxy[i] * b;
+    p.icestar[i] = ist;
+    p.beta[i] = b;
+    p.cstar[i] = cs;
+    p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
+  }
+}
+
+// ice increments and the first guess of the ocean-load increment (solver.py:857-892;
+// solver_opt.py:504-527); so = sum |load_inc|
+struct GuessArgs {
+  int extrap_ready, uniform;
+  double c1, c2;
+};
+__global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
+  __shared__ double sh[256];
+  const int n = c.ncoef;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0;
+  if (i < n) {
+    const double2 old = p.oldice[i], ist = p.icestar_lm[i];
+    const double2 d = make_double2(ist.x - old.x, ist.y - old.y);
+    const double2 pi = p.p_ice[i];
+    p.dice[i] = d;
+    p.dI[i] = make_double2(pi.x + d.x, pi.y + d.y);
+    double2 ld;
+    if (g.extrap_ready) {
+      const double2 a1 = p.h1[i], a2 = p.h2[i];
+      if (g.uniform)
+        ld = make_double2(a1.x * 2.0 - a2.x, a1.y * 2.0 - a2.y);
+      else
+        ld = make_double2(a1.x * g.c1 - g.c2 * a2.x, a1.y * g.c1 - g.c2 * a2.y);
+      const double2 sp = p.p_ocean[i];
+      p.dS_cur[i] = make_double2(sp.x + ld.x, sp.y + ld.y);
+    } else {
+      // (cstar_lm / cstar_lm[0,0]) * ((-rho_i/rho_w) dice[0,0]); cstar_lm[0,0] and dice[0,0] are real
+      const double2 c00 = p.cstar_lm[0];
+      const double2 d00 = make_double2(p.icestar_lm[0].x - p.oldice[0].x, p.icestar_lm[0].y - p.oldice[0].y);
+      const double scl = 1.0 / c00.x;
+      const double2 q = make_double2(p.cstar_lm[i].x * scl, p.cstar_lm[i].y * scl);
+      const double f = -c.rhoi / c.rhow;
+      const double2 e = make_double2(f * d00.x, f * d00.y);
+      ld = make_double2(q.x * e.x - q.y * e.y, q.x * e.y + q.y * e.x);
+      p.dS_cur[i] = make_double2(0.0, 0.0);
+    }
+    p.load_inc[i] = ld;
+    a = cabs_d(ld);
+  }
+  const double bs = block_sum(a, sh);
+  if (post_partial(bs, p.partial, &p.st->counter)) {
+    const double tot = sum_partials(p.partial, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      p.st->so = tot;
+      p.st->counter = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------- inner iteration
+// rotation block (solver.py:910-975): products of inertia from the degree-2 load, Liouville
+// equations, centrifugal potential Lambda and the rotational sea-level / radial terms
+__global__ void k_rotation(Pts p, Consts c) {
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done) return;
+  if (threadIdx.x != 0 || blockIdx.x != 0 || !c.tpw) return;
+  const double2 dS20 = p.dS_cur[c.idx20], dS21 = p.dS_cur[c.idx21];
+  const double2 dI20 = p.dI[c.idx20], dI21 = p.dI[c.idx21];
+  const double z20r = c.rhow * dS20.x + c.rhoi * dI20.x;
+  const double z21r = c.rhow * dS21.x + c.rhoi * dI21.x;
+  const double z21i = c.rhow * dS21.y + c.rhoi * dI21.y;
+  double* il = s->st.il;
+  il[8] = c.cI33 * z20r;
+  il[2] = c.cI13 * z21r;
+  il[5] = c.cI23 * z21i;
+  for (int q = 0; q < 9; ++q) s->dil[q] = il[q] - s->p_il[q];
+  double* mm = s->st.mm;
+  for (int i = 0; i < 2; ++i)
+    mm[i] = c.alpha * (c.inv_CA * (c.one_kE2 * il[3 * i + 2] + s->sum_il[3 * i + 2]) + c.inv_kf * s->sum_m[i]);
+  mm[2] = -c.inv_C * (c.one_kE2 * il[8] + s->sum_il[8]);
+  for (int q = 0; q < 3; ++q) s->dm[q] = mm[q] - s->p_m[q];
+  const double m1 = mm[0], m2 = mm[1], m3 = mm[2];
+  double* lam = s->st.lam;  // [3][3] complex, (re, im) interleaved
+  lam[0] = c.K0 * ((m1 * m1 + m2 * m2 + m3 * m3) + 2.0 * m3);
+  lam[1] = 0.0;
+  lam[12] = c.K20 * (m1 * m1 + m2 * m2 - 2.0 * m3 * m3 - 4.0 * m3);
+  lam[13] = 0.0;
+  lam[14] = -c.K21 * (m1 * (1.0 + m3));
+  lam[15] = -c.K21 * (-m2 * (1.0 + m3));
+  lam[16] = c.K22 * (m2 * m2 - m1 * m1);
+  lam[17] = c.K22 * (2.0 * m1 * m2);
+  for (int q = 0; q < 6; ++q) s->dlam[q] = lam[12 + q] - s->p_lam[q];
+  const double ig = 1.0 / c.gacc;
+  for (int q = 0; q < 6; ++q) {
+    const double tot = s->p_lam[q] + s->dlam[q];
+    s->dsl_rot[q] = (c.ekhTE * tot + s->sum_tsl[q]) * ig;
+    s->rr_rot[q] = c.calcRG ? (c.hTE2 * tot + s->sum_trr[q]) * ig : 0.0;
+  }
+}
+
+// eq. B18 coefficient, assembled while the synthesis stages its coefficients:
+// w1[l] ((rho_i dI + rho_w dS_prev) + rho_w load_inc) + w2[l] resp (+ rot at l = 2, m <= 2)
+struct CoefB18 {
+  const double2 *dI, *dSp, *ld, *resp;
+  const double *w1, *w2;
+  const double* rot;  // 6 doubles: (re, im) of (2,0), (2,1), (2,2)
+  double rhoi, rhow;
+  __device__ double2 operator()(int, int idx, int l, int m) const {
+    const double2 a = dI[idx], b = dSp[idx], q = ld[idx], v = resp[idx];
+    double ar = (a.x * rhoi + b.x * rhow) + q.x * rhow;
+    double ai = (a.y * rhoi + b.y * rhow) + q.y * rhow;
+    const double u = w1[l], t = w2[l];
+    double2 r = make_double2(ar * u + v.x * t, ai * u + v.y * t);
+    if (l == 2 && m <= 2) {
+      r.x += rot[2 * m];
+      r.y += rot[2 * m + 1];
+    }
+    return r;
+  }
+};
+
+// analysis input of the inner sweep: field 0 = dslxy C* - tO (rO - tO), field 1 = C*
+struct LoadRT {
+  const double *dsl, *cstar, *tO;
+  __device__ double operator()(int f, long i) const { return f == 0 ? dsl[i] * cstar[i] - tO[i] : cstar[i]; }
+};
+template <int NF>
+struct LoadPtrs {
+  const double* g[NF];
+  __device__ double operator()(int f, long i) const { return g[f][i]; }
+};
+
+// conservation constant, ocean-load update and convergence ratio (solver.py:1002-1030)
+__global__ void __launch_bounds__(256) k_update(Pts p, Consts c) {
+  __shared__ double sh[256];
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done) return;
+  const int n = c.ncoef;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double c00 = p.cstar_lm[0].x;
+  const double conserv = (1.0 / c00) * (-(c.rhoi / c.rhow) * p.dI[0].x - p.rt_lm[0].x);
+  double a = 0.0;
+  if (i < n) {
+    const double2 sp = p.p_ocean[i], rt = p.rt_lm[i], cs = p.cstar_lm[i];
+    double2 nw = make_double2((-sp.x + rt.x) + cs.x * conserv, (-sp.y + rt.y) + cs.y * conserv);
+    p.load_inc[i] = nw;
+    p.dS_cur[i] = make_double2(sp.x + nw.x, sp.y + nw.y);
+    a = cabs_d(nw);
+  }
+  const double bs = block_sum(a, sh);
+  if (post_partial(bs, p.partial, &s->counter)) {
+    const double sa = sum_partials(p.partial, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      const double so = s->so;
+      double xi;
+      if (fabs(sa - so) < EPS0 && fabs(so) < EPS0)
+        xi = 0.0;
+      else if (fabs(so) < EPS0)
+        xi = fabs((sa - so) / (EPS0 * 10));
+      else
+        xi = fabs((sa - so) / so);
+      s->sa = sa;
+      s->so = sa;
+      s->st.xi = xi;
+      s->st.conserv = conserv;
+      s->st.numiter += 1;
+      if (xi <= c.eps1) {
+        s->st.done = 1;
+      } else if (s->st.numiter >= MAX_INNER) {
+        s->st.failed = 1;
+        s->st.done = 1;
+      }
+      s->counter = 0;
+    }
+  }
+}
+
+// topography / ocean-function re-guess and the grounded-ice flip count (solver.py:1032-1040,661-684)
+__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c) {
+  __shared__ double sh[256];
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done) return;
+  const long n = c.npts;
+  const double cadd = s->st.conserv * c.K00;
+  const bool regs = c.checkmarine && s->st.nacc < GR_MAXACC;
+  double flips = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double t = p.tinit[i] - (p.dsl[i] + cadd);
+    p.topo[i] = t;
+    p.cxy[i] = t < 0.0 ? 1.0 : 0.0;
+    if (regs) {
+      const double ice = p.ice_cur[i];
+      const double cand = ice * keep_factor(ice, t, c.rhow, c.rhoi);
+      const bool bnew = cand < EPS0;
+      if (bnew != (p.beta[i] != 0.0)) flips += 1.0;
+    }
+  }
+  const double bs = block_sum(flips, sh);
+  if (post_partial(bs, p.partial, &s->counter)) {
+    const double tot = sum_partials(p.partial, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      int changed = 0;
+      if (regs) {
+        s->st.reguess += 1;
+        if (tot > 0.0) {
+          changed = 1;
+          s->st.nacc += 1;
+          s->st.ice_analyses += 1;
+        }
+      }
+      s->ice_changed = changed;
+      s->counter = 0;
+    }
+  }
+}
+
+__global__ void k_refresh2(Pts p, Consts c) {
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done) return;
+  const long n = c.npts;
+  const bool changed = s->ice_changed != 0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    if (changed) {
+      const double ice = p.ice_cur[i];
+      const double cand = ice * keep_factor(ice, p.topo[i], c.rhow, c.rhoi);
+      p.icestar[i] = cand;
+      p.beta[i] = cand < EPS0 ? 1.0 : 0.0;
+    }
+    const double cs = p.cxy[i] * p.beta[i];
+    p.cstar[i] = cs;
+    p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
+  }
+}
+
+__global__ void k_dice(Pts p, Consts c) {
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done || !*(volatile int*)&s->ice_changed) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.ncoef) return;
+  const double2 ist = p.icestar_lm[i], old = p.oldice[i], pi = p.p_ice[i];
+  const double2 d = make_double2(ist.x - old.x, ist.y - old.y);
+  p.dice[i] = d;
+  p.dI[i] = make_double2(pi.x + d.x, pi.y + d.y);
+}
+
+// ---------------------------------------------------------------------- step finish
+// commit (solver.py:1080-1095): load increment without degree 0 into P; ice, ocean, previous ice
+__device__ __forceinline__ bool finish_active(const DevState* s) {
+  return *(volatile const int*)&s->st.done && !*(volatile const int*)&s->st.finished;
+}
+
+__global__ void k_commit(Pts p, Consts c, int extrap) {
+  if (!finish_active(p.st)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.ncoef) return;
+  const int l = p.l_of[i];
+  const double2 d = p.dice[i], q = p.load_inc[i];
+  double2 cur = make_double2(d.x * c.rhoi + q.x * c.rhow, d.y * c.rhoi + q.y * c.rhow);
+  if (l == 0) cur = make_double2(0.0, 0.0);
+  const double2 pl = p.p_load[i];
+  p.p_load[i] = make_double2(pl.x + cur.x, pl.y + cur.y);
+  p.p_ice[i] = p.dI[i];
+  p.p_ocean[i] = p.dS_cur[i];
+  p.prev_ice[i] = p.icestar_lm[i];
+  if (extrap) p.h2[i] = q;  // becomes the newest history entry (the host swaps h1/h2)
+}
+
+// final topography, ocean function, G and the row sums of the degree-0 diagnostics (one block per row)
+__global__ void __launch_bounds__(256) k_post_grid(Pts p, Consts c) {
+  if (!finish_active(p.st)) return;
+  __shared__ double sh[256];
+  const int j = blockIdx.x;
+  const double cadd = p.st->st.conserv * c.K00;
+  double a = 0.0, b = 0.0, e = 0.0, f = 0.0;
+  for (int i = threadIdx.x; i < c.nlon; i += blockDim.x) {
+    const long k = (long)j * c.nlon + i;
+    const double dsl = p.dsl[k] + cadd;
+    const double t = p.tinit[k] - dsl;
+    const double o = t < 0.0 ? 1.0 : 0.0;
+    p.topo[k] = t;
+    p.topo_prev[k] = t;
+    p.cxy[k] = o;
+    if (c.calcRG) p.G[k] = dsl + p.rr[k];
+    const double sl = dsl * o, bt = p.beta[k];
+    a += sl;
+    b += o;
+    e += sl * bt;
+    f += o * bt;
+  }
+  double r;
+  r = block_sum(a, sh);
+  if (threadIdx.x == 0) p.rows[j] = r;
+  __syncthreads();
+  r = block_sum(b, sh);
+  if (threadIdx.x == 0) p.rows[c.nlat + j] = r;
+  __syncthreads();
+  r = block_sum(e, sh);
+  if (threadIdx.x == 0) p.rows[2 * c.nlat + j] = r;
+  __syncthreads();
+  r = block_sum(f, sh);
+  if (threadIdx.x == 0) p.rows[3 * c.nlat + j] = r;
+}
+
+// degree-0 diagnostics and the rotational commit (solver.py:1060-1110; solver_opt.py:525-547)
+__global__ void k_finish(Pts p, Consts c) {
+  if (!finish_active(p.st) || threadIdx.x != 0 || blockIdx.x != 0) return;
+  DevState* s = p.st;
+  double q[4] = {0, 0, 0, 0};
+  for (int k = 0; k < 4; ++k)
+    for (int j = 0; j < c.nlat; ++j) q[k] += p.w00[j] * p.rows[k * c.nlat + j];
+  s->st.gmslc_ocn = q[0] / q[1];
+  s->st.gmslc_ocnBeta = q[2] / q[3];
+  s->st.ocean_area = q[1] * c.area;
+  s->st.oceanBeta_area = q[3] * c.area;
+  s->st.ice_volume = p.icestar_lm[0].x * c.area;
+  if (c.tpw) {
+    for (int k = 0; k < 9; ++k) s->p_il[k] += s->dil[k];
+    for (int k = 0; k < 3; ++k) s->p_m[k] += s->dm[k];
+    for (int k = 0; k < 6; ++k) s->p_lam[k] += s->dlam[k];
+  }
+  s->st.finished = 1;
+}
+
+__global__ void k_init_areas(Pts p, Consts c, const double2* c0, const double2* cs0, const double2* i0) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DevState* s = p.st;
+  s->st.ocean_area0 = c0[0].x * c.area;
+  s->st.oceanBeta_area0 = cs0[0].x * c.area;
+  s->st.ice_volume0 = i0[0].x * c.area;
+}
+
+}  // namespace
+
+// ============================================================================== host side
+static Pts make_pts(slx_solver* s, const double* ice_cur, const double2* oldice) {
+  Pts p;
+  p.st = s->st;
+  p.partial = s->partial;
+  p.tinit = s->tinit;
+  p.cxy0 = s->cxy0;
+  p.beta0 = s->beta0;
+  p.cstar0 = s->cstar0;
+  p.cxy = s->cxy;
+  p.beta = s->beta;
+  p.cstar = s->cstar;
+  p.tO = s->tO;
+  p.icestar = s->icestar;
+  p.dsl = s->dsl;
+  p.rr = s->rr;
+  p.G = s->G;
+  p.topo = s->topo;
+  p.topo_prev = s->topo_prev;
+  p.ice_prev = s->ice_prev;
+  p.tmp = s->tmp;
+  p.rows = s->rows;
+  p.w00 = s->w00;
+  p.ice_cur = ice_cur;
+  p.p_load = s->p_load;
+  p.p_ice = s->p_ice;
+  p.p_ocean = s->p_ocean;
+  p.prev_ice = s->prev_ice;
+  p.icestar_lm = s->icestar_lm;
+  p.oldice = const_cast<double2*>(oldice);
+  p.dice = s->dice;
+  p.dI = s->dI;
+  p.cstar_lm = s->cstar_lm;
+  p.rt_lm = s->rt_lm;
+  p.load_inc = s->load_inc;
+  p.dS_cur = s->dS_cur;
+  p.visc = s->visc;
+  p.rad = s->rad;
+  p.h1 = s->h1;
+  p.h2 = s->h2;
+  p.ttl = s->ttl;
+  p.ttlekhl = s->ttlekhl;
+  p.ttlhe = s->ttlhe;
+  p.decay = s->decay;
+  p.a2 = s->a2;
+  p.r2 = s->r2;
+  p.l_of = s->h->l_of;
+  return p;
+}
+
+static int grid_blocks(long n) {
+  long b = (n + 255) / 256;
+  return (int)(b > 4096 ? 4096 : b);
+}
+
+extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const slx_outputs* o, slx_solver** out) {
+  SLX_CHECK(h && p && o && out, "slx_solver_create: null argument");
+  SLX_CHECK(o->tgrid && o->ocean && o->beta && o->R && o->G, "slx_solver_create: all output grids are required");
+  SLX_CHECK(p->lmax == h->lmax, "solver lmax must equal the transform's lmax");
+  SLX_CHECK(h->max_fields >= 3, "the solver needs a transform handle with max_fields >= 3");
+  SLX_CHECK(p->kmax >= 0 && p->nmod2 >= 0 && p->nmod2 <= p->kmax, "bad mode counts");
+  SLX_CHECK(!p->tpw || p->lmax >= 2, "rotational feedback needs lmax >= 2");
+  auto* s = new slx_solver();
+  s->h = h;
+  const int L = p->lmax, nl = L + 1, K = p->kmax;
+  Consts& c = s->c;
+  c.rhoi = p->rhoi;
+  c.rhow = p->rhow;
+  c.gacc = p->gacc;
+  c.K00 = SLX_SQRT2_LEGACY * std::sqrt(0.5);
+  c.eps1 = p->epsilon1;
+  c.area = 4 * PI_LEGACY * p->radius * p->radius;
+  const double r4 = p->radius * p->radius * p->radius * p->radius;
+  c.cI33 = (-8 * PI_LEGACY * r4 / (3 * std::sqrt(5.0)));
+  c.cI13 = (-8 * PI_LEGACY * r4 / (std::sqrt(30.0)));
+  c.cI23 = (8 * PI_LEGACY * r4 / (std::sqrt(30.0)));
+  c.alpha = 1 / (1 - p->kTE2 / p->kf);
+  c.inv_CA = 1 / (p->moiC - p->moiA);
+  c.one_kE2 = 1 + p->kE2;
+  c.inv_kf = 1 / p->kf;
+  c.inv_C = 1.0 / p->moiC;
+  const double r2w2 = p->radius * p->radius * p->omega * p->omega;
+  c.K0 = r2w2 / 3.0;
+  c.K20 = r2w2 / (6.0 * std::sqrt(5.0));
+  c.K21 = r2w2 / std::sqrt(30.0);
+  c.K22 = r2w2 / std::sqrt(5.0 * 24.0);
+  c.ekhTE = 1 + p->kTE2 - p->hTE2;
+  c.hTE2 = p->hTE2;
+  c.L = L;
+  c.nl = nl;
+  c.ncoef = h->ncoef;
+  c.nlat = h->nlat;
+  c.nlon = h->nlon;
+  c.npts = h->nlat * h->nlon;
+  c.kmax = K;
+  c.nmod2 = p->nmod2;
+  c.tpw = p->tpw;
+  c.checkmarine = p->checkmarine;
+  c.calcRG = p->calcRG;
+  c.idx20 = 2;
+  c.idx21 = h->h_moff[1] + 1;
+  c.idx22 = h->h_moff[2];
+  s->extrap = p->extrap;
+  const size_t G = (size_t)c.npts * sizeof(double), S = (size_t)h->ncoef * sizeof(double2);
+  const size_t nblk_max = std::max<size_t>(4096, (h->ncoef + 255) / 256 + 1);
+  SLX_HIP(hipMalloc(&s->st, sizeof(DevState)));
+  SLX_HIP(hipMemset(s->st, 0, sizeof(DevState)));
+  SLX_HIP(hipMalloc(&s->partial, nblk_max * sizeof(double)));
+  SLX_HIP(hipMalloc(&s->rows, 4 * (size_t)c.nlat * sizeof(double)));
+  SLX_HIP(hipMalloc(&s->w00, (size_t)c.nlat * sizeof(double)));
+  std::vector<double> w00(c.nlat);
+  for (int j = 0; j < c.nlat; ++j) w00[j] = h->h_w[j] / c.nlon * std::sqrt(0.5) / SLX_SQRT2_LEGACY;
+  SLX_HIP(hipMemcpy(s->w00, w00.data(), w00.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<int> nmod(nl);
+  for (int l = 0; l < nl; ++l) nmod[l] = p->nmod[l];
+  SLX_HIP(hipMalloc(&s->nmod, nl * sizeof(int)));
+  SLX_HIP(hipMemcpy(s->nmod, nmod.data(), nl * sizeof(int), hipMemcpyHostToDevice));
+  const size_t KL = (size_t)std::max(K, 1) * nl * sizeof(double);
+  s->h_s.assign(p->s, p->s + (size_t)K * nl);
+  for (double** q : {&s->s, &s->decay, &s->omd, &s->a_sl, &s->a_rr}) SLX_HIP(hipMalloc(q, KL));
+  SLX_HIP(hipMemset(s->decay, 0, KL));
+  SLX_HIP(hipMemset(s->omd, 0, KL));
+  if (K > 0) {
+    SLX_HIP(hipMemcpy(s->s, p->s, (size_t)K * nl * sizeof(double), hipMemcpyHostToDevice));
+    SLX_HIP(hipMemcpy(s->a_sl, p->a_sl, (size_t)K * nl * sizeof(double), hipMemcpyHostToDevice));
+    SLX_HIP(hipMemcpy(s->a_rr, p->a_rr, (size_t)K * nl * sizeof(double), hipMemcpyHostToDevice));
+  }
+  std::vector<double> a2(4 * (size_t)std::max(K, 1), 0.0);
+  for (int k = 0; k < K; ++k) {
+    a2[k] = p->a_il[k];
+    a2[K + k] = p->a_m[k];
+    a2[2 * K + k] = p->a_tsl[k];
+    a2[3 * K + k] = p->a_trr[k];
+  }
+  SLX_HIP(hipMalloc(&s->a2, a2.size() * sizeof(double)));
+  SLX_HIP(hipMemcpy(s->a2, a2.data(), a2.size() * sizeof(double), hipMemcpyHostToDevice));
+  SLX_HIP(hipMalloc(&s->r2, 18 * (size_t)std::max(K, 1) * sizeof(double)));
+  SLX_HIP(hipMemset(s->r2, 0, 18 * (size_t)std::max(K, 1) * sizeof(double)));
+  for (auto pr : {std::make_pair(&s->ttl, p->ttl), std::make_pair(&s->ttlekhl, p->ttlekhl),
+                  std::make_pair(&s->ttlhe, p->ttlhe)}) {
+    SLX_HIP(hipMalloc(pr.first, nl * sizeof(double)));
+    SLX_HIP(hipMemcpy(*pr.first, pr.second, nl * sizeof(double), hipMemcpyHostToDevice));
+  }
+  s->topo = o->tgrid;
+  s->cxy = o->ocean;
+  s->beta = o->beta;
+  s->rr = o->R;
+  s->G = o->G;
+  for (double** q : {&s->tinit, &s->cxy0, &s->beta0, &s->cstar0, &s->cstar, &s->tO, &s->icestar, &s->dsl,
+                     &s->topo_prev, &s->ice_prev, &s->tmp}) {
+    SLX_HIP(hipMalloc(q, G));
+    SLX_HIP(hipMemset(*q, 0, G));
+  }
+  for (double2** q : {&s->p_load, &s->p_ice, &s->p_ocean, &s->prev_ice, &s->icestar_lm, &s->oldice_lm, &s->dice,
+                      &s->dI, &s->cstar_lm, &s->rt_lm, &s->load_inc, &s->dS_cur, &s->visc, &s->rad, &s->h1, &s->h2}) {
+    SLX_HIP(hipMalloc(q, S));
+    SLX_HIP(hipMemset(*q, 0, S));
+  }
+  SLX_HIP(hipMalloc(&s->scratch_lm, 3 * S));
+  SLX_HIP(hipMalloc(&s->R, (size_t)std::max(K, 1) * S));
+  SLX_HIP(hipMemset(s->R, 0, (size_t)std::max(K, 1) * S));
+  SLX_HIP(hipHostMalloc(&s->pinned, sizeof(slx_status), hipHostMallocDefault));
+  *out = s;
+  return 0;
+}
+
+extern "C" int slx_solver_destroy(slx_solver* s) {
+  if (!s) return 0;
+  void* ptrs[] = {s->st, s->partial, s->rows, s->w00, s->nmod, s->s, s->decay, s->omd, s->a_sl, s->a_rr, s->a2,
+                  s->r2, s->ttl, s->ttlekhl, s->ttlhe, s->tinit, s->cxy0, s->beta0, s->cstar0,
+                  s->cstar, s->tO, s->icestar, s->dsl, s->topo_prev, s->ice_prev, s->tmp,
+                  s->p_load, s->p_ice, s->p_ocean, s->prev_ice, s->icestar_lm, s->oldice_lm, s->dice, s->dI,
+                  s->cstar_lm, s->rt_lm, s->load_inc, s->dS_cur, s->visc, s->rad, s->h1, s->h2, s->scratch_lm,
+                  s->R};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (s->pinned) (void)hipHostFree(s->pinned);
+  delete s;
+  return 0;
+}
+
+extern "C" int slx_solver_set_dt(slx_solver* s, double dt) {
+  SLX_CHECK(s != nullptr, "null solver");
+  if (dt == s->dt) return 0;
+  const int K = s->c.kmax, nl = s->c.nl;
+  std::vector<int> nmod(nl);
+  SLX_HIP(hipMemcpy(nmod.data(), s->nmod, nl * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<double> d((size_t)K * nl, 0.0), o((size_t)K * nl, 0.0);
+  for (int l = 1; l < nl; ++l)
+    for (int k = 0; k < nmod[l] && k < K; ++k) d[(size_t)k * nl + l] = std::exp(-s->h_s[(size_t)k * nl + l] * dt / 1000.0);
+  for (size_t i = 0; i < d.size(); ++i) o[i] = 1.0 - d[i];
+  if (K > 0) {
+    SLX_HIP(hipMemcpy(s->decay, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+    SLX_HIP(hipMemcpy(s->omd, o.data(), o.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  s->dt = dt;
+  return 0;
+}
+
+extern "C" int slx_solver_init(slx_solver* s, const double* topo0, const double* ice0, void* stream) {
+  SLX_CHECK(s && topo0 && ice0, "slx_solver_init: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  Pts p = make_pts(s, nullptr, s->oldice_lm);
+  k_init_grid<<<grid_blocks(s->c.npts), 256, 0, st>>>(p, s->c, topo0, ice0);
+  SLX_LAUNCH_CHECK();
+  // degree-0 areas of C0, C*0 and I*0 (solver.py:571-580); the I*0 spectrum is the "previous ice"
+  // of step 1 (solver.py:859-862)
+  int rc = launch_fft_ana<256>(s->h, LoadPtrs<3>{{s->cxy0, s->cstar0, s->tmp}}, 3, Pred{}, st);
+  if (rc) return rc;
+  rc = launch_leg_ana<3>(s->h, s->h->WS, s->h->WD,
+                         OutPtrs<3>{{s->scratch_lm, s->scratch_lm + s->c.ncoef, s->oldice_lm}}, Pred{}, st);
+  if (rc) return rc;
+  k_init_areas<<<1, 1, 0, st>>>(p, s->c, s->scratch_lm, s->scratch_lm + s->c.ncoef, s->oldice_lm);
+  SLX_LAUNCH_CHECK();
+  s->step = 0;
+  s->nhist = 0;
+  return 0;
+}
+
+extern "C" int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, double dt1, double dt2, void* stream) {
+  SLX_CHECK(s && ice_cur, "slx_step_begin: null argument");
+  SLX_CHECK(s->dt > 0.0, "slx_solver_set_dt must be called before the first step");
+  hipStream_t st = (hipStream_t)stream;
+  SLX_CHECK(s->closed, "slx_step_begin: the previous step has not finished (check slx_solver_status)");
+  s->step += 1;
+  s->closed = 0;
+  s->iters_enqueued = 0;
+  s->ice_cur = ice_cur;
+  const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
+  Pts p = make_pts(s, ice_cur, oldice);
+  const Consts& c = s->c;
+  k_begin<<<1, 64, 0, st>>>(p, c);
+  SLX_LAUNCH_CHECK();
+  int rc = launch_modal(s->h, c.kmax, s->nmod, s->decay, s->omd, s->a_sl, s->a_rr, s->R, s->p_load, s->visc,
+                        s->rad, st);
+  if (rc) return rc;
+  k_ice_grid<<<grid_blocks(c.npts), 256, 0, st>>>(p, c);
+  SLX_LAUNCH_CHECK();
+  rc = launch_fft_ana<256>(s->h, LoadPtrs<2>{{s->icestar, s->cstar}}, 2, Pred{}, st);
+  if (rc) return rc;
+  rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->icestar_lm, s->cstar_lm}}, Pred{}, st);
+  if (rc) return rc;
+  GuessArgs g;
+  g.extrap_ready = (s->extrap && s->nhist >= 2) ? 1 : 0;
+  g.uniform = (dt0 == dt1 && dt1 == dt2) ? 1 : 0;
+  const double w = (dt0 + dt1) / (dt1 + dt2);
+  g.c1 = dt0 / dt1 * (1.0 + w);
+  g.c2 = dt0 / dt2 * w;
+  k_guess<<<(c.ncoef + 255) / 256, 256, 0, st>>>(p, c, g);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
+  SLX_CHECK(s != nullptr, "null solver");
+  hipStream_t st = (hipStream_t)stream;
+  const Consts& c = s->c;
+  const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
+  Pts p = make_pts(s, s->ice_cur, oldice);
+  const int* done = &s->st->st.done;
+  const int* changed = &s->st->ice_changed;
+  const int nsb = (c.ncoef + 255) / 256;
+  for (int it = 0; it < n; ++it) {
+    const bool first = (s->iters_enqueued == 0);
+    s->iters_enqueued += 1;
+    k_rotation<<<1, 64, 0, st>>>(p, c);
+    SLX_LAUNCH_CHECK();
+    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->ttlekhl, s->ttl, s->st->dsl_rot, c.rhoi, c.rhow};
+    int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
+    if (rc) return rc;
+    rc = launch_fft_syn<256>(s->h, StorePlain{s->dsl, 0}, 1, Pred{done, 0}, st);
+    if (rc) return rc;
+    LoadRT lr{s->dsl, s->cstar, s->tO};
+    if (first) {
+      rc = launch_fft_ana<256>(s->h, lr, 1, Pred{done, 0}, st);
+      if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
+    } else {
+      rc = launch_fft_ana<256>(s->h, lr, 2, Pred{done, 0}, st);
+      if (!rc)
+        rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
+    }
+    if (rc) return rc;
+    k_update<<<nsb, 256, 0, st>>>(p, c);
+    SLX_LAUNCH_CHECK();
+    k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c);
+    SLX_LAUNCH_CHECK();
+    k_refresh2<<<grid_blocks(c.npts), 256, 0, st>>>(p, c);
+    SLX_LAUNCH_CHECK();
+    if (c.checkmarine) {
+      rc = launch_fft_ana<256>(s->h, LoadPtrs<1>{{s->icestar}}, 1, Pred{done, 0, changed, 1}, st);
+      if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
+      if (rc) return rc;
+      k_dice<<<nsb, 256, 0, st>>>(p, c);
+      SLX_LAUNCH_CHECK();
+    }
+  }
+  return 0;
+}
+
+extern "C" int slx_step_finish(slx_solver* s, void* stream) {
+  SLX_CHECK(s != nullptr, "null solver");
+  hipStream_t st = (hipStream_t)stream;
+  const Consts& c = s->c;
+  const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
+  Pts p = make_pts(s, s->ice_cur, oldice);
+  if (c.calcRG) {
+    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->rad, s->ttlhe, s->ttl, s->st->rr_rot, c.rhoi, c.rhow};
+    const Pred fin{&s->st->st.done, 1, &s->st->st.finished, 0};
+    int rc = launch_leg_syn<1>(s->h, cb, fin, st);
+    if (rc) return rc;
+    rc = launch_fft_syn<256>(s->h, StorePlain{s->rr, 0}, 1, fin, st);
+    if (rc) return rc;
+  }
+  k_commit<<<(c.ncoef + 255) / 256, 256, 0, st>>>(p, c, s->extrap);
+  SLX_LAUNCH_CHECK();
+  k_post_grid<<<c.nlat, 256, 0, st>>>(p, c);
+  SLX_LAUNCH_CHECK();
+  k_finish<<<1, 64, 0, st>>>(p, c);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
+// Host bookkeeping once the step's finish has run on the device (status.finished): the newest
+// converged increment (written to h2 by the commit) becomes h1.
+static void step_close(slx_solver* s) {
+  if (s->extrap) {
+    std::swap(s->h1, s->h2);
+    s->nhist = std::min(2, s->nhist + 1);
+  }
+  s->closed = 1;
+}
+
+extern "C" int slx_solver_status(slx_solver* s, slx_status* out, void* stream) {
+  SLX_CHECK(s && out, "slx_solver_status: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  SLX_HIP(hipMemcpyAsync(s->pinned, &s->st->st, sizeof(slx_status), hipMemcpyDeviceToHost, st));
+  SLX_HIP(hipStreamSynchronize(st));
+  std::memcpy(out, s->pinned, sizeof(slx_status));
+  if (out->finished && !s->closed && s->step > 0) step_close(s);
+  return 0;
+}
+
+extern "C" int slx_solver_fields(slx_solver* s, slx_fields* o) {
+  SLX_CHECK(s && o, "slx_solver_fields: null argument");
+  o->icestar = s->icestar;
+  o->tinit = s->tinit;
+  o->cstar0 = s->cstar0;
+  o->cxy0 = s->cxy0;
+  o->beta0 = s->beta0;
+  o->dS_converged = s->dS_cur;
+  o->p_load = s->p_load;
+  o->p_ice = s->p_ice;
+  o->p_ocean = s->p_ocean;
+  o->prev_ice = s->prev_ice;
+  o->load_inc = s->load_inc;
+  o->modal_R = s->R;
+  return 0;
+}

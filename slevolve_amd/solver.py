@@ -1,0 +1,303 @@
+"""Sea-level solver on the GPU with the interface of the reference's ``SeaLevelSolver``
+(python/slevolve/solver.py:499-1144) and ``OptSeaLevelSolver`` (python/slevolve/solver_opt.py:267-979).
+
+The whole time step runs on the device (libslx.so): modal advance, grounded-ice projection, the
+inner fixed point with its ocean-function / grounded-ice re-guesses and FP64 convergence test,
+the radial displacement, the commit and the degree-0 diagnostics.  The host enqueues one step,
+reads one small status block back, and returns the reference's result dictionary; the grids in
+it are device tensors that the solver overwrites at the next step (the reference also returns
+its reused buffers).
+
+Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap".
+The second-order ramp, analytic rates, horizontal read-out, axial-rotation variants, Anderson
+mixing and the outer-iteration reuse store of the reference are not provided and raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+from .memory import decay_factors, love_weights
+from .sht import SHT
+
+PI = 3.1415926535898
+EPS0 = float(np.finfo(np.float64).eps)
+GR_MAXACC = 3
+ALL_OPTS = ("raw", "diag", "dsl", "numba", "sums")
+INNER_MODES = ("off", "extrap", "anderson")
+
+
+class Planet:
+    """Earth constants (solver.py:98-110)."""
+
+    def __init__(self):
+        self.radius = 6.371e6
+        self.mass = 5.976e24
+        self.rhoi = 920.0
+        self.rhow = 1000.0
+        self.gacc = 9.80665
+        self.omega = 7.292e-5
+        self.moiA = 0.3296145 * self.mass * self.radius**2
+        self.moiC = 0.3307007 * self.mass * self.radius**2
+        self.kf = 0.9342 + 0.008
+
+
+def parse_opts(spec):
+    """``"all"`` / ``""`` / ``"raw,dsl"`` -> set of option names (solver_opt.py:168-185)."""
+    if spec is None:
+        return set()
+    spec = spec.strip()
+    if spec in ("", "none", "0"):
+        return set()
+    if spec == "all":
+        return set(ALL_OPTS)
+    out = set()
+    for tok in (t.strip() for t in spec.split(",")):
+        if tok and tok not in ALL_OPTS:
+            raise ValueError(f"unknown --opt {tok!r} (known: {','.join(ALL_OPTS)})")
+        if tok:
+            out.add(tok)
+    return out
+
+
+class _StateView:
+    """Dense [l, m] views of the device-resident committed state (solver.state in the reference)."""
+
+    def __init__(self, slv):
+        self._slv = slv
+
+    def _dense(self, name):
+        f = self._slv._fields
+        n = self._slv.sht.ncoef
+        t = torch.empty(n, dtype=torch.complex128, device=self._slv.dev)
+        _copy_dev(t, getattr(f, name), n * 16)
+        return self._slv.sht.to_dense(t)
+
+    p_load = property(lambda self: self._dense("p_load"))
+    p_ice = property(lambda self: self._dense("p_ice"))
+    p_ocean = property(lambda self: self._dense("p_ocean"))
+    prev_ice = property(lambda self: self._dense("prev_ice"))
+
+
+def _copy_dev(dst: torch.Tensor, src_ptr: int, nbytes: int):
+    """Device-to-device copy from a library-owned buffer into a torch tensor (current stream)."""
+    lib = _HIPRT.get()
+    rc = lib.hipMemcpyAsync(C.c_void_p(dst.data_ptr()), C.c_void_p(src_ptr), C.c_size_t(nbytes), 3,
+                            C.c_void_p(D.stream_ptr()))
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpyAsync failed ({rc})")
+
+
+class _HIPRT:
+    _lib = None
+
+    @classmethod
+    def get(cls):
+        if cls._lib is None:
+            cls._lib = C.CDLL("libamdhip64.so")
+            cls._lib.hipMemcpyAsync.restype = C.c_int
+            cls._lib.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        return cls._lib
+
+
+class SeaLevelSolver:
+    """Exact-memory sea-level solver; constructor and ``step`` as in solver.py:521-1144."""
+
+    def __init__(self, sht, love, topo0, ice0, *, norder=512, dt=1, tpw=True, checkmarine=True, calcRG=True,
+                 epsilon1=1.0e-5, planet=None, mem_threads=1, ramp=False, axial_rotation="off",
+                 axial_tau_c=None, inner="off"):
+        if ramp:
+            raise NotImplementedError("the second-order ramp placement is not provided")
+        if axial_rotation != "off":
+            raise NotImplementedError("axial_rotation other than 'off' is not provided")
+        if inner not in ("off", "extrap"):
+            raise NotImplementedError(f"inner={inner!r} is not provided (off | extrap)")
+        if not isinstance(sht, SHT):
+            raise TypeError("sht must be a slevolve_amd.SHT (the GPU transform)")
+        if sht.lmax != norder:
+            raise ValueError("norder must equal the transform's lmax")
+        if sht.max_fields < 3:
+            raise ValueError("the solver needs an SHT with max_fields >= 3")
+        self.sht, self.love = sht, love
+        self.norder, self.nl = int(norder), int(norder) + 1
+        self.dt = float(dt)
+        self.tpw, self.checkmarine, self.calcRG = bool(tpw), bool(checkmarine), bool(calcRG)
+        self.epsilon1 = float(epsilon1)
+        self.inner = inner
+        self.horizontal = False
+        self.ramp = False
+        self.p = planet or Planet()
+        self.nlat, self.nlon = sht.nlat, sht.nlon
+        self.dev = dev = sht.dev
+        p = self.p
+        nl = self.nl
+
+        nmod, s, a_sl, a_rr, a2, nmod2 = love_weights(love, norder)
+        self._nmod, self._s = nmod, s
+        K = s.shape[0]
+        ell = np.arange(nl, dtype=np.float64)
+        ttl = np.zeros(nl)
+        ttl[1:] = (4.0 * PI * (p.radius**3)) / (p.mass * (2 * ell[1:] + 1.0))
+        ekhl = np.zeros(nl)
+        ekhl[1:] = 1 + love.ke[1:nl] - love.he[1:nl]
+        he = np.concatenate(([0.0], love.he[1:nl]))
+        self._host = dict(nmod=np.ascontiguousarray(nmod, dtype=np.int32), s=np.ascontiguousarray(s),
+                          a_sl=np.ascontiguousarray(a_sl), a_rr=np.ascontiguousarray(a_rr),
+                          a2=np.ascontiguousarray(a2), ttl=ttl, ttlekhl=ttl * ekhl, ttlhe=ttl * he)
+        h = self._host
+        dp = lambda a: a.ctypes.data_as(_lib.P_double)  # noqa: E731
+        prm = _lib.SlxSolverParams(
+            lmax=self.norder, kmax=K, nmod2=nmod2, tpw=int(self.tpw), checkmarine=int(self.checkmarine),
+            calcRG=int(self.calcRG), extrap=int(inner == "extrap"), epsilon1=self.epsilon1,
+            radius=p.radius, mass=p.mass, rhoi=p.rhoi, rhow=p.rhow, gacc=p.gacc, omega=p.omega, moiA=p.moiA,
+            moiC=p.moiC, kf=p.kf, kTE2=float(love.kTE[2]) if norder >= 2 else 0.0,
+            hTE2=float(love.hTE[2]) if norder >= 2 else 0.0, kE2=float(love.ke[2]) if norder >= 2 else 0.0,
+            nmod=h["nmod"].ctypes.data_as(_lib.P_int), s=dp(h["s"]), a_sl=dp(h["a_sl"]), a_rr=dp(h["a_rr"]),
+            a_il=dp(h["a2"][0]), a_m=dp(h["a2"][1]), a_tsl=dp(h["a2"][2]), a_trr=dp(h["a2"][3]),
+            ttl=dp(h["ttl"]), ttlekhl=dp(h["ttlekhl"]), ttlhe=dp(h["ttlhe"]))
+        g = lambda: torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=dev)  # noqa: E731
+        self.topoxy, self.cxy, self.beta, self.rrxy, self.Gxy = g(), g(), g(), g(), g()
+        outs = _lib.SlxOutputs(tgrid=self.topoxy.data_ptr(), ocean=self.cxy.data_ptr(), beta=self.beta.data_ptr(),
+                               R=self.rrxy.data_ptr(), G=self.Gxy.data_ptr())
+        handle = C.c_void_p()
+        _lib.check(_lib.load().slx_solver_create(sht.handle, C.byref(prm), C.byref(outs), C.byref(handle)),
+                   "slx_solver_create")
+        self._s_handle = handle
+        f = _lib.SlxFields()
+        _lib.call("slx_solver_fields", handle, C.byref(f))
+        self._fields = f
+        topo = D.as_device(topo0, torch.float64, dev, (self.nlat, self.nlon))
+        ice = D.as_device(ice0, torch.float64, dev, (self.nlat, self.nlon))
+        _lib.call("slx_solver_init", handle, topo.data_ptr(), ice.data_ptr(), D.stream_ptr())
+        _lib.call("slx_solver_set_dt", handle, self.dt)
+        st = self._status()
+        self.ocean_area0, self.oceanBeta_area0, self.ice_volume0 = st.ocean_area0, st.oceanBeta_area0, st.ice_volume0
+        self.tinit = torch.empty_like(self.topoxy)
+        _copy_dev(self.tinit, f.tinit, self.tinit.numel() * 8)
+        self.tinit_0 = self.tinit
+        self.cxy0 = torch.empty_like(self.topoxy)
+        _copy_dev(self.cxy0, f.cxy0, self.cxy0.numel() * 8)
+        self.beta0 = torch.empty_like(self.topoxy)
+        _copy_dev(self.beta0, f.beta0, self.beta0.numel() * 8)
+        self.iter = 0
+        self.gr_counts = {"reguess": 0, "ice_analyses": 0}
+        self._dt_hist = [self.dt, self.dt]  # lengths of steps n-1, n-2
+        self._ice_buf = torch.empty((self.nlat, self.nlon), dtype=torch.float64, device=dev)
+        self._dS = torch.empty((self.nl, self.nl), dtype=torch.complex128, device=dev)
+        self._spec_tmp = torch.empty(sht.ncoef, dtype=torch.complex128, device=dev)
+        self._guess_iters = 4
+        self.prof = {}
+        self.state = _StateView(self)
+        self.il = np.zeros((3, 3))
+        self.mm = np.zeros(3)
+        self.lam = np.zeros((3, 3), dtype=np.complex128)
+
+    # ---------------------------------------------------------------------------- helpers
+    def _status(self):
+        st = _lib.SlxStatus()
+        _lib.call("slx_solver_status", self._s_handle, C.byref(st), D.stream_ptr())
+        return st
+
+    def set_step_dt(self, dt):
+        if dt is not None:
+            self.dt = float(dt)
+        _lib.call("slx_solver_set_dt", self._s_handle, self.dt)
+
+    def __del__(self):
+        h = getattr(self, "_s_handle", None)
+        if h is not None and h.value:
+            try:
+                torch.cuda.synchronize(self.dev)
+                _lib.load().slx_solver_destroy(h)
+            except Exception:
+                pass
+            self._s_handle = None
+
+    # ------------------------------------------------------------------------------- step
+    def step(self, icexy_cur, profile=False, dt=None, rates=False):
+        """Advance one time step (solver.py:814-1144); returns the reference's result dict."""
+        if rates:
+            raise NotImplementedError("analytic rates are not provided")
+        t_all = time.perf_counter()
+        self.set_step_dt(dt)
+        self.iter += 1
+        if isinstance(icexy_cur, torch.Tensor) and icexy_cur.device == self.dev and \
+                icexy_cur.dtype == torch.float64 and icexy_cur.is_contiguous():
+            ice = icexy_cur
+        else:
+            ice = self._ice_buf
+            ice.copy_(D.as_device(icexy_cur, torch.float64, self.dev, (self.nlat, self.nlon)))
+        h = self._s_handle
+        sp = D.stream_ptr()
+        _lib.call("slx_step_begin", h, ice.data_ptr(), self.dt, self._dt_hist[0], self._dt_hist[1], sp)
+        n = self._guess_iters
+        total = 0
+        while True:
+            _lib.call("slx_step_iterate", h, n, sp)
+            total += n
+            _lib.call("slx_step_finish", h, sp)
+            st = self._status()
+            if st.failed:
+                raise RuntimeError(f"inner loop failed to converge, xi={st.xi}")
+            if st.finished:
+                break
+            n = 1 if total < 16 else max(1, total // 2)
+        self._guess_iters = max(1, st.numiter)
+        self._dt_hist = [self.dt, self._dt_hist[0]]
+        self.gr_counts = {"reguess": st.reguess, "ice_analyses": st.ice_analyses}
+        self.il = np.array(st.il).reshape(3, 3)
+        self.mm = np.array(st.mm)
+        lam = np.array(st.lam).reshape(3, 3, 2)
+        self.lam = lam[..., 0] + 1j * lam[..., 1]
+        _copy_dev(self._spec_tmp, self._fields.dS_converged, self.sht.ncoef * 16)
+        self.sht.to_dense(self._spec_tmp, self._dS)
+        wall = time.perf_counter() - t_all
+        if profile:
+            self.prof = {"total": wall}
+        return {
+            "iter": self.iter,
+            "numiter": st.numiter,
+            "tgrid": self.topoxy,
+            "ocean": self.cxy,
+            "beta": self.beta,
+            "R": self.rrxy if self.calcRG else None,
+            "G": self.Gxy if self.calcRG else None,
+            "horiz": None,
+            "dS_converged": self._dS,
+            "ocean_area": st.ocean_area,
+            "oceanBeta_area": st.oceanBeta_area,
+            "gmslc_ocn": st.gmslc_ocn,
+            "gmslc_ocnBeta": st.gmslc_ocnBeta,
+            "ice_volume": st.ice_volume,
+            "il": self.il.copy(),
+            "mm": self.mm.copy(),
+            "lambda": self.lam.copy(),
+            "rates": None,
+            "wall": wall,
+            "prof": self.prof if profile else None,
+        }
+
+
+class OptSeaLevelSolver(SeaLevelSolver):
+    """``OptSeaLevelSolver`` signature (solver_opt.py:268-296).  The optimisation switches ``opt``
+    are accepted (the device path always uses the fused forms); ``inner`` may be "off" or
+    "extrap"."""
+
+    def __init__(self, *args, opt=(), inner="off", anderson_depth=3, max_mask_refresh=3, anderson_freeze=2,
+                 anderson_keep=False, reuse="off", reuse_store=None, **kw):
+        if inner == "anderson":
+            raise NotImplementedError("Anderson mixing is not provided")
+        if reuse != "off":
+            raise NotImplementedError("the cross-pass reuse store is not provided")
+        self.opt = set(opt)
+        super().__init__(*args, inner=inner, **kw)
+        self.inner_stats = {}
+
+
+__all__ = ["SeaLevelSolver", "OptSeaLevelSolver", "Planet", "PI", "EPS0", "GR_MAXACC", "parse_opts"]

@@ -1,0 +1,118 @@
+"""Whole-step parity of the GPU solver against the oracle and the reference's fixtures."""
+import numpy as np
+import pytest
+
+from oracle.love import LoveNumbers as OLove
+from oracle.sht import OracleSHT
+from oracle.solver import SeaLevelSolver as OSolver
+from slevolve_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_solver(nlat, L, text, topo, ice0, **kw):
+    from slevolve_amd import SHT, LoveNumbers, OptSeaLevelSolver
+
+    return OptSeaLevelSolver(SHT(nlat, 2 * nlat, L), LoveNumbers(text, L, text=True), topo, ice0, norder=L, **kw)
+
+
+def oracle_solver(nlat, L, text, topo, ice0, **kw):
+    return OSolver(OracleSHT(nlat, 2 * nlat, L), OLove(text, L, text=True), topo, ice0, norder=L, **kw)
+
+
+def compare(a, b, tol=1e-9):
+    h = lambda t: t.cpu().numpy()  # noqa: E731
+    assert a["numiter"] == b["numiter"], (a["numiter"], b["numiter"])
+    e = {k: float(np.abs(h(a[k]) - b[k]).max()) for k in ("tgrid", "R", "G")}
+    assert e["tgrid"] < tol and e["R"] < tol and e["G"] < tol, e
+    assert np.array_equal(h(a["ocean"]), b["ocean"]) and np.array_equal(h(a["beta"]), b["beta"])
+    assert np.abs(h(a["dS_converged"]) - b["dS_converged"]).max() < 1e-9
+    for k in ("ocean_area", "oceanBeta_area", "ice_volume"):
+        assert abs(a[k] - b[k]) <= 1e-12 * abs(b[k]) + 1e-3, (k, a[k], b[k])
+    for k in ("gmslc_ocn", "gmslc_ocnBeta"):
+        assert abs(a[k] - b[k]) < 1e-9, (k, a[k], b[k])
+    assert np.allclose(a["mm"], b["mm"], rtol=1e-9, atol=1e-20)
+    assert np.allclose(a["il"], b["il"], rtol=1e-9, atol=1e10)
+    return e
+
+
+@pytest.mark.parametrize("checkmarine", [True, False])
+def test_quickstart_case(prem_text, topo_seed, checkmarine):
+    lat, lon, topo, ice = syn.synthetic_earth(128, topo_seed)
+    kw = dict(dt=1000.0, tpw=True, checkmarine=checkmarine, calcRG=True, epsilon1=1e-5)
+    a = gpu_solver(128, 63, prem_text, topo, ice[0], **kw)
+    b = oracle_solver(128, 63, prem_text, topo, ice[0], **kw)
+    nit = []
+    for k in range(1, 9):
+        print(f"[quickstart cm={checkmarine}] step {k}", flush=True)
+        ra, rb = a.step(ice[k]), b.step(ice[k])
+        compare(ra, rb)
+        nit.append(ra["numiter"])
+    assert nit == [6, 4, 6, 7, 7, 7, 7, 7]
+    rsl = -(ra["tgrid"].cpu().numpy() - topo)
+    i, j = int(np.argmin(np.abs(lat - 10.0))), int(np.argmin(np.abs(lon - 180.0)))
+    assert round(rsl[i, j], 3) == (78.067 if checkmarine else 78.065)
+
+
+def test_small_case_floating_ice(prem_text):
+    from slevolve_amd import SHT
+
+    gl = SHT(128, 256, 63).gaulats
+    topo, ice = syn.small_case(gl, 256)
+    a = gpu_solver(128, 63, prem_text, topo, ice[0], dt=1000.0, epsilon1=1e-5)
+    b = oracle_solver(128, 63, prem_text, topo, ice[0], dt=1000.0, epsilon1=1e-5)
+    nit = []
+    for k in range(1, 9):
+        print(f"[small case] step {k}", flush=True)
+        ra, rb = a.step(ice[k]), b.step(ice[k])
+        compare(ra, rb)
+        nit.append(ra["numiter"])
+    assert nit == [4, 4, 5, 8, 6, 6, 5, 6]
+    assert a.gr_counts == {"reguess": 36, "ice_analyses": 1}
+
+
+def test_tpw_off_and_no_rg(prem_text, topo_seed):
+    lat, lon, topo, ice = syn.synthetic_earth(64, topo_seed)
+    kw = dict(dt=500.0, tpw=False, checkmarine=True, calcRG=True, epsilon1=1e-7)
+    a = gpu_solver(64, 63, prem_text, topo, ice[0], **kw)
+    b = oracle_solver(64, 63, prem_text, topo, ice[0], **kw)
+    for k in range(1, 5):
+        compare(a.step(ice[k]), b.step(ice[k]))
+
+
+def test_extrap_nonuniform_steps(prem_text, topo_seed):
+    lat, lon, topo, ice = syn.synthetic_earth(128, topo_seed)
+    kw = dict(dt=1.0, epsilon1=1e-5, inner="extrap")
+    a = gpu_solver(128, 63, prem_text, topo, ice[0], **kw)
+    b = oracle_solver(128, 63, prem_text, topo, ice[0], **kw)
+    for k, dt in enumerate([1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0, 200.0], start=1):
+        print(f"[extrap vardt] step {k} dt {dt}", flush=True)
+        compare(a.step(ice[k], dt=dt), b.step(ice[k], dt=dt))
+
+
+def test_reference_fixture_tgrid8(prem_text, topo_seed, tgrid8_change):
+    """reference/synthetic/tgrid8.npy (512 x 1024, lmax 511, 8 x 1 yr, extrapolated start):
+    1e-9 m with numiter 5,5,6,6,5,5,5,5."""
+    lat, lon, topo, ice = syn.synthetic_earth(512, topo_seed)
+    a = gpu_solver(512, 511, prem_text, topo, ice[0], dt=1.0, epsilon1=1e-5, inner="extrap")
+    nit = []
+    for k in range(1, 9):
+        r = a.step(ice[k])
+        nit.append(r["numiter"])
+        print(f"[tgrid8] step {k} numiter {r['numiter']}", flush=True)
+    err = float(np.abs(r["tgrid"].cpu().numpy() - (topo_seed + tgrid8_change)).max())
+    print(f"[tgrid8] max |GPU - reference| = {err:.3e} m", flush=True)
+    assert nit == [5, 5, 6, 6, 5, 5, 5, 5]
+    assert err < 1e-9
+
+
+def test_lmax512_workload_vs_oracle():
+    """The 512 x 1024, lmax 512 (aliased top degree, Nyquist order) benchmark workload, first steps."""
+    w = syn.workload("lmax512")
+    kw = dict(dt=w.dt, epsilon1=1e-5, inner="extrap")
+    a = gpu_solver(w.nlat, w.lmax, w.love_text, w.topo, w.ice(0), **kw)
+    b = oracle_solver(w.nlat, w.lmax, w.love_text, w.topo, w.ice(0), **kw)
+    for k in range(1, 4):
+        print(f"[lmax512] step {k}", flush=True)
+        e = compare(a.step(w.ice(k)), b.step(w.ice(k)))
+        print(f"[lmax512] step {k} errors {e}", flush=True)
