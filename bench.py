@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Solver steps/sec of the sea-level time step on one MI355X.
+
+Workload (default "lmax512"): Gauss-Legendre 512 x 1024 grid, lmax 512, 30 viscous modes per
+degree, 550 steps of 1 yr (seeded synthetic Love numbers, topography to degree 256, a polar cap and
+a retreating marine sector), FP64, extrapolated inner start, rotation and marine check on,
+epsilon1 = 1e-5.  A "step" is one solver step (modal advance, inner fixed point to convergence,
+radial displacement, commit).  All ice grids are uploaded before the timed region; W warm-up steps,
+then K timed steps bracketed by device synchronisation.  Prints one JSON line.
+
+Also reported: the roofline of the dominant kernel (the per-order Legendre analysis contraction),
+timed live with HIP events on the stream it is launched on, and the CPU oracle's steps/s on a
+bounded sample of the same workload on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lmax512|lmax1023|lmax63]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solver steps/sec at lmax=512/1023, mixed vs FP64 GPU, at max|ΔSL|≤1e-6 m"
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's steps - warmup)")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="lmax512", choices=("lmax63", "lmax512", "lmax1023"))
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, ice, budget):
+    """The FP64 NumPy oracle on this host: set-up, then as many steps as fit in `budget` seconds
+    (at least 1) of the same workload; steps/s over the timed steps."""
+    import numpy as np
+
+    from oracle.love import LoveNumbers as OLove
+    from oracle.sht import OracleSHT
+    from oracle.solver import SeaLevelSolver as OSolver
+
+    slv = OSolver(OracleSHT(w.nlat, w.nlon, w.lmax), OLove(w.love_text, w.lmax, text=True), w.topo, ice(0),
+                  norder=w.lmax, dt=w.dt, epsilon1=1e-5, inner="extrap")
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        slv.step(ice(n + 1))
+        n += 1
+        if time.perf_counter() - t0 > budget or n >= w.nsteps:
+            break
+    el = time.perf_counter() - t0
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    return {"value": n / el, "unit": "steps/s", "cores": int(min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))),
+            "kind": "port",
+            "sample": f"first {n} steps of the {w.name} workload (FP64 NumPy/SciPy oracle, BLAS threads "
+                      f"{os.environ.get('OMP_NUM_THREADS', 'default')}), {el:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding of "
+              f"this path is a design note, not a measured configuration", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from slevolve_amd import SHT, LoveNumbers, OptSeaLevelSolver
+    from slevolve_amd import _lib
+    from slevolve_amd import synthetic as syn
+
+    w = syn.workload(a.workload)
+    steps = a.steps if a.steps is not None else w.nsteps - a.warmup
+    nsteps = a.warmup + steps
+    if nsteps > w.nsteps:
+        raise SystemExit(f"warmup + steps = {nsteps} exceeds the workload's {w.nsteps} steps")
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    ice_dev = torch.empty((nsteps + 1, w.nlat, w.nlon), dtype=torch.float64, device=dev)
+    for k in range(nsteps + 1):
+        ice_dev[k].copy_(torch.from_numpy(w.ice(k)))
+    print(f"[bench] {w.name}: inputs on the device ({ice_dev.numel() * 8 / 1e9:.2f} GB, "
+          f"{time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    sht = SHT(w.nlat, w.nlon, w.lmax, device=dev)
+    love = LoveNumbers(w.love_text, w.lmax, text=True)
+    slv = OptSeaLevelSolver(sht, love, w.topo, w.ice(0), norder=w.lmax, dt=w.dt, tpw=True, checkmarine=True,
+                            calcRG=True, epsilon1=1e-5, inner="extrap")
+    for k in range(1, a.warmup + 1):
+        slv.step(ice_dev[k])
+    torch.cuda.synchronize()
+    numiter = []
+    t0 = time.perf_counter()
+    for k in range(a.warmup + 1, nsteps + 1):
+        numiter.append(slv.step(ice_dev[k])["numiter"])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sps = steps / el
+    print(f"[bench] {steps} steps in {el:.3f} s, numiter mean {np.mean(numiter):.2f}", file=sys.stderr, flush=True)
+
+    # ---- dominant kernel: the per-order analysis contraction of a 2-field sweep, timed with HIP
+    # events on the current stream (the stream the library launches on)
+    grids = torch.from_numpy(np.stack([w.topo, w.ice(1)])).to(dev)
+    coef = torch.zeros((2, sht.ncoef), dtype=torch.complex128, device=dev)
+    sht.analysis_packed(grids, coef)
+    reps = 50
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(5):
+        sht.analysis_contraction(2, coef)
+    ev0.record(stream)
+    for _ in range(reps):
+        sht.analysis_contraction(2, coef)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_kernel = ev0.elapsed_time(ev1) / 1e3 / reps
+    flops = 2.0 * 2 * sht.ncoef * w.nlat  # 4 flops per (l, m, ring pair) per field, 2 fields
+    roof = {"bound": "mfma", "achieved": flops / t_kernel / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+            "frac": flops / t_kernel / PEAK_FP64, "traffic": None,
+            "kernel": "k_leg_ana<2,R,2> (per-order FP64 Legendre analysis contraction, 2 fields)",
+            "kernel_us": t_kernel * 1e6, "flops_per_launch": flops}
+    out = {
+        "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * el / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{w.name}: {w.nlat}x{w.nlon} GL grid, lmax {w.lmax}, {w.nmodes} modes/degree, "
+                               f"{w.nsteps} x {w.dt:g} yr, FP64, inner extrap, tpw+checkmarine on, eps1 1e-5",
+                   "numiter_mean": float(np.mean(numiter)), "numiter_total": int(np.sum(numiter)),
+                   "ice": w.note},
+        "roofline": roof,
+    }
+    del slv
+    torch.cuda.synchronize()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, w.ice, a.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -42,6 +42,10 @@ int slx_sht_info(const slx_sht* h, int* dims, double* gaulats, double* weights);
 /* Replaces SHT.analysis (sht.py:274-302) / DuccSHT.analysis (sht_ducc.py:134-154) for nf fields
  * at once: grid (device) nf x [nlat][nlon] -> coef (device) nf x [ncoef] packed. */
 int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* coef, void* stream);
+/* The second stage of slx_sht_analysis alone: the per-order Legendre contraction of the folded
+ * ring spectra left in the handle's work buffers by the last analysis of nf fields (no
+ * counterpart in the reference; lets a profiler time the contraction by itself). */
+int slx_sht_analysis_contraction(slx_sht* h, int nf, void* coef, void* stream);
 /* Replaces SHT.synthesis (sht.py:305-343) / DuccSHT.synthesis (sht_ducc.py:156-187): coef
  * (device) nf x [ncoef] -> grid (device) nf x [nlat][nlon].  Im of the m = 0 (and Nyquist)
  * coefficients is ignored, as irfft does. */

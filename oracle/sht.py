@@ -50,6 +50,10 @@ def gauss_legendre(nlat: int):
     return nodes, w
 
 
+SCALE_K = 600  # one scale unit: 2^600
+SCALE_T = 300  # a scaled value is renormalised once it exceeds 2^300
+
+
 def legendre_block(x: np.ndarray, lmax: int):
     """List over m = 0..lmax of arrays ``P[m][l - m, j] = P~_lm(x_j)``.
 
@@ -57,10 +61,18 @@ def legendre_block(x: np.ndarray, lmax: int):
     P~_{m+1,m} = sqrt(2m+3) x P~_mm; three-term step
     P~_n = (x P~_{n-1} - eps_{n-1} P~_{n-2}) / eps_n, eps_n = sqrt((n^2 - m^2)/(4 n^2 - 1))
     (sht.py:105-144).
+
+    Extended-range arithmetic: prod is kept as mantissa * 2^E (exact power-of-two rescaling),
+    and a start value below 2^-300 is carried as P~ 2^(600 s) with an integer s > 0 that the
+    recurrence renormalises when the scaled value passes 2^300; values with s > 0 (true value
+    < 2^-300) are set to 0.  Wherever the reference's prod stays a normal number the results
+    are identical to sht.py; where it underflows (nlat >~ 700 at high degree) the reference's
+    start values lose precision and its high-degree functions are wrong, these are not.
     """
     x = np.asarray(x, dtype=np.float64)
     s2 = 1.0 - x * x
     prod = np.ones_like(x)
+    E = np.zeros(x.size, dtype=np.int64)
     a, b = 1.0, 0.0
     out = []
     for m in range(lmax + 1):
@@ -68,19 +80,31 @@ def legendre_block(x: np.ndarray, lmax: int):
             a += 2.0
             b += 2.0
             prod = prod * s2 * a / b
+            low = prod < 2.0**-500
+            prod[low] *= 2.0**500
+            E[low] -= 500
         nl = lmax + 1 - m
         P = np.zeros((nl, x.size))
-        p0 = np.sqrt(0.5 * prod)
-        P[0] = p0
+        base = np.sqrt(0.5 * prod)
+        e2 = E // 2
+        lg = np.frexp(base)[1] - 1 + e2  # floor(log2 |P~_mm|)
+        sc = np.maximum(0, np.ceil((-SCALE_T - lg) / SCALE_K)).astype(np.int64)
+        p0 = np.ldexp(base, e2 + SCALE_K * sc)
+        P[0] = np.where(sc == 0, p0, 0.0)
         if nl >= 2:
             p1 = np.sqrt(float(2 * m + 3)) * x * p0
-            P[1] = p1
+            P[1] = np.where(sc == 0, p1, 0.0)
             eps_prev = np.sqrt(float((m + 1) ** 2 - m * m) / float(4 * (m + 1) ** 2 - 1))
             for i in range(2, nl):
                 n = m + i
                 eps = np.sqrt(float(n * n - m * m) / float(4 * n * n - 1))
                 p2 = (x * p1 - eps_prev * p0) / eps
-                P[i] = p2
+                big = (sc > 0) & (np.abs(p2) >= 2.0**SCALE_T)
+                if big.any():
+                    p2[big] *= 2.0**-SCALE_K
+                    p1[big] *= 2.0**-SCALE_K
+                    sc[big] -= 1
+                P[i] = np.where(sc == 0, p2, 0.0)
                 p0, p1, eps_prev = p1, p2, eps
         out.append(P)
     return out

@@ -74,21 +74,38 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   }
   const int nh = h->nh, nm = h->nm, L = lmax;
   std::vector<double> x(nh), wn(nh), ws(nh), pmm((size_t)nm * nh), s2m3(nm);
+  std::vector<int> pms((size_t)nm * nh);
   for (int j = 0; j < nh; ++j) {
     x[j] = h->h_x[j];
     wn[j] = h->h_w[j] / nlon;
     ws[j] = h->h_w[nlat - 1 - j] / nlon;
   }
-  // sectoral start P~_mm = sqrt(0.5 prod_m), prod_m = prod (1 - x^2) A/B (sht.py:118-126)
+  // sectoral start P~_mm = sqrt(0.5 prod_m), prod_m = prod (1 - x^2) A/B (sht.py:118-126), in
+  // extended range: prod = mantissa 2^E with exact power-of-two renormalisation, and a start value
+  // below 2^-SLX_SCALE_T stored as P~ 2^(SLX_SCALE_K s) with its scale count s (pms).  Identical to
+  // the reference wherever its prod is a normal number.
   for (int j = 0; j < nh; ++j) {
     double sinsq = 1.0 - x[j] * x[j], A = 1.0, B = 0.0, prod = 1.0;
+    long E = 0;
     for (int m = 0; m < nm; ++m) {
       if (m != 0) {
         A += 2.0;
         B += 2.0;
         prod = prod * sinsq * A / B;
+        if (prod < 0x1p-500) {
+          prod *= 0x1p500;
+          E -= 500;
+        }
       }
-      pmm[(size_t)m * nh + j] = std::sqrt(0.5 * prod);
+      const double base = std::sqrt(0.5 * prod);
+      const long e2 = E / 2;
+      int ex = 0;
+      std::frexp(base, &ex);
+      const long lg = (long)(ex - 1) + e2;
+      long sc = 0;
+      if (-SLX_SCALE_T - lg > 0) sc = (-SLX_SCALE_T - lg + SLX_SCALE_K - 1) / SLX_SCALE_K;
+      pmm[(size_t)m * nh + j] = std::ldexp(base, (int)(e2 + SLX_SCALE_K * sc));
+      pms[(size_t)m * nh + j] = (int)sc;
     }
   }
   h->h_moff.resize(nm + 1);
@@ -123,6 +140,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   ALLOC_COPY(h->wn, wn);
   ALLOC_COPY(h->ws, ws);
   ALLOC_COPY(h->pmm, pmm);
+  ALLOC_COPY(h->pms, pms);
   ALLOC_COPY(h->rec, rec);
   ALLOC_COPY(h->s2m3, s2m3);
   ALLOC_COPY(h->moff, h->h_moff);
@@ -140,7 +158,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
 
 extern "C" int slx_sht_destroy(slx_sht* h) {
   if (!h) return 0;
-  void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->tw,
+  void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->tw,
                   h->WS, h->WD, h->FN, h->FSo};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -163,14 +181,11 @@ extern "C" int slx_sht_info(const slx_sht* h, int* dims, double* gaulats, double
 }
 
 
-extern "C" int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* coef_v, void* stream) {
+extern "C" int slx_sht_analysis_contraction(slx_sht* h, int nf, void* coef_v, void* stream) {
   double2* coef = static_cast<double2*>(coef_v);
-  SLX_CHECK(h && grid && coef, "slx_sht_analysis: null argument");
-  SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_analysis: nf out of range");
+  SLX_CHECK(h && coef, "slx_sht_analysis_contraction: null argument");
+  SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_analysis_contraction: nf out of range");
   hipStream_t s = (hipStream_t)stream;
-  const long gstride = (long)h->nlat * h->nlon;
-  int rc = launch_fft_ana<256>(h, LoadPlain{grid, gstride}, nf, Pred{}, s);
-  if (rc) return rc;
   switch (nf) {
     case 1: return launch_leg_ana<1>(h, h->WS, h->WD, OutPtrs<1>{{coef}}, Pred{}, s);
     case 2: return launch_leg_ana<2>(h, h->WS, h->WD, OutPtrs<2>{{coef, coef + h->ncoef}}, Pred{}, s);
@@ -178,6 +193,15 @@ extern "C" int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* co
       return launch_leg_ana<3>(h, h->WS, h->WD, OutPtrs<3>{{coef, coef + h->ncoef, coef + 2 * (long)h->ncoef}},
                                Pred{}, s);
   }
+}
+
+extern "C" int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* coef, void* stream) {
+  SLX_CHECK(h && grid && coef, "slx_sht_analysis: null argument");
+  SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_analysis: nf out of range");
+  const long gstride = (long)h->nlat * h->nlon;
+  int rc = launch_fft_ana<256>(h, LoadPlain{grid, gstride}, nf, Pred{}, (hipStream_t)stream);
+  if (rc) return rc;
+  return slx_sht_analysis_contraction(h, nf, coef, stream);
 }
 
 extern "C" int slx_sht_synthesis(slx_sht* h, const void* coef_v, int nf, double* grid, void* stream) {

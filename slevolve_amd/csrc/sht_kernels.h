@@ -104,6 +104,7 @@ struct ShtDev {
   const double* wn;
   const double* ws;
   const double* pmm;
+  const int* pms;
   const double2* rec;
   const double* s2m3;
   const int* moff;
@@ -168,6 +169,7 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, const double2* __
   const int L = d.lmax, nh = d.nh;
   const long fstride = (long)d.nm * nh;
   double xr[R], pa[R], pb[R];
+  int sc[R];
   double2 S[R][NF], D[R][NF];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -176,6 +178,7 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, const double2* __
     xr[r] = ok ? d.x[j] : 0.0;
     pa[r] = 0.0;
     pb[r] = ok ? d.pmm[(long)m * nh + j] : 0.0;  // P_{m}; pa is P_{m-1} := 0
+    sc[r] = ok ? d.pms[(long)m * nh + j] : 0;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       S[r][f] = ok ? WS[f * fstride + (long)m * nh + j] : make_double2(0.0, 0.0);
@@ -189,6 +192,9 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, const double2* __
     double acc[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = 0.0;
+    double mk[R];  // 1 where the ring's values are in range this chunk, 0 while still scaled
+#pragma unroll
+    for (int r = 0; r < R; ++r) mk[r] = sc[r] == 0 ? 1.0 : 0.0;
 #pragma unroll
     for (int il = 0; il < CH; ++il) {
       const int l = l0 + il;
@@ -214,12 +220,21 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, const double2* __
           pa[r] = pb[r];
           pb[r] = p;
         }
+        p *= mk[r];
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
           double2 v = even ? S[r][f] : D[r][f];
           acc[(il * NF + f) * 2 + 0] += p * v.x;
           acc[(il * NF + f) * 2 + 1] += p * v.y;
         }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (sc[r] > 0 && fabs(pb[r]) >= 0x1p300) {
+        pa[r] *= 0x1p-600;
+        pb[r] *= 0x1p-600;
+        sc[r] -= 1;
       }
     }
     // reduce-scatter over the 64 lanes: after the step with offset o, lane keeps the half of
@@ -271,17 +286,20 @@ __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, Coef cf, double2* __re
   if (j >= d.nh) return;
   const double x = d.x[j];
   double pa = 0.0, pb = d.pmm[(long)m * d.nh + j];
+  int sc = d.pms[(long)m * d.nh + j];
+  double mk = sc == 0 ? 1.0 : 0.0;
   double2 E[NF], O[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     double2 c = cl[f * (L + 1)];
-    E[f] = make_double2(pb * c.x, pb * c.y);
+    E[f] = make_double2(pb * mk * c.x, pb * mk * c.y);
     O[f] = make_double2(0.0, 0.0);
   }
   if (nl > 1) {
     double p = d.s2m3[m] * x * pb;
     pa = pb;
     pb = p;
+    p *= mk;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       double2 c = cl[f * (L + 1) + 1];
@@ -296,6 +314,14 @@ __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, Coef cf, double2* __re
     double p3 = (x * p2 - c3.x * pb) * c3.y;
     pa = p2;
     pb = p3;
+    p2 *= mk;
+    p3 *= mk;
+    if (sc > 0 && fabs(pb) >= 0x1p300) {
+      pa *= 0x1p-600;
+      pb *= 0x1p-600;
+      sc -= 1;
+      mk = sc == 0 ? 1.0 : 0.0;
+    }
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       double2 ce = cl[f * (L + 1) + i], co = cl[f * (L + 1) + i + 1];
@@ -307,7 +333,7 @@ __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, Coef cf, double2* __re
   }
   if (i < nl) {
     double2 c2 = d.rec[mo + i];
-    double p2 = (x * pb - c2.x * pa) * c2.y;
+    double p2 = (x * pb - c2.x * pa) * c2.y * mk;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       double2 ce = cl[f * (L + 1) + i];

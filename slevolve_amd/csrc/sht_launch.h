@@ -16,6 +16,7 @@ inline ShtDev dev_of(const slx_sht* h) {
   d.wn = h->wn;
   d.ws = h->ws;
   d.pmm = h->pmm;
+  d.pms = h->pms;
   d.rec = h->rec;
   d.s2m3 = h->s2m3;
   d.moff = h->moff;

@@ -7,6 +7,12 @@
 #include <vector>
 
 #define SLX_SQRT2_LEGACY 1.41421356
+// Extended-range Legendre values: a value below 2^-SLX_SCALE_T is carried as P~ 2^(SLX_SCALE_K s)
+// with an integer scale count s > 0 and renormalised once the scaled value passes 2^SLX_SCALE_T;
+// while s > 0 it contributes nothing (its true magnitude is then below about 2^-180, the bound
+// including the growth between two renormalisation checks).
+#define SLX_SCALE_K 600
+#define SLX_SCALE_T 300
 
 // ------------------------------------------------------------------ error plumbing
 void slx_set_error(const std::string& msg);
@@ -43,7 +49,8 @@ struct slx_sht {
   double* x = nullptr;    // [nh] mu_j of the northern rings (north -> equator)
   double* wn = nullptr;   // [nh] w_j / nlon
   double* ws = nullptr;   // [nh] w_{nlat-1-j} / nlon
-  double* pmm = nullptr;  // [nm][nh] sectoral P~_mm(mu_j)
+  double* pmm = nullptr;  // [nm][nh] sectoral P~_mm(mu_j), scaled by 2^(SLX_SCALE_K pms)
+  int* pms = nullptr;     // [nm][nh] scale counts of pmm
   double2* rec = nullptr; // [ncoef] (eps_{l-1,m}, 1/eps_{l,m}) at (l, m), l >= m+2
   double* s2m3 = nullptr; // [nm] sqrt(2m+3)
   int* moff = nullptr;    // [nm+1]

@@ -75,6 +75,11 @@ class SHT:
         _lib.call("slx_sht_analysis", self._h, grids.data_ptr(), nf, out.data_ptr(), D.stream_ptr())
         return out
 
+    def analysis_contraction(self, nf: int, out: torch.Tensor) -> torch.Tensor:
+        """Re-run only the per-order contraction of the last analysis of nf fields."""
+        _lib.call("slx_sht_analysis_contraction", self._h, nf, out.data_ptr(), D.stream_ptr())
+        return out
+
     def synthesis_packed(self, coefs: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
         nf = coefs.shape[0]
         _lib.call("slx_sht_synthesis", self._h, coefs.data_ptr(), nf, out.data_ptr(), D.stream_ptr())

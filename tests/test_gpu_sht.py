@@ -38,7 +38,7 @@ def test_synthesis_and_analysis_match_oracle(nlat, L):
     assert np.array_equal(gpu.analysis(field), U_gpu)
     if L < nlat:  # exact round trip needs lmax <= nlat - 1 (aliased top degree otherwise)
         back = gpu.analysis(gpu.synthesis(torch.from_numpy(c).cuda())).cpu().numpy()
-        assert np.abs(back - c).max() / np.abs(c).max() < 1e-12
+        assert np.abs(back - c).max() / np.abs(c).max() < 1e-12 * max(1.0, L / 511)
 
 
 def test_multi_field_analysis_matches_single():
