@@ -63,7 +63,6 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   h->nm = lmax + 1;
   h->ncoef = (lmax + 1) * (lmax + 2) / 2;
   h->max_fields = max_fields;
-  h->ana_waves = 2;
   int lg = 0;
   while ((1 << lg) < nlon) ++lg;
   h->log2n = lg;
@@ -147,6 +146,41 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   ALLOC_COPY(h->l_of, l_of);
   ALLOC_COPY(h->jb, jb);
   ALLOC_COPY(h->tw, tw);
+  // degree segments: one analysis block per (order, segment); S (a multiple of 8) is the
+  // largest of 256, 128, 64, 32 that still gives >= 2048 waves
+  h->ana_nw = (nh > 256) ? 2 : 1;
+  auto count = [&](int S) {
+    long n = 0;
+    for (int m = 0; m < nm; ++m) n += (L - m + S) / S;
+    return n;
+  };
+  int S = 256;
+  while (S > 32 && count(S) * h->ana_nw < 2048) S /= 2;
+  h->seg_len = S;
+  std::vector<int> seg_m, seg_l0, first(nm);
+  for (int m = 0; m < nm; ++m) {
+    first[m] = (int)seg_m.size();
+    for (int l0 = m; l0 <= L; l0 += S) {
+      seg_m.push_back(m);
+      seg_l0.push_back(l0);
+    }
+  }
+  h->nseg = (int)seg_m.size();
+  ALLOC_COPY(h->seg_m, seg_m);
+  ALLOC_COPY(h->seg_l0, seg_l0);
+  ALLOC_COPY(h->seg_first, first);
+  SLX_HIP(hipMalloc(&h->seed_a, (size_t)h->nseg * nh * sizeof(double)));
+  SLX_HIP(hipMalloc(&h->seed_b, (size_t)h->nseg * nh * sizeof(double)));
+  SLX_HIP(hipMalloc(&h->seed_s, (size_t)h->nseg * nh * sizeof(int)));
+  SLX_HIP(hipMemset(h->seed_a, 0, (size_t)h->nseg * nh * sizeof(double)));
+  SLX_HIP(hipMemset(h->seed_b, 0, (size_t)h->nseg * nh * sizeof(double)));
+  SLX_HIP(hipMemset(h->seed_s, 0, (size_t)h->nseg * nh * sizeof(int)));
+  {
+    dim3 grid(nm, (nh + 63) / 64);
+    k_leg_seed<<<grid, 64>>>(dev_of(h), seg_of(h), h->seed_a, h->seed_b, h->seed_s);
+    SLX_LAUNCH_CHECK();
+    SLX_HIP(hipDeviceSynchronize());
+  }
 #undef ALLOC_COPY
   SLX_HIP(hipMalloc(&h->WS, wsz));
   SLX_HIP(hipMalloc(&h->WD, wsz));
@@ -159,7 +193,8 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
 extern "C" int slx_sht_destroy(slx_sht* h) {
   if (!h) return 0;
   void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->tw,
-                  h->WS, h->WD, h->FN, h->FSo};
+                  h->WS, h->WD, h->FN, h->FSo, h->seg_m, h->seg_l0, h->seg_first, h->seed_a, h->seed_b,
+                  h->seed_s};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -209,14 +244,10 @@ extern "C" int slx_sht_synthesis(slx_sht* h, const void* coef_v, int nf, double*
   SLX_CHECK(h && grid && coef, "slx_sht_synthesis: null argument");
   SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_synthesis: nf out of range");
   hipStream_t s = (hipStream_t)stream;
-  int rc;
-  CoefPlain cp{coef, h->ncoef};
-  switch (nf) {
-    case 1: rc = launch_leg_syn<1>(h, cp, Pred{}, s); break;
-    case 2: rc = launch_leg_syn<2>(h, cp, Pred{}, s); break;
-    default: rc = launch_leg_syn<3>(h, cp, Pred{}, s); break;
+  for (int f = 0; f < nf; ++f) {
+    const int rc = launch_leg_syn<1>(h, CoefPlain{coef + (long)f * h->ncoef, 0}, Pred{}, s, f);
+    if (rc) return rc;
   }
-  if (rc) return rc;
   return launch_fft_syn<256>(h, StorePlain{grid, (long)h->nlat * h->nlon}, nf, Pred{}, s);
 }
 

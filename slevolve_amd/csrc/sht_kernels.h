@@ -147,38 +147,112 @@ __global__ void __launch_bounds__(NT) k_fft_ana(ShtDev d, Loader ld, double2* __
 }
 
 // -------------------------------------------------------------------------------------------
-// Per-order analysis contraction.  One block of NW waves per order m; lane owns R rings
-// (j = lane + 64 (wave + NW r)); the FP64 recurrence runs per ring along l; each chunk of
-// CH = 32/NF degrees produces 64 partial sums per lane (degree x field x re/im) that are summed
-// over the 64 lanes by a reduce-scatter (lane i ends with value i), then over the waves in LDS
-// in a fixed order.  out[f] is packed; the result is divided by 1.41421356.
+// The degree recurrence, shared by every kernel that walks it so that all of them produce
+// bit-identical P~_lm: explicit FMA, renormalisation test every 8 degrees (after l with
+// (l - m) % 8 == 7) in extended range (SLX_SCALE_K / SLX_SCALE_T).
+struct LegState {
+  double pa, pb;  // P~_{l-2}, P~_{l-1} (scaled by 2^(SLX_SCALE_K sc))
+  int sc;
+};
+
+// advance to degree l (> m+1) with (eps_{l-1}, 1/eps_l) = c
+__device__ __forceinline__ double leg_next(double x, double2 c, LegState& s) {
+  const double p = fma(x, s.pb, -(c.x * s.pa)) * c.y;
+  s.pa = s.pb;
+  s.pb = p;
+  return p;
+}
+__device__ __forceinline__ void leg_renorm(LegState& s) {
+  if (s.sc > 0 && fabs(s.pb) >= 0x1p300) {
+    s.pa *= 0x1p-600;
+    s.pb *= 0x1p-600;
+    s.sc -= 1;
+  }
+}
+
+// Segments of the degree range: order m, first degree l0 (segments start every S degrees from
+// l = m, S a multiple of 8); the recurrence state before l0 for every northern ring (seed) is
+// produced once on the device by k_leg_seed with the same arithmetic.
+struct SegDev {
+  int nseg, S;
+  const int* seg_m;
+  const int* seg_l0;
+  const double* sa;  // [nseg][nh] P~_{l0-2}
+  const double* sb;  // [nseg][nh] P~_{l0-1}
+  const int* ss;     // [nseg][nh] scale count
+  const int* first;  // [nm] index of the first segment of order m
+};
+
+// One thread per (m, ring): walks l = m .. L and records the state at every segment start.
+static __global__ void k_leg_seed(ShtDev d, SegDev g, double* sa, double* sb, int* ss) {
+  const int m = blockIdx.x;
+  const int j = blockIdx.y * blockDim.x + threadIdx.x;
+  if (j >= d.nh) return;
+  const int L = d.lmax, mo = d.moff[m];
+  const double x = d.x[j];
+  LegState s{0.0, d.pmm[(long)m * d.nh + j], d.pms[(long)m * d.nh + j]};
+  int seg = g.first[m] + 1;
+  for (int l = m + 1; l <= L; ++l) {
+    if (l == m + 1) {
+      const double p = d.s2m3[m] * x * s.pb;
+      s.pa = s.pb;
+      s.pb = p;
+    } else {
+      leg_next(x, d.rec[mo + (l - m)], s);
+    }
+    if (((l - m) & 7) == 7) leg_renorm(s);
+    if (((l + 1 - m) % g.S) == 0 && l + 1 <= L) {
+      sa[(long)seg * d.nh + j] = s.pa;
+      sb[(long)seg * d.nh + j] = s.pb;
+      ss[(long)seg * d.nh + j] = s.sc;
+      ++seg;
+    }
+  }
+}
+
+// -------------------------------------------------------------------------------------------
+// Analysis contraction over one segment of degrees of one order per block (NW waves; lane owns
+// R rings j = lane + 64 (wave + NW r)).  Each chunk of CH degrees gives 64 partial sums per lane
+// (degree x field x re/im) that a reduce-scatter over the 64 lanes turns into one total per lane
+// (lane i holds value i), then a fixed-order sum over the waves in LDS.  out[f] is packed; the
+// result is divided by 1.41421356.
 template <int NF>
 struct OutPtrs {
   double2* p[NF];
 };
 
 template <int NF, int R, int NW>
-__global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, const double2* __restrict__ WS,
+__global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, SegDev g, const double2* __restrict__ WS,
                                                      const double2* __restrict__ WD, OutPtrs<NF> out,
                                                      Pred pr) {
   if (!pr.active()) return;
-  constexpr int CH = 32 / NF;
+  constexpr int CH = (NF == 1) ? 32 : (NF == 2 ? 16 : 8);
   __shared__ double red[2][NW][64];
-  const int m = blockIdx.x;
+  const int seg = blockIdx.x;
+  const int m = g.seg_m[seg], l0 = g.seg_l0[seg];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int L = d.lmax, nh = d.nh;
+  const int lend = min(l0 + g.S, L + 1);
   const long fstride = (long)d.nm * nh;
-  double xr[R], pa[R], pb[R];
-  int sc[R];
+  double xr[R], mk[R];
+  LegState st[R];
   double2 S[R][NF], D[R][NF];
+  const bool head = (l0 == m);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = lane + 64 * (wave + NW * r);
     const bool ok = j < nh;
     xr[r] = ok ? d.x[j] : 0.0;
-    pa[r] = 0.0;
-    pb[r] = ok ? d.pmm[(long)m * nh + j] : 0.0;  // P_{m}; pa is P_{m-1} := 0
-    sc[r] = ok ? d.pms[(long)m * nh + j] : 0;
+    if (head) {
+      st[r].pa = 0.0;
+      st[r].pb = ok ? d.pmm[(long)m * nh + j] : 0.0;
+      st[r].sc = ok ? d.pms[(long)m * nh + j] : 0;
+    } else {
+      st[r].pa = ok ? g.sa[(long)seg * nh + j] : 0.0;
+      st[r].pb = ok ? g.sb[(long)seg * nh + j] : 0.0;
+      st[r].sc = ok ? g.ss[(long)seg * nh + j] : 0;
+    }
+    mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       S[r][f] = ok ? WS[f * fstride + (long)m * nh + j] : make_double2(0.0, 0.0);
@@ -188,165 +262,205 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, const double2* __
   const double s23 = d.s2m3[m];
   const int mo = d.moff[m];
   int buf = 0;
-  for (int l0 = m; l0 <= L; l0 += CH) {
+  for (int lc = l0; lc < lend; lc += CH) {
     double acc[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = 0.0;
-    double mk[R];  // 1 where the ring's values are in range this chunk, 0 while still scaled
-#pragma unroll
-    for (int r = 0; r < R; ++r) mk[r] = sc[r] == 0 ? 1.0 : 0.0;
 #pragma unroll
     for (int il = 0; il < CH; ++il) {
-      const int l = l0 + il;
-      double c0 = 0.0, c1 = 0.0;  // (eps_{l-1}, 1/eps_l)
-      const bool first = (l == m), second = (l == m + 1);
-      if (l > m + 1 && l <= L) {
-        double2 c = d.rec[mo + (l - m)];
-        c0 = c.x;
-        c1 = c.y;
-      }
-      const bool even = ((l - m) & 1) == 0;
+      const int l = lc + il;
+      if (l < lend) {
+        double2 c = make_double2(0.0, 0.0);
+        if (l > m + 1) c = d.rec[mo + (l - m)];
+        const bool even = ((l - m) & 1) == 0;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        double p;
-        if (first) {
-          p = pb[r];
-        } else if (second) {
-          p = s23 * xr[r] * pb[r];
-          pa[r] = pb[r];
-          pb[r] = p;
-        } else {
-          p = (xr[r] * pb[r] - c0 * pa[r]) * c1;
-          pa[r] = pb[r];
-          pb[r] = p;
-        }
-        p *= mk[r];
+        for (int r = 0; r < R; ++r) {
+          double p;
+          if (l == m) {
+            p = st[r].pb;
+          } else if (l == m + 1) {
+            p = s23 * xr[r] * st[r].pb;
+            st[r].pa = st[r].pb;
+            st[r].pb = p;
+          } else {
+            p = leg_next(xr[r], c, st[r]);
+          }
+          p *= mk[r];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          double2 v = even ? S[r][f] : D[r][f];
-          acc[(il * NF + f) * 2 + 0] += p * v.x;
-          acc[(il * NF + f) * 2 + 1] += p * v.y;
+          for (int f = 0; f < NF; ++f) {
+            const double2 v = even ? S[r][f] : D[r][f];
+            acc[(il * NF + f) * 2 + 0] += p * v.x;
+            acc[(il * NF + f) * 2 + 1] += p * v.y;
+          }
+          if ((il & 7) == 7) {
+            leg_renorm(st[r]);
+            mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+          }
         }
       }
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (sc[r] > 0 && fabs(pb[r]) >= 0x1p300) {
-        pa[r] *= 0x1p-600;
-        pb[r] *= 0x1p-600;
-        sc[r] -= 1;
-      }
-    }
-    // reduce-scatter over the 64 lanes: after the step with offset o, lane keeps the half of
-    // the remaining values selected by its bit o; lane i ends with the sum of value i
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
       const bool up = (lane & o) != 0;
 #pragma unroll
       for (int i = 0; i < o; ++i) {
-        double send = up ? acc[i] : acc[i + o];
-        double keep = up ? acc[i + o] : acc[i];
+        const double send = up ? acc[i] : acc[i + o];
+        const double keep = up ? acc[i + o] : acc[i];
         acc[i] = keep + __shfl_xor(send, o, 64);
       }
     }
-    red[buf][wave][lane] = acc[0];
-    __syncthreads();
-    if (wave == 0) {
-      double v = red[buf][0][lane];
+    double v = acc[0];
+    if (NW > 1) {
+      red[buf][wave][lane] = v;
+      __syncthreads();
+      v = red[buf][0][lane];
 #pragma unroll
       for (int w = 1; w < NW; ++w) v += red[buf][w][lane];
-      const int il = lane / (2 * NF), f = (lane / 2) % NF, c = lane & 1;
-      const int l = l0 + il;
-      if (l <= L) reinterpret_cast<double*>(out.p[f])[2 * (long)(mo + l - m) + c] = v / SLX_SQRT2_LEGACY;
+      buf ^= 1;
     }
-    buf ^= 1;
+    if (wave == 0 && lane < 2 * NF * CH) {
+      const int il = lane / (2 * NF), f = (lane / 2) % NF, cc = lane & 1;
+      const int l = lc + il;
+      if (l < lend) reinterpret_cast<double*>(out.p[f])[2 * (long)(mo + l - m) + cc] = v / SLX_SQRT2_LEGACY;
+    }
   }
 }
 
 // -------------------------------------------------------------------------------------------
-// Per-order synthesis contraction.  Block (order m, ring block); one thread per northern ring;
-// the order's coefficients are staged in LDS through the Coef loader (which may assemble them
-// from several spectral arrays); E = sum over even l-m, O = sum over odd l-m; outputs
-// FN = 1.41421356 (E + O) (north), FS = 1.41421356 (E - O) (south), order-major [f][m][j].
+// Synthesis contraction.  Block (order pair m1 = p, m2 = L - p; ring block of NT rings): the two
+// orders' coefficients (through the Coef loader, which may assemble them from several spectral
+// arrays) and recurrence coefficients are staged in LDS; each thread owns one northern ring and
+// walks the degree segments of both orders, four segments interleaved (independent recurrences
+// from their seeds), accumulating E (even l - m) and O (odd l - m) per order in a fixed order.
+// Outputs FN = 1.41421356 (E + O) (north), FS = 1.41421356 (E - O) (south), [f][m][j].
 // Coef: double2 operator()(int f, int idx, int l, int m) const, idx = packed index.
 template <int NF, int NT, class Coef>
-__global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, Coef cf, double2* __restrict__ FN,
+__global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN,
                                                 double2* __restrict__ FS, Pred pr) {
   if (!pr.active()) return;
-  extern __shared__ double2 cl[];  // [NF][L+1]
-  const int m = blockIdx.x;
-  const int L = d.lmax, nl = L + 1 - m;
-  const int mo = d.moff[m];
-  for (int i = threadIdx.x; i < nl; i += NT) {
+  extern __shared__ double2 lds[];
+  const int L = d.lmax, nl = L + 1, S = g.S;
+  const int p = blockIdx.x;
+  const int mord[2] = {p, L - p};
+  const int nord = (L - p == p) ? 1 : 2;
+  double2* cl = lds;                 // [2][NF][nl]
+  double2* rc = lds + 2 * NF * nl;   // [2][nl]
+  for (int o = 0; o < nord; ++o) {
+    const int m = mord[o], mo = d.moff[m], cnt = nl - m;
+    for (int i = threadIdx.x; i < cnt; i += NT) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) cl[f * (L + 1) + i] = cf(f, mo + i, m + i, m);
+      for (int f = 0; f < NF; ++f) cl[(o * NF + f) * nl + i] = cf(f, mo + i, m + i, m);
+      rc[o * nl + i] = d.rec[mo + i];
+    }
   }
   __syncthreads();
   const int j = blockIdx.y * NT + threadIdx.x;
   if (j >= d.nh) return;
   const double x = d.x[j];
-  double pa = 0.0, pb = d.pmm[(long)m * d.nh + j];
-  int sc = d.pms[(long)m * d.nh + j];
-  double mk = sc == 0 ? 1.0 : 0.0;
-  double2 E[NF], O[NF];
+  const int nh = d.nh;
+  double2 E[2][NF], O[2][NF];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    double2 c = cl[f * (L + 1)];
-    E[f] = make_double2(pb * mk * c.x, pb * mk * c.y);
-    O[f] = make_double2(0.0, 0.0);
-  }
-  if (nl > 1) {
-    double p = d.s2m3[m] * x * pb;
-    pa = pb;
-    pb = p;
-    p *= mk;
+  for (int o = 0; o < 2; ++o)
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      double2 c = cl[f * (L + 1) + 1];
-      O[f].x += p * c.x;
-      O[f].y += p * c.y;
+    for (int f = 0; f < NF; ++f) E[o][f] = O[o][f] = make_double2(0.0, 0.0);
+  const int ns0 = (nl - mord[0] + S - 1) / S;
+  const int ns1 = nord == 2 ? (nl - mord[1] + S - 1) / S : 0;
+  const int C = ns0 + ns1;
+  for (int c0 = 0; c0 < C; c0 += 4) {
+    LegState st[4];
+    double mk[4];
+    int ord[4], m_[4], len[4];
+    bool head[4];
+    double2 Eq[4][NF], Oq[4][NF];
+    int tmax = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = c0 + q;
+      const bool act = c < C;
+      const int o = (c < ns0) ? 0 : 1;
+      const int s = (c < ns0) ? c : c - ns0;
+      const int m = mord[o];
+      const int seg = g.first[m] + s;
+      const int l0 = m + s * S;
+      ord[q] = o;
+      m_[q] = m;
+      len[q] = act ? min(S, nl - l0) : 0;
+      head[q] = (s == 0);
+      if (act && s == 0) {
+        st[q] = LegState{0.0, d.pmm[(long)m * nh + j], d.pms[(long)m * nh + j]};
+      } else if (act) {
+        st[q] = LegState{g.sa[(long)seg * nh + j], g.sb[(long)seg * nh + j], g.ss[(long)seg * nh + j]};
+      } else {
+        st[q] = LegState{0.0, 0.0, 0};
+      }
+      mk[q] = st[q].sc == 0 ? 1.0 : 0.0;
+      tmax = max(tmax, len[q]);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) Eq[q][f] = Oq[q][f] = make_double2(0.0, 0.0);
+    }
+    for (int t = 0; t < tmax; ++t) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (t < len[q]) {
+          const int o = ord[q];
+          const int i = (c0 + q < ns0 ? c0 + q : c0 + q - ns0) * S + t;  // l - m
+          double pv;
+          if (head[q] && t == 0) {
+            pv = st[q].pb;
+          } else if (head[q] && t == 1) {
+            pv = d.s2m3[m_[q]] * x * st[q].pb;
+            st[q].pa = st[q].pb;
+            st[q].pb = pv;
+          } else {
+            pv = leg_next(x, rc[o * nl + i], st[q]);
+          }
+          pv *= mk[q];
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            const double2 cv = cl[(o * NF + f) * nl + i];
+            if ((t & 1) == 0) {
+              Eq[q][f].x += pv * cv.x;
+              Eq[q][f].y += pv * cv.y;
+            } else {
+              Oq[q][f].x += pv * cv.x;
+              Oq[q][f].y += pv * cv.y;
+            }
+          }
+          if ((t & 7) == 7) {
+            leg_renorm(st[q]);
+            mk[q] = st[q].sc == 0 ? 1.0 : 0.0;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (len[q] > 0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          if (ord[q] == 0) {
+            E[0][f] = cadd(E[0][f], Eq[q][f]);
+            O[0][f] = cadd(O[0][f], Oq[q][f]);
+          } else {
+            E[1][f] = cadd(E[1][f], Eq[q][f]);
+            O[1][f] = cadd(O[1][f], Oq[q][f]);
+          }
+        }
+      }
     }
   }
-  int i = 2;
-  for (; i + 1 < nl; i += 2) {
-    double2 c2 = d.rec[mo + i], c3 = d.rec[mo + i + 1];
-    double p2 = (x * pb - c2.x * pa) * c2.y;
-    double p3 = (x * p2 - c3.x * pb) * c3.y;
-    pa = p2;
-    pb = p3;
-    p2 *= mk;
-    p3 *= mk;
-    if (sc > 0 && fabs(pb) >= 0x1p300) {
-      pa *= 0x1p-600;
-      pb *= 0x1p-600;
-      sc -= 1;
-      mk = sc == 0 ? 1.0 : 0.0;
-    }
+  const long fstride = (long)d.nm * nh;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      double2 ce = cl[f * (L + 1) + i], co = cl[f * (L + 1) + i + 1];
-      E[f].x += p2 * ce.x;
-      E[f].y += p2 * ce.y;
-      O[f].x += p3 * co.x;
-      O[f].y += p3 * co.y;
-    }
-  }
-  if (i < nl) {
-    double2 c2 = d.rec[mo + i];
-    double p2 = (x * pb - c2.x * pa) * c2.y * mk;
+  for (int o = 0; o < 2; ++o) {
+    if (o < nord) {
+      const int m = mord[o];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      double2 ce = cl[f * (L + 1) + i];
-      E[f].x += p2 * ce.x;
-      E[f].y += p2 * ce.y;
+      for (int f = 0; f < NF; ++f) {
+        const long idx = f * fstride + (long)m * nh + j;
+        FN[idx] = make_double2(SLX_SQRT2_LEGACY * (E[o][f].x + O[o][f].x), SLX_SQRT2_LEGACY * (E[o][f].y + O[o][f].y));
+        FS[idx] = make_double2(SLX_SQRT2_LEGACY * (E[o][f].x - O[o][f].x), SLX_SQRT2_LEGACY * (E[o][f].y - O[o][f].y));
+      }
     }
-  }
-  const long fstride = (long)d.nm * d.nh;
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    const long o = f * fstride + (long)m * d.nh + j;
-    FN[o] = make_double2(SLX_SQRT2_LEGACY * (E[f].x + O[f].x), SLX_SQRT2_LEGACY * (E[f].y + O[f].y));
-    FS[o] = make_double2(SLX_SQRT2_LEGACY * (E[f].x - O[f].x), SLX_SQRT2_LEGACY * (E[f].y - O[f].y));
   }
 }
 

@@ -42,26 +42,43 @@ struct StorePlain {
 };
 
 // Launch helpers shared with the solver ------------------------------------------------------
+inline SegDev seg_of(const slx_sht* h) {
+  SegDev g;
+  g.nseg = h->nseg;
+  g.S = h->seg_len;
+  g.seg_m = h->seg_m;
+  g.seg_l0 = h->seg_l0;
+  g.sa = h->seed_a;
+  g.sb = h->seed_b;
+  g.ss = h->seed_s;
+  g.first = h->seg_first;
+  return g;
+}
+
 inline int ana_rings_per_lane(const slx_sht* h) {
-  int r = (h->nh + 64 * h->ana_waves - 1) / (64 * h->ana_waves);
+  const int r = (h->nh + 64 * h->ana_nw - 1) / (64 * h->ana_nw);
   return r <= 1 ? 1 : r <= 2 ? 2 : 4;
 }
 
 template <int NF>
-int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out,
-                   Pred pr, hipStream_t s) {
-  ShtDev d = dev_of(h);
+int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
+                   hipStream_t s) {
+  const ShtDev d = dev_of(h);
+  const SegDev g = seg_of(h);
   const int R = ana_rings_per_lane(h);
-  dim3 grid(h->nm);
-  switch (R) {
-    case 1: k_leg_ana<NF, 1, 2><<<grid, 128, 0, s>>>(d, WS, WD, out, pr); break;
-    case 2: k_leg_ana<NF, 2, 2><<<grid, 128, 0, s>>>(d, WS, WD, out, pr); break;
-    default: k_leg_ana<NF, 4, 2><<<grid, 128, 0, s>>>(d, WS, WD, out, pr); break;
+  dim3 grid(h->nseg);
+  if (h->ana_nw == 1) {
+    switch (R) {
+      case 1: k_leg_ana<NF, 1, 1><<<grid, 64, 0, s>>>(d, g, WS, WD, out, pr); break;
+      case 2: k_leg_ana<NF, 2, 1><<<grid, 64, 0, s>>>(d, g, WS, WD, out, pr); break;
+      default: k_leg_ana<NF, 4, 1><<<grid, 64, 0, s>>>(d, g, WS, WD, out, pr); break;
+    }
+  } else {
+    k_leg_ana<NF, 4, 2><<<grid, 128, 0, s>>>(d, g, WS, WD, out, pr);
   }
   SLX_LAUNCH_CHECK();
   return 0;
 }
-
 }  // namespace slx
 
 namespace slx {
@@ -82,11 +99,12 @@ int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
   return 0;
 }
 template <int NF, class Coef>
-int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s) {
-  constexpr int NT = 128;
-  dim3 grid(h->nm, (h->nh + NT - 1) / NT);
-  size_t lds = (size_t)NF * (h->lmax + 1) * sizeof(double2);
-  k_leg_syn<NF, NT, Coef><<<grid, NT, lds, s>>>(dev_of(h), cf, h->FN, h->FSo, pr);
+int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field = 0) {
+  constexpr int NT = 256;
+  dim3 grid((h->lmax + 2) / 2, (h->nh + NT - 1) / NT);
+  size_t lds = (size_t)2 * (NF + 1) * (h->lmax + 1) * sizeof(double2);
+  const long off = (long)field * h->nm * h->nh;
+  k_leg_syn<NF, NT, Coef><<<grid, NT, lds, s>>>(dev_of(h), seg_of(h), cf, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }

@@ -45,7 +45,6 @@ void slx_set_error(const std::string& msg);
 // moff[m] + (l - m), moff[m] = m (L+1) - m (m-1)/2, complex as double2.
 struct slx_sht {
   int nlat = 0, nlon = 0, lmax = 0, nh = 0, nm = 0, ncoef = 0, log2n = 0, max_fields = 0;
-  int ana_waves = 1;  // waves per order in the analysis contraction
   double* x = nullptr;    // [nh] mu_j of the northern rings (north -> equator)
   double* wn = nullptr;   // [nh] w_j / nlon
   double* ws = nullptr;   // [nh] w_{nlat-1-j} / nlon
@@ -57,6 +56,11 @@ struct slx_sht {
   int* l_of = nullptr;    // [ncoef] degree of each packed index
   int* jb = nullptr;      // [nm] first ring (from the pole) with non-negligible P~_lm, multiple of 64
   double2* tw = nullptr;  // [nlon] exp(-2 pi i t / nlon)
+  // degree segments of the contractions (order m, first degree l0) and recurrence seeds
+  int nseg = 0, seg_len = 0, ana_nw = 1;
+  int *seg_m = nullptr, *seg_l0 = nullptr, *seg_first = nullptr;
+  double *seed_a = nullptr, *seed_b = nullptr;
+  int* seed_s = nullptr;
   double2* WS = nullptr;  // [max_fields][nm][nh] analysis: folded even parts
   double2* WD = nullptr;  // [max_fields][nm][nh] analysis: folded odd parts
   double2* FN = nullptr;  // [max_fields][nm][nh] synthesis: northern Fourier coefficients

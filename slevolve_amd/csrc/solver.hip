@@ -28,7 +28,8 @@ static constexpr double PI_LEGACY = 3.1415926535898;
 // Scalar state of a run.  The status block comes first so it can be copied out as is.
 struct DevState {
   slx_status st;
-  int ice_changed, nflip, counter, pad0;
+  int ice_changed, counter;
+  int nflip[2], nacc2[2];  // grounded-ice flip count / accepted re-guesses, by iteration parity
   double sa, so;
   double p_il[9], p_m[3], p_lam[6];
   double dil[9], dm[3], dlam[6];
@@ -138,150 +139,12 @@ __device__ double sum_partials(const double* partial, int n, double* sh) {
   return block_sum(v, sh);
 }
 
-// ---------------------------------------------------------------------- set-up (step 0)
-__global__ void k_init_grid(Pts p, Consts c, const double* topo0, const double* ice0) {
-  const long n = c.npts;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double t = topo0[i], ice = ice0[i];
-    double cx = t < 0.0 ? 1.0 : 0.0;
-    double g0 = c.checkmarine ? ice * keep_factor(ice, t, c.rhow, c.rhoi) : ice;
-    double b0 = g0 < EPS0 ? 1.0 : 0.0;
-    const_cast<double*>(p.tinit)[i] = t;
-    const_cast<double*>(p.cxy0)[i] = cx;
-    const_cast<double*>(p.beta0)[i] = b0;
-    const_cast<double*>(p.cstar0)[i] = cx * b0;
-    p.cxy[i] = cx;
-    p.topo_prev[i] = t;
-    p.topo[i] = t;
-    p.ice_prev[i] = ice;
-    p.tmp[i] = g0;
-    p.beta[i] = b0;
-    p.icestar[i] = g0;
-  }
-}
-
-// ---------------------------------------------------------------------- step begin
-// degree-2 rotational states (solver.py:407-421) and the per-step resets
-__global__ void k_begin(Pts p, Consts c) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  DevState* s = p.st;
-  double sil[9] = {0}, sm[3] = {0}, stsl[6] = {0}, strr[6] = {0};
-  if (c.L >= 2) {
-    for (int k = 0; k < c.nmod2; ++k) {
-      const double d = p.decay[k * c.nl + 2], o = 1.0 - d;
-      double* r = p.r2 + 18 * k;
-      const double ail = p.a2[k], am = p.a2[c.kmax + k], atsl = p.a2[2 * c.kmax + k], atrr = p.a2[3 * c.kmax + k];
-      for (int q = 0; q < 9; ++q) {
-        r[q] = d * r[q] + o * s->p_il[q];
-        sil[q] += ail * r[q];
-      }
-      for (int q = 0; q < 3; ++q) {
-        r[9 + q] = d * r[9 + q] + o * s->p_m[q];
-        sm[q] += am * r[9 + q];
-      }
-      for (int q = 0; q < 6; ++q) {
-        r[12 + q] = d * r[12 + q] + o * s->p_lam[q];
-        stsl[q] += atsl * r[12 + q];
-        strr[q] += atrr * r[12 + q];
-      }
-    }
-  }
-  for (int q = 0; q < 9; ++q) s->sum_il[q] = sil[q];
-  for (int q = 0; q < 3; ++q) s->sum_m[q] = sm[q];
-  for (int q = 0; q < 6; ++q) {
-    s->sum_tsl[q] = stsl[q];
-    s->sum_trr[q] = strr[q];
-    s->dsl_rot[q] = 0.0;
-    s->rr_rot[q] = 0.0;
-  }
-  for (int q = 0; q < 9; ++q) s->dil[q] = 0.0;
-  for (int q = 0; q < 3; ++q) s->dm[q] = 0.0;
-  for (int q = 0; q < 6; ++q) s->dlam[q] = 0.0;
-  s->st.numiter = 0;
-  s->st.done = 0;
-  s->st.failed = 0;
-  s->st.nacc = 0;
-  s->st.finished = 0;
-  s->st.step += 1;
-  s->ice_changed = 0;
-  s->nflip = 0;
-  s->counter = 0;
-}
-
-// grounded ice on the previous converged topography, beta, C*, tO (solver.py:853-887)
-__global__ void k_ice_grid(Pts p, Consts c) {
-  const long n = c.npts;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double ice = p.ice_cur[i];
-    double ist = c.checkmarine ? ice * keep_factor(ice, p.topo_prev[i], c.rhow, c.rhoi) : ice;
-    double b = ist < EPS0 ? 1.0 : 0.0;
-    double cs = p.cxy[i] * b;
-    p.icestar[i] = ist;
-    p.beta[i] = b;
-    p.cstar[i] = cs;
-    p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
-  }
-}
-
-// ice increments and the first guess of the ocean-load increment (solver.py:857-892;
-// solver_opt.py:504-527); so = sum |load_inc|
-struct GuessArgs {
-  int extrap_ready, uniform;
-  double c1, c2;
-};
-__global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
-  __shared__ double sh[256];
-  const int n = c.ncoef;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double a = 0.0;
-  if (i < n) {
-    const double2 old = p.oldice[i], ist = p.icestar_lm[i];
-    const double2 d = make_double2(ist.x - old.x, ist.y - old.y);
-    const double2 pi = p.p_ice[i];
-    p.dice[i] = d;
-    p.dI[i] = make_double2(pi.x + d.x, pi.y + d.y);
-    double2 ld;
-    if (g.extrap_ready) {
-      const double2 a1 = p.h1[i], a2 = p.h2[i];
-      if (g.uniform)
-        ld = make_double2(a1.x * 2.0 - a2.x, a1.y * 2.0 - a2.y);
-      else
-        ld = make_double2(a1.x * g.c1 - g.c2 * a2.x, a1.y * g.c1 - g.c2 * a2.y);
-      const double2 sp = p.p_ocean[i];
-      p.dS_cur[i] = make_double2(sp.x + ld.x, sp.y + ld.y);
-    } else {
-      // (cstar_lm / cstar_lm[0,0]) * ((-rho_i/rho_w) dice[0,0]); cstar_lm[0,0] and dice[0,0] are real
-      const double2 c00 = p.cstar_lm[0];
-      const double2 d00 = make_double2(p.icestar_lm[0].x - p.oldice[0].x, p.icestar_lm[0].y - p.oldice[0].y);
-      const double scl = 1.0 / c00.x;
-      const double2 q = make_double2(p.cstar_lm[i].x * scl, p.cstar_lm[i].y * scl);
-      const double f = -c.rhoi / c.rhow;
-      const double2 e = make_double2(f * d00.x, f * d00.y);
-      ld = make_double2(q.x * e.x - q.y * e.y, q.x * e.y + q.y * e.x);
-      p.dS_cur[i] = make_double2(0.0, 0.0);
-    }
-    p.load_inc[i] = ld;
-    a = cabs_d(ld);
-  }
-  const double bs = block_sum(a, sh);
-  if (post_partial(bs, p.partial, &p.st->counter)) {
-    const double tot = sum_partials(p.partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      p.st->so = tot;
-      p.st->counter = 0;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------- inner iteration
 // rotation block (solver.py:910-975): products of inertia from the degree-2 load, Liouville
-// equations, centrifugal potential Lambda and the rotational sea-level / radial terms
-__global__ void k_rotation(Pts p, Consts c) {
-  DevState* s = p.st;
-  if (*(volatile int*)&s->st.done) return;
-  if (threadIdx.x != 0 || blockIdx.x != 0 || !c.tpw) return;
-  const double2 dS20 = p.dS_cur[c.idx20], dS21 = p.dS_cur[c.idx21];
-  const double2 dI20 = p.dI[c.idx20], dI21 = p.dI[c.idx21];
+// equations, centrifugal potential Lambda and the rotational sea-level / radial terms, from the
+// (2,0), (2,1) coefficients of the ocean-load increment dS_cur and the ice increment dI.
+__device__ void rotation_update(DevState* s, const Consts& c, double2 dS20, double2 dS21, double2 dI20,
+                                double2 dI21) {
+  if (!c.tpw) return;
   const double z20r = c.rhow * dS20.x + c.rhoi * dI20.x;
   const double z21r = c.rhow * dS21.x + c.rhoi * dI21.x;
   const double z21i = c.rhow * dS21.y + c.rhoi * dI21.y;
@@ -314,6 +177,149 @@ __global__ void k_rotation(Pts p, Consts c) {
   }
 }
 
+__device__ __forceinline__ double2 load2_relaxed(const double2* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return make_double2(load_relaxed(q), load_relaxed(q + 1));
+}
+
+// ---------------------------------------------------------------------- set-up (step 0)
+__global__ void k_init_grid(Pts p, Consts c, const double* topo0, const double* ice0) {
+  const long n = c.npts;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double t = topo0[i], ice = ice0[i];
+    double cx = t < 0.0 ? 1.0 : 0.0;
+    double g0 = c.checkmarine ? ice * keep_factor(ice, t, c.rhow, c.rhoi) : ice;
+    double b0 = g0 < EPS0 ? 1.0 : 0.0;
+    const_cast<double*>(p.tinit)[i] = t;
+    const_cast<double*>(p.cxy0)[i] = cx;
+    const_cast<double*>(p.beta0)[i] = b0;
+    const_cast<double*>(p.cstar0)[i] = cx * b0;
+    p.cxy[i] = cx;
+    p.topo_prev[i] = t;
+    p.topo[i] = t;
+    p.ice_prev[i] = ice;
+    p.tmp[i] = g0;
+    p.beta[i] = b0;
+    p.icestar[i] = g0;
+  }
+}
+
+// ---------------------------------------------------------------------- step begin
+// degree-2 rotational states (solver.py:407-421), one thread per state component (r_il 9,
+// r_m 3, r_lambda 6 reals), and the per-step resets
+__global__ void k_begin(Pts p, Consts c) {
+  if (blockIdx.x != 0) return;
+  DevState* s = p.st;
+  const int q = threadIdx.x;
+  if (q < 18) {
+    const int row = q < 9 ? 0 : (q < 12 ? 1 : 2);
+    const double pv = q < 9 ? s->p_il[q] : (q < 12 ? s->p_m[q - 9] : s->p_lam[q - 12]);
+    double sum1 = 0.0, sum2 = 0.0;
+    if (c.L >= 2) {
+      for (int k = 0; k < c.nmod2; ++k) {
+        const double d = p.decay[k * c.nl + 2], o = 1.0 - d;
+        double* r = p.r2 + 18 * k + q;
+        const double rv = d * *r + o * pv;
+        *r = rv;
+        sum1 += p.a2[row * c.kmax + k] * rv;
+        if (row == 2) sum2 += p.a2[3 * c.kmax + k] * rv;
+      }
+    }
+    if (q < 9) {
+      s->sum_il[q] = sum1;
+      s->dil[q] = 0.0;
+    } else if (q < 12) {
+      s->sum_m[q - 9] = sum1;
+      s->dm[q - 9] = 0.0;
+    } else {
+      s->sum_tsl[q - 12] = sum1;
+      s->sum_trr[q - 12] = sum2;
+      s->dsl_rot[q - 12] = 0.0;
+      s->rr_rot[q - 12] = 0.0;
+      s->dlam[q - 12] = 0.0;
+    }
+  }
+  if (q == 32) {
+    s->st.numiter = 0;
+    s->st.done = 0;
+    s->st.failed = 0;
+    s->st.nacc = 0;
+    s->st.finished = 0;
+    s->st.step += 1;
+    s->ice_changed = 0;
+    s->nflip[0] = s->nflip[1] = 0;
+    s->nacc2[0] = s->nacc2[1] = 0;
+    s->counter = 0;
+  }
+}
+
+// grounded ice on the previous converged topography, beta, C*, tO (solver.py:853-887)
+__global__ void k_ice_grid(Pts p, Consts c) {
+  const long n = c.npts;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double ice = p.ice_cur[i];
+    double ist = c.checkmarine ? ice * keep_factor(ice, p.topo_prev[i], c.rhow, c.rhoi) : ice;
+    double b = ist < EPS0 ? 1.0 : 0.0;
+    double cs = p.cxy[i] * b;
+    p.icestar[i] = ist;
+    p.beta[i] = b;
+    p.cstar[i] = cs;
+    p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
+  }
+}
+
+// ice increments and the first guess of the ocean-load increment (solver.py:857-892;
+// solver_opt.py:504-527); so = sum |load_inc|
+struct GuessArgs {
+  int extrap_ready, uniform;
+  double c1, c2;
+};
+__global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
+  __shared__ double sh[256];
+  const int n = c.ncoef;
+  const double2 c00 = p.cstar_lm[0];
+  const double2 d00 = make_double2(p.icestar_lm[0].x - p.oldice[0].x, p.icestar_lm[0].y - p.oldice[0].y);
+  const double scl = 1.0 / c00.x;
+  const double fr = -c.rhoi / c.rhow;
+  const double2 e = make_double2(fr * d00.x, fr * d00.y);
+  double a = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double2 old = p.oldice[i], ist = p.icestar_lm[i];
+    const double2 d = make_double2(ist.x - old.x, ist.y - old.y);
+    const double2 pi = p.p_ice[i];
+    p.dice[i] = d;
+    p.dI[i] = make_double2(pi.x + d.x, pi.y + d.y);
+    double2 ld;
+    if (g.extrap_ready) {
+      const double2 a1 = p.h1[i], a2 = p.h2[i];
+      if (g.uniform)
+        ld = make_double2(a1.x * 2.0 - a2.x, a1.y * 2.0 - a2.y);
+      else
+        ld = make_double2(a1.x * g.c1 - g.c2 * a2.x, a1.y * g.c1 - g.c2 * a2.y);
+      const double2 sp = p.p_ocean[i];
+      p.dS_cur[i] = make_double2(sp.x + ld.x, sp.y + ld.y);
+    } else {
+      // (cstar_lm / cstar_lm[0,0]) * ((-rho_i/rho_w) dice[0,0]); cstar_lm[0,0] and dice[0,0] are real
+      const double2 q = make_double2(p.cstar_lm[i].x * scl, p.cstar_lm[i].y * scl);
+      ld = make_double2(q.x * e.x - q.y * e.y, q.x * e.y + q.y * e.x);
+      p.dS_cur[i] = make_double2(0.0, 0.0);
+    }
+    p.load_inc[i] = ld;
+    a += cabs_d(ld);
+  }
+  const double bs = block_sum(a, sh);
+  if (post_partial(bs, p.partial, &p.st->counter)) {
+    const double tot = sum_partials(p.partial, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      p.st->so = tot;
+      p.st->counter = 0;
+      rotation_update(p.st, c, load2_relaxed(p.dS_cur + c.idx20), load2_relaxed(p.dS_cur + c.idx21),
+                      load2_relaxed(p.dI + c.idx20), load2_relaxed(p.dI + c.idx21));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------- inner iteration
 // eq. B18 coefficient, assembled while the synthesis stages its coefficients:
 // w1[l] ((rho_i dI + rho_w dS_prev) + rho_w load_inc) + w2[l] resp (+ rot at l = 2, m <= 2)
 struct CoefB18 {
@@ -346,22 +352,22 @@ struct LoadPtrs {
   __device__ double operator()(int f, long i) const { return g[f][i]; }
 };
 
-// conservation constant, ocean-load update and convergence ratio (solver.py:1002-1030)
+// conservation constant, ocean-load update and convergence ratio (solver.py:1002-1030); the last
+// block also evaluates the rotation block of the next iteration
 __global__ void __launch_bounds__(256) k_update(Pts p, Consts c) {
   __shared__ double sh[256];
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
   const int n = c.ncoef;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double c00 = p.cstar_lm[0].x;
   const double conserv = (1.0 / c00) * (-(c.rhoi / c.rhow) * p.dI[0].x - p.rt_lm[0].x);
   double a = 0.0;
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double2 sp = p.p_ocean[i], rt = p.rt_lm[i], cs = p.cstar_lm[i];
-    double2 nw = make_double2((-sp.x + rt.x) + cs.x * conserv, (-sp.y + rt.y) + cs.y * conserv);
+    const double2 nw = make_double2((-sp.x + rt.x) + cs.x * conserv, (-sp.y + rt.y) + cs.y * conserv);
     p.load_inc[i] = nw;
     p.dS_cur[i] = make_double2(sp.x + nw.x, sp.y + nw.y);
-    a = cabs_d(nw);
+    a += cabs_d(nw);
   }
   const double bs = block_sum(a, sh);
   if (post_partial(bs, p.partial, &s->counter)) {
@@ -385,21 +391,27 @@ __global__ void __launch_bounds__(256) k_update(Pts p, Consts c) {
       } else if (s->st.numiter >= MAX_INNER) {
         s->st.failed = 1;
         s->st.done = 1;
+      } else {
+        rotation_update(s, c, load2_relaxed(p.dS_cur + c.idx20), load2_relaxed(p.dS_cur + c.idx21),
+                        load2_relaxed(p.dI + c.idx20), load2_relaxed(p.dI + c.idx21));
       }
       s->counter = 0;
     }
   }
 }
 
-// topography / ocean-function re-guess and the grounded-ice flip count (solver.py:1032-1040,661-684)
-__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c) {
-  __shared__ double sh[256];
+// topography / ocean-function re-guess and the grounded-ice flip count (solver.py:1032-1040,661-684).
+// it = iteration index within the step: the counters of this iteration live in slot it & 1.
+__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it) {
+  __shared__ int shf;
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
+  if (threadIdx.x == 0) shf = 0;
+  __syncthreads();
   const long n = c.npts;
   const double cadd = s->st.conserv * c.K00;
-  const bool regs = c.checkmarine && s->st.nacc < GR_MAXACC;
-  double flips = 0.0;
+  const bool regs = c.checkmarine && s->nacc2[it & 1] < GR_MAXACC;
+  int flips = 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double t = p.tinit[i] - (p.dsl[i] + cadd);
     p.topo[i] = t;
@@ -408,33 +420,22 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c) {
       const double ice = p.ice_cur[i];
       const double cand = ice * keep_factor(ice, t, c.rhow, c.rhoi);
       const bool bnew = cand < EPS0;
-      if (bnew != (p.beta[i] != 0.0)) flips += 1.0;
+      if (bnew != (p.beta[i] != 0.0)) flips += 1;
     }
   }
-  const double bs = block_sum(flips, sh);
-  if (post_partial(bs, p.partial, &s->counter)) {
-    const double tot = sum_partials(p.partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      int changed = 0;
-      if (regs) {
-        s->st.reguess += 1;
-        if (tot > 0.0) {
-          changed = 1;
-          s->st.nacc += 1;
-          s->st.ice_analyses += 1;
-        }
-      }
-      s->ice_changed = changed;
-      s->counter = 0;
-    }
-  }
+  if (flips) atomicAdd(&shf, flips);
+  __syncthreads();
+  if (threadIdx.x == 0 && shf) atomicAdd(&s->nflip[it & 1], shf);
 }
 
-__global__ void k_refresh2(Pts p, Consts c) {
+// apply the re-guess when a grounded flag flipped (at most GR_MAXACC times per step); C*, tO
+__global__ void k_refresh2(Pts p, Consts c, int it) {
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
+  const int slot = it & 1;
+  const bool regs = c.checkmarine && s->nacc2[slot] < GR_MAXACC;
+  const bool changed = regs && s->nflip[slot] > 0;
   const long n = c.npts;
-  const bool changed = s->ice_changed != 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     if (changed) {
       const double ice = p.ice_cur[i];
@@ -446,11 +447,28 @@ __global__ void k_refresh2(Pts p, Consts c) {
     p.cstar[i] = cs;
     p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s->ice_changed = changed ? 1 : 0;
+    s->nacc2[slot ^ 1] = s->nacc2[slot] + (changed ? 1 : 0);
+    s->nflip[slot ^ 1] = 0;
+    if (regs) s->st.reguess += 1;
+    if (changed) {
+      s->st.ice_analyses += 1;
+      s->st.nacc += 1;
+    }
+  }
 }
 
 __global__ void k_dice(Pts p, Consts c) {
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done || !*(volatile int*)&s->ice_changed) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    auto dI_at = [&](int k) {
+      const double2 ist = p.icestar_lm[k], old = p.oldice[k], pi = p.p_ice[k];
+      return make_double2(pi.x + (ist.x - old.x), pi.y + (ist.y - old.y));
+    };
+    rotation_update(s, c, p.dS_cur[c.idx20], p.dS_cur[c.idx21], dI_at(c.idx20), dI_at(c.idx21));
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.ncoef) return;
   const double2 ist = p.icestar_lm[i], old = p.oldice[i], pi = p.p_ice[i];
@@ -518,12 +536,18 @@ __global__ void __launch_bounds__(256) k_post_grid(Pts p, Consts c) {
 }
 
 // degree-0 diagnostics and the rotational commit (solver.py:1060-1110; solver_opt.py:525-547)
-__global__ void k_finish(Pts p, Consts c) {
-  if (!finish_active(p.st) || threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(256) k_finish(Pts p, Consts c) {
+  __shared__ double sh[256];
   DevState* s = p.st;
-  double q[4] = {0, 0, 0, 0};
-  for (int k = 0; k < 4; ++k)
-    for (int j = 0; j < c.nlat; ++j) q[k] += p.w00[j] * p.rows[k * c.nlat + j];
+  if (!finish_active(s)) return;
+  double q[4];
+  for (int k = 0; k < 4; ++k) {
+    double v = 0.0;
+    for (int j = threadIdx.x; j < c.nlat; j += blockDim.x) v += p.w00[j] * p.rows[k * c.nlat + j];
+    q[k] = block_sum(v, sh);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   s->st.gmslc_ocn = q[0] / q[1];
   s->st.gmslc_ocnBeta = q[2] / q[3];
   s->st.ocean_area = q[1] * c.area;
@@ -599,7 +623,12 @@ static Pts make_pts(slx_solver* s, const double* ice_cur, const double2* oldice)
 
 static int grid_blocks(long n) {
   long b = (n + 255) / 256;
-  return (int)(b > 4096 ? 4096 : b);
+  return (int)(b > 2048 ? 2048 : b);
+}
+// blocks of the kernels that end in a last-block reduction: few, so the arrival counter is cheap
+static int red_blocks(long n) {
+  long b = (n + 1023) / 1024;
+  return (int)(b > 128 ? 128 : (b < 1 ? 1 : b));
 }
 
 extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const slx_outputs* o, slx_solver** out) {
@@ -795,7 +824,7 @@ extern "C" int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, 
   const double w = (dt0 + dt1) / (dt1 + dt2);
   g.c1 = dt0 / dt1 * (1.0 + w);
   g.c2 = dt0 / dt2 * w;
-  k_guess<<<(c.ncoef + 255) / 256, 256, 0, st>>>(p, c, g);
+  k_guess<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, g);
   SLX_LAUNCH_CHECK();
   return 0;
 }
@@ -811,9 +840,8 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
   const int nsb = (c.ncoef + 255) / 256;
   for (int it = 0; it < n; ++it) {
     const bool first = (s->iters_enqueued == 0);
+    const int it_step = s->iters_enqueued;
     s->iters_enqueued += 1;
-    k_rotation<<<1, 64, 0, st>>>(p, c);
-    SLX_LAUNCH_CHECK();
     CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->ttlekhl, s->ttl, s->st->dsl_rot, c.rhoi, c.rhow};
     int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
     if (rc) return rc;
@@ -829,11 +857,11 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
         rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
     }
     if (rc) return rc;
-    k_update<<<nsb, 256, 0, st>>>(p, c);
+    k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c);
     SLX_LAUNCH_CHECK();
-    k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c);
+    k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
-    k_refresh2<<<grid_blocks(c.npts), 256, 0, st>>>(p, c);
+    k_refresh2<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
     if (c.checkmarine) {
       rc = launch_fft_ana<256>(s->h, LoadPtrs<1>{{s->icestar}}, 1, Pred{done, 0, changed, 1}, st);
@@ -864,7 +892,7 @@ extern "C" int slx_step_finish(slx_solver* s, void* stream) {
   SLX_LAUNCH_CHECK();
   k_post_grid<<<c.nlat, 256, 0, st>>>(p, c);
   SLX_LAUNCH_CHECK();
-  k_finish<<<1, 64, 0, st>>>(p, c);
+  k_finish<<<1, 256, 0, st>>>(p, c);
   SLX_LAUNCH_CHECK();
   return 0;
 }
