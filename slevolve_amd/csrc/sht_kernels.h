@@ -212,14 +212,53 @@ static __global__ void k_leg_seed(ShtDev d, SegDev g, double* sa, double* sb, in
 
 // -------------------------------------------------------------------------------------------
 // Analysis contraction over one segment of degrees of one order per block (NW waves; lane owns
-// R rings j = lane + 64 (wave + NW r)).  Each chunk of CH degrees gives 64 partial sums per lane
-// (degree x field x re/im) that a reduce-scatter over the 64 lanes turns into one total per lane
-// (lane i holds value i), then a fixed-order sum over the waves in LDS.  out[f] is packed; the
-// result is divided by 1.41421356.
+// R rings j = lane + 64 (wave + NW r)).  Each chunk of CH degrees (a multiple of 8, so parity and
+// renormalisation points are compile-time) gives 64 partial sums per lane (degree x field x
+// re/im); a reduce-scatter over the 64 lanes leaves lane i with the total of value i (the two
+// upper levels with the gfx950 half-wave / row swaps, the four lower ones with xor shuffles),
+// then a fixed-order sum over the waves in LDS.  out[f] is packed; results are / 1.41421356.
 template <int NF>
 struct OutPtrs {
   double2* p[NF];
 };
+
+// acc[i] + acc[i + 32] over lane halves: lanes 0..31 end with value i, 32..63 with value i + 32
+__device__ __forceinline__ double swap_add32(double a, double b) {
+  const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  const double a2 = __longlong_as_double(((unsigned long long)hi[0] << 32) | lo[0]);
+  const double b2 = __longlong_as_double(((unsigned long long)hi[1] << 32) | lo[1]);
+  return a2 + b2;
+}
+// the same across 16-lane rows (bit 4 of the lane)
+__device__ __forceinline__ double swap_add16(double a, double b) {
+  const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  const double a2 = __longlong_as_double(((unsigned long long)hi[0] << 32) | lo[0]);
+  const double b2 = __longlong_as_double(((unsigned long long)hi[1] << 32) | lo[1]);
+  return a2 + b2;
+}
+
+// sum of acc[v] over the wave for v = lane (reduce-scatter, fixed pairing order)
+__device__ __forceinline__ double reduce_scatter64(double* acc, int lane) {
+#pragma unroll
+  for (int i = 0; i < 32; ++i) acc[i] = swap_add32(acc[i], acc[i + 32]);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = swap_add16(acc[i], acc[i + 16]);
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) {
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < o; ++i) {
+      const double send = up ? acc[i] : acc[i + o];
+      const double keep = up ? acc[i + o] : acc[i];
+      acc[i] = keep + __shfl_xor(send, o, 64);
+    }
+  }
+  return acc[0];
+}
 
 template <int NF, int R, int NW>
 __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, SegDev g, const double2* __restrict__ WS,
@@ -242,74 +281,63 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, SegDev g, const d
   for (int r = 0; r < R; ++r) {
     const int j = lane + 64 * (wave + NW * r);
     const bool ok = j < nh;
-    xr[r] = ok ? d.x[j] : 0.0;
-    if (head) {
-      st[r].pa = 0.0;
-      st[r].pb = ok ? d.pmm[(long)m * nh + j] : 0.0;
-      st[r].sc = ok ? d.pms[(long)m * nh + j] : 0;
-    } else {
-      st[r].pa = ok ? g.sa[(long)seg * nh + j] : 0.0;
-      st[r].pb = ok ? g.sb[(long)seg * nh + j] : 0.0;
-      st[r].sc = ok ? g.ss[(long)seg * nh + j] : 0;
-    }
+    const int jj = ok ? j : 0;
+    xr[r] = ok ? d.x[jj] : 0.0;
+    const double a0 = head ? 0.0 : g.sa[(long)seg * nh + jj];
+    const double b0 = head ? d.pmm[(long)m * nh + jj] : g.sb[(long)seg * nh + jj];
+    const int s0 = head ? d.pms[(long)m * nh + jj] : g.ss[(long)seg * nh + jj];
+    st[r] = LegState{ok ? a0 : 0.0, ok ? b0 : 0.0, ok ? s0 : 0};
     mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      S[r][f] = ok ? WS[f * fstride + (long)m * nh + j] : make_double2(0.0, 0.0);
-      D[r][f] = ok ? WD[f * fstride + (long)m * nh + j] : make_double2(0.0, 0.0);
+      S[r][f] = ok ? WS[f * fstride + (long)m * nh + jj] : make_double2(0.0, 0.0);
+      D[r][f] = ok ? WD[f * fstride + (long)m * nh + jj] : make_double2(0.0, 0.0);
     }
   }
   const double s23 = d.s2m3[m];
   const int mo = d.moff[m];
   int buf = 0;
   for (int lc = l0; lc < lend; lc += CH) {
+    const bool hd = (lc == m);
+    double2 cc[CH];
+#pragma unroll
+    for (int il = 0; il < CH; ++il) {
+      const int off = min(max(lc + il - m, 0), L - m);
+      const double2 c = d.rec[mo + off];
+      const bool use = (lc + il >= m + 2) && (lc + il <= L);
+      cc[il] = use ? c : make_double2(0.0, 0.0);
+    }
     double acc[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = 0.0;
 #pragma unroll
     for (int il = 0; il < CH; ++il) {
-      const int l = lc + il;
-      if (l < lend) {
-        double2 c = make_double2(0.0, 0.0);
-        if (l > m + 1) c = d.rec[mo + (l - m)];
-        const bool even = ((l - m) & 1) == 0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          double p;
-          if (l == m) {
-            p = st[r].pb;
-          } else if (l == m + 1) {
-            p = s23 * xr[r] * st[r].pb;
-            st[r].pa = st[r].pb;
-            st[r].pb = p;
-          } else {
-            p = leg_next(xr[r], c, st[r]);
-          }
-          p *= mk[r];
+      for (int r = 0; r < R; ++r) {
+        double p;
+        if (il == 0 && hd) {
+          p = st[r].pb;
+        } else if (il == 1 && hd) {
+          p = s23 * xr[r] * st[r].pb;
+          st[r].pa = st[r].pb;
+          st[r].pb = p;
+        } else {
+          p = leg_next(xr[r], cc[il], st[r]);
+        }
+        p *= mk[r];
 #pragma unroll
-          for (int f = 0; f < NF; ++f) {
-            const double2 v = even ? S[r][f] : D[r][f];
-            acc[(il * NF + f) * 2 + 0] += p * v.x;
-            acc[(il * NF + f) * 2 + 1] += p * v.y;
-          }
-          if ((il & 7) == 7) {
-            leg_renorm(st[r]);
-            mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
-          }
+        for (int f = 0; f < NF; ++f) {
+          const double2 v = (il & 1) == 0 ? S[r][f] : D[r][f];
+          acc[(il * NF + f) * 2 + 0] += p * v.x;
+          acc[(il * NF + f) * 2 + 1] += p * v.y;
+        }
+        if ((il & 7) == 7) {
+          leg_renorm(st[r]);
+          mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
         }
       }
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-      const bool up = (lane & o) != 0;
-#pragma unroll
-      for (int i = 0; i < o; ++i) {
-        const double send = up ? acc[i] : acc[i + o];
-        const double keep = up ? acc[i + o] : acc[i];
-        acc[i] = keep + __shfl_xor(send, o, 64);
-      }
-    }
-    double v = acc[0];
+    double v = reduce_scatter64(acc, lane);
     if (NW > 1) {
       red[buf][wave][lane] = v;
       __syncthreads();
@@ -319,9 +347,9 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, SegDev g, const d
       buf ^= 1;
     }
     if (wave == 0 && lane < 2 * NF * CH) {
-      const int il = lane / (2 * NF), f = (lane / 2) % NF, cc = lane & 1;
+      const int il = lane / (2 * NF), f = (lane / 2) % NF, c2 = lane & 1;
       const int l = lc + il;
-      if (l < lend) reinterpret_cast<double*>(out.p[f])[2 * (long)(mo + l - m) + cc] = v / SLX_SQRT2_LEGACY;
+      if (l < lend) reinterpret_cast<double*>(out.p[f])[2 * (long)(mo + l - m) + c2] = v / SLX_SQRT2_LEGACY;
     }
   }
 }
@@ -329,28 +357,80 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, SegDev g, const d
 // -------------------------------------------------------------------------------------------
 // Synthesis contraction.  Block (order pair m1 = p, m2 = L - p; ring block of NT rings): the two
 // orders' coefficients (through the Coef loader, which may assemble them from several spectral
-// arrays) and recurrence coefficients are staged in LDS; each thread owns one northern ring and
-// walks the degree segments of both orders, four segments interleaved (independent recurrences
-// from their seeds), accumulating E (even l - m) and O (odd l - m) per order in a fixed order.
-// Outputs FN = 1.41421356 (E + O) (north), FS = 1.41421356 (E - O) (south), [f][m][j].
-// Coef: double2 operator()(int f, int idx, int l, int m) const, idx = packed index.
+// arrays) and recurrence coefficients are staged in LDS, zero-padded by 8 past the last degree;
+// each thread owns one northern ring and walks the degree segments of both orders, four segments
+// interleaved (independent recurrences from their seeds), in blocks of 8 degrees so that parity
+// and renormalisation points are compile-time; E (even l - m) and O (odd l - m) are accumulated
+// per order in a fixed order.  Outputs FN = 1.41421356 (E + O) (north), FS = 1.41421356 (E - O)
+// (south), [f][m][j].  Coef: double2 operator()(int f, int idx, int l, int m) const.
+template <int NF>
+struct SynChains {
+  LegState st[4];
+  double mk[4];
+  int o[4], base[4], len8[4];
+  bool head[4];
+  double2 E[4][NF], O[4][NF];
+};
+
+template <int NF, bool FIRST>
+__device__ __forceinline__ void syn_block8(SynChains<NF>& ch, int t0, double x, const double* s23, const double2* cl,
+                                           const double2* rc, int nlp) {
+#pragma unroll
+  for (int il = 0; il < 8; ++il) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (t0 < ch.len8[q]) {
+        const int o = ch.o[q];
+        const int i = ch.base[q] + t0 + il;
+        double p;
+        if (FIRST && il == 0 && ch.head[q]) {
+          p = ch.st[q].pb;
+        } else if (FIRST && il == 1 && ch.head[q]) {
+          p = s23[o] * x * ch.st[q].pb;
+          ch.st[q].pa = ch.st[q].pb;
+          ch.st[q].pb = p;
+        } else {
+          p = leg_next(x, rc[o * nlp + i], ch.st[q]);
+        }
+        p *= ch.mk[q];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          const double2 cv = cl[(o * NF + f) * nlp + i];
+          if ((il & 1) == 0) {
+            ch.E[q][f].x += p * cv.x;
+            ch.E[q][f].y += p * cv.y;
+          } else {
+            ch.O[q][f].x += p * cv.x;
+            ch.O[q][f].y += p * cv.y;
+          }
+        }
+        if (il == 7) {
+          leg_renorm(ch.st[q]);
+          ch.mk[q] = ch.st[q].sc == 0 ? 1.0 : 0.0;
+        }
+      }
+    }
+  }
+}
+
 template <int NF, int NT, class Coef>
 __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN,
                                                 double2* __restrict__ FS, Pred pr) {
   if (!pr.active()) return;
   extern __shared__ double2 lds[];
-  const int L = d.lmax, nl = L + 1, S = g.S;
+  const int L = d.lmax, nl = L + 1, S = g.S, nlp = nl + 8;
   const int p = blockIdx.x;
   const int mord[2] = {p, L - p};
   const int nord = (L - p == p) ? 1 : 2;
-  double2* cl = lds;                 // [2][NF][nl]
-  double2* rc = lds + 2 * NF * nl;   // [2][nl]
-  for (int o = 0; o < nord; ++o) {
+  double2* cl = lds;                 // [2][NF][nlp]
+  double2* rc = lds + 2 * NF * nlp;  // [2][nlp]
+  for (int o = 0; o < 2; ++o) {
     const int m = mord[o], mo = d.moff[m], cnt = nl - m;
-    for (int i = threadIdx.x; i < cnt; i += NT) {
+    for (int i = threadIdx.x; i < nlp; i += NT) {
+      const bool v = (o < nord) && (i < cnt);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) cl[(o * NF + f) * nl + i] = cf(f, mo + i, m + i, m);
-      rc[o * nl + i] = d.rec[mo + i];
+      for (int f = 0; f < NF; ++f) cl[(o * NF + f) * nlp + i] = v ? cf(f, mo + i, m + i, m) : make_double2(0.0, 0.0);
+      rc[o * nlp + i] = v ? d.rec[mo + i] : make_double2(0.0, 0.0);
     }
   }
   __syncthreads();
@@ -358,6 +438,7 @@ __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, SegDev g, Coef cf, dou
   if (j >= d.nh) return;
   const double x = d.x[j];
   const int nh = d.nh;
+  const double s23[2] = {d.s2m3[mord[0]], d.s2m3[mord[1]]};
   double2 E[2][NF], O[2][NF];
 #pragma unroll
   for (int o = 0; o < 2; ++o)
@@ -367,11 +448,7 @@ __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, SegDev g, Coef cf, dou
   const int ns1 = nord == 2 ? (nl - mord[1] + S - 1) / S : 0;
   const int C = ns0 + ns1;
   for (int c0 = 0; c0 < C; c0 += 4) {
-    LegState st[4];
-    double mk[4];
-    int ord[4], m_[4], len[4];
-    bool head[4];
-    double2 Eq[4][NF], Oq[4][NF];
+    SynChains<NF> ch;
     int tmax = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -380,70 +457,39 @@ __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, SegDev g, Coef cf, dou
       const int o = (c < ns0) ? 0 : 1;
       const int s = (c < ns0) ? c : c - ns0;
       const int m = mord[o];
-      const int seg = g.first[m] + s;
+      const int seg = act ? g.first[m] + s : 0;
       const int l0 = m + s * S;
-      ord[q] = o;
-      m_[q] = m;
-      len[q] = act ? min(S, nl - l0) : 0;
-      head[q] = (s == 0);
+      const int len = act ? min(S, nl - l0) : 0;
+      ch.o[q] = o;
+      ch.base[q] = s * S;
+      ch.len8[q] = (len + 7) & ~7;
+      ch.head[q] = act && (s == 0);
+      const long ix = (long)m * nh + j, sx = (long)seg * nh + j;
       if (act && s == 0) {
-        st[q] = LegState{0.0, d.pmm[(long)m * nh + j], d.pms[(long)m * nh + j]};
+        ch.st[q] = LegState{0.0, d.pmm[ix], d.pms[ix]};
       } else if (act) {
-        st[q] = LegState{g.sa[(long)seg * nh + j], g.sb[(long)seg * nh + j], g.ss[(long)seg * nh + j]};
+        ch.st[q] = LegState{g.sa[sx], g.sb[sx], g.ss[sx]};
       } else {
-        st[q] = LegState{0.0, 0.0, 0};
+        ch.st[q] = LegState{0.0, 0.0, 0};
       }
-      mk[q] = st[q].sc == 0 ? 1.0 : 0.0;
-      tmax = max(tmax, len[q]);
+      ch.mk[q] = ch.st[q].sc == 0 ? 1.0 : 0.0;
+      tmax = max(tmax, ch.len8[q]);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) Eq[q][f] = Oq[q][f] = make_double2(0.0, 0.0);
+      for (int f = 0; f < NF; ++f) ch.E[q][f] = ch.O[q][f] = make_double2(0.0, 0.0);
     }
-    for (int t = 0; t < tmax; ++t) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (t < len[q]) {
-          const int o = ord[q];
-          const int i = (c0 + q < ns0 ? c0 + q : c0 + q - ns0) * S + t;  // l - m
-          double pv;
-          if (head[q] && t == 0) {
-            pv = st[q].pb;
-          } else if (head[q] && t == 1) {
-            pv = d.s2m3[m_[q]] * x * st[q].pb;
-            st[q].pa = st[q].pb;
-            st[q].pb = pv;
-          } else {
-            pv = leg_next(x, rc[o * nl + i], st[q]);
-          }
-          pv *= mk[q];
-#pragma unroll
-          for (int f = 0; f < NF; ++f) {
-            const double2 cv = cl[(o * NF + f) * nl + i];
-            if ((t & 1) == 0) {
-              Eq[q][f].x += pv * cv.x;
-              Eq[q][f].y += pv * cv.y;
-            } else {
-              Oq[q][f].x += pv * cv.x;
-              Oq[q][f].y += pv * cv.y;
-            }
-          }
-          if ((t & 7) == 7) {
-            leg_renorm(st[q]);
-            mk[q] = st[q].sc == 0 ? 1.0 : 0.0;
-          }
-        }
-      }
-    }
+    syn_block8<NF, true>(ch, 0, x, s23, cl, rc, nlp);
+    for (int t0 = 8; t0 < tmax; t0 += 8) syn_block8<NF, false>(ch, t0, x, s23, cl, rc, nlp);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (len[q] > 0) {
+      if (ch.len8[q] > 0) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-          if (ord[q] == 0) {
-            E[0][f] = cadd(E[0][f], Eq[q][f]);
-            O[0][f] = cadd(O[0][f], Oq[q][f]);
+          if (ch.o[q] == 0) {
+            E[0][f] = cadd(E[0][f], ch.E[q][f]);
+            O[0][f] = cadd(O[0][f], ch.O[q][f]);
           } else {
-            E[1][f] = cadd(E[1][f], Eq[q][f]);
-            O[1][f] = cadd(O[1][f], Oq[q][f]);
+            E[1][f] = cadd(E[1][f], ch.E[q][f]);
+            O[1][f] = cadd(O[1][f], ch.O[q][f]);
           }
         }
       }

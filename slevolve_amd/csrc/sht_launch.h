@@ -102,7 +102,7 @@ template <int NF, class Coef>
 int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field = 0) {
   constexpr int NT = 256;
   dim3 grid((h->lmax + 2) / 2, (h->nh + NT - 1) / NT);
-  size_t lds = (size_t)2 * (NF + 1) * (h->lmax + 1) * sizeof(double2);
+  size_t lds = (size_t)2 * (NF + 1) * (h->lmax + 9) * sizeof(double2);
   const long off = (long)field * h->nm * h->nh;
   k_leg_syn<NF, NT, Coef><<<grid, NT, lds, s>>>(dev_of(h), seg_of(h), cf, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();
