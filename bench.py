@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--workload", default="lmax512", choices=("lmax63", "lmax512", "lmax1023"))
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the dominant kernel (20x) and a 256 MiB device copy (for PMC counter runs)")
     return ap.parse_args()
 
 
@@ -82,6 +84,21 @@ def main():
     from slevolve_amd import synthetic as syn
 
     w = syn.workload(a.workload)
+    if a.kernels_only:
+        dev = torch.device("cuda", 0)
+        sht = SHT(w.nlat, w.nlon, w.lmax, device=dev)
+        grids = torch.from_numpy(np.stack([w.topo, w.ice(1)])).to(dev)
+        coef = torch.zeros((2, sht.ncoef), dtype=torch.complex128, device=dev)
+        sht.analysis_packed(grids, coef)
+        for _ in range(20):
+            sht.analysis_contraction(2, coef)
+        src = torch.ones(32 * 2**20, dtype=torch.float64, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernels_only": True, "copy_bytes": src.numel() * 8}))
+        return
     steps = a.steps if a.steps is not None else w.nsteps - a.warmup
     nsteps = a.warmup + steps
     if nsteps > w.nsteps:

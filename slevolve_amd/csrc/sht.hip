@@ -147,7 +147,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   ALLOC_COPY(h->jb, jb);
   ALLOC_COPY(h->tw, tw);
   // degree segments: one analysis block per (order, segment); S (a multiple of 8) is the
-  // largest of 256, 128, 64, 32 that still gives >= 2048 waves
+  // largest of 256, 128, 64, 32 that still gives >= 2048 waves (with one wave per segment)
   h->ana_nw = (nh > 256) ? 2 : 1;
   auto count = [&](int S) {
     long n = 0;
@@ -186,6 +186,43 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   SLX_HIP(hipMalloc(&h->WD, wsz));
   SLX_HIP(hipMalloc(&h->FN, wsz));
   SLX_HIP(hipMalloc(&h->FSo, wsz));
+  SLX_HIP(hipMemset(h->WS, 0, wsz));
+  SLX_HIP(hipMemset(h->WD, 0, wsz));
+  // autotune: waves per segment of the 2-field analysis contraction (1 or 2, rings per lane
+  // <= 4), timed on the device with the handle's own shapes
+  {
+    double2* tmp = nullptr;
+    SLX_HIP(hipMalloc(&tmp, 2 * (size_t)h->ncoef * sizeof(double2)));
+    hipEvent_t e0, e1;
+    SLX_HIP(hipEventCreate(&e0));
+    SLX_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_nw = h->ana_nw;
+    for (int nw = 1; nw <= 2; ++nw) {
+      if ((nh + 64 * nw - 1) / (64 * nw) > 4) continue;
+      h->ana_nw = nw;
+      OutPtrs<2> o2{{tmp, tmp + h->ncoef}};
+      int rc = launch_leg_ana<2>(h, h->WS, h->WD, o2, Pred{}, nullptr);
+      if (rc) return rc;
+      SLX_HIP(hipEventRecord(e0, nullptr));
+      for (int rep = 0; rep < 3; ++rep) {
+        rc = launch_leg_ana<2>(h, h->WS, h->WD, o2, Pred{}, nullptr);
+        if (rc) return rc;
+      }
+      SLX_HIP(hipEventRecord(e1, nullptr));
+      SLX_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SLX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_nw = nw;
+      }
+    }
+    h->ana_nw = best_nw;
+    SLX_HIP(hipEventDestroy(e0));
+    SLX_HIP(hipEventDestroy(e1));
+    SLX_HIP(hipFree(tmp));
+  }
   *out = h;
   return 0;
 }

@@ -241,22 +241,28 @@ __device__ __forceinline__ double swap_add16(double a, double b) {
   return a2 + b2;
 }
 
+// one lower level of the reduce-scatter: pairs (i, i + O) across lanes differing in bit O
+template <int O>
+__device__ __forceinline__ void xor_level(double (&acc)[64], int lane) {
+  const bool up = (lane & O) != 0;
+#pragma unroll
+  for (int i = 0; i < O; ++i) {
+    const double send = up ? acc[i] : acc[i + O];
+    const double keep = up ? acc[i + O] : acc[i];
+    acc[i] = keep + __shfl_xor(send, O, 64);
+  }
+}
+
 // sum of acc[v] over the wave for v = lane (reduce-scatter, fixed pairing order)
-__device__ __forceinline__ double reduce_scatter64(double* acc, int lane) {
+__device__ __forceinline__ double reduce_scatter64(double (&acc)[64], int lane) {
 #pragma unroll
   for (int i = 0; i < 32; ++i) acc[i] = swap_add32(acc[i], acc[i + 32]);
 #pragma unroll
   for (int i = 0; i < 16; ++i) acc[i] = swap_add16(acc[i], acc[i + 16]);
-#pragma unroll
-  for (int o = 8; o >= 1; o >>= 1) {
-    const bool up = (lane & o) != 0;
-#pragma unroll
-    for (int i = 0; i < o; ++i) {
-      const double send = up ? acc[i] : acc[i + o];
-      const double keep = up ? acc[i + o] : acc[i];
-      acc[i] = keep + __shfl_xor(send, o, 64);
-    }
-  }
+  xor_level<8>(acc, lane);
+  xor_level<4>(acc, lane);
+  xor_level<2>(acc, lane);
+  xor_level<1>(acc, lane);
   return acc[0];
 }
 

@@ -74,7 +74,11 @@ int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPt
       default: k_leg_ana<NF, 4, 1><<<grid, 64, 0, s>>>(d, g, WS, WD, out, pr); break;
     }
   } else {
-    k_leg_ana<NF, 4, 2><<<grid, 128, 0, s>>>(d, g, WS, WD, out, pr);
+    switch (R) {
+      case 1: k_leg_ana<NF, 1, 2><<<grid, 128, 0, s>>>(d, g, WS, WD, out, pr); break;
+      case 2: k_leg_ana<NF, 2, 2><<<grid, 128, 0, s>>>(d, g, WS, WD, out, pr); break;
+      default: k_leg_ana<NF, 4, 2><<<grid, 128, 0, s>>>(d, g, WS, WD, out, pr); break;
+    }
   }
   SLX_LAUNCH_CHECK();
   return 0;

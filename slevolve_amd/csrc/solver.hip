@@ -352,6 +352,70 @@ struct LoadPtrs {
   __device__ double operator()(int f, long i) const { return g[f][i]; }
 };
 
+// Ring-pair pass of an inner iteration: the synthesised Fourier coefficients of eq. B18 are
+// inverse-transformed (one complex FFT for rows j and nlat-1-j), written as dslxy, multiplied by
+// C* and reduced by tO in LDS (rO - tO, solver.py:993-996), forward-transformed and folded into
+// analysis field 0; with nf == 2 the ocean function C* of the same rows is transformed into
+// analysis field 1 as well.  Replaces separate synthesis and analysis ring passes.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __restrict__ FN,
+                                                   const double2* __restrict__ FS, double* __restrict__ dsl,
+                                                   const double* __restrict__ cstar, const double* __restrict__ tO,
+                                                   double2* __restrict__ WS, double2* __restrict__ WD, int nf,
+                                                   Pred pr) {
+  if (!pr.active()) return;
+  extern __shared__ double2 smem[];
+  const int N = d.nlon, L = d.lmax;
+  double2* a = smem;
+  double2* b = smem + N;
+  const int j = blockIdx.x, js = d.nlat - 1 - j;
+  const long rn = (long)j * N, rs = (long)js * N;
+  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int m = threadIdx.x; m <= L; m += NT) {
+    const double2 fn = FN[(long)m * d.nh + j], fs = FS[(long)m * d.nh + j];
+    if (m == 0 || 2 * m == N) {
+      a[m] = make_double2(fn.x, fs.x);
+    } else {
+      a[m] = make_double2(fn.x - fs.y, fn.y + fs.x);
+      a[N - m] = make_double2(fn.x + fs.y, -fn.y + fs.x);
+    }
+  }
+  __syncthreads();
+  double2* z = fft_lds<NT>(a, b, d.tw, N, d.log2n, true);
+  double2* w = (z == a) ? b : a;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    const double2 v = z[i];
+    dsl[rn + i] = v.x;
+    dsl[rs + i] = v.y;
+    z[i] = make_double2(v.x * cstar[rn + i] - tO[rn + i], v.y * cstar[rs + i] - tO[rs + i]);
+  }
+  __syncthreads();
+  const double wn = d.wn[j], ws = d.ws[j];
+  const long fstride = (long)d.nm * d.nh;
+  for (int f = 0; f < nf; ++f) {
+    if (f == 1) {
+      for (int i = threadIdx.x; i < N; i += NT) z[i] = make_double2(cstar[rn + i], cstar[rs + i]);
+      __syncthreads();
+    }
+    double2* Z = fft_lds<NT>(z, w, d.tw, N, d.log2n, false);
+    for (int m = threadIdx.x; m <= L; m += NT) {
+      const double2 zm = Z[m], zc = Z[(N - m) & (N - 1)];
+      const double2 fn = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y - zc.y));
+      const double2 fs = make_double2(0.5 * (zm.y + zc.y), -0.5 * (zm.x - zc.x));
+      const double2 tn = make_double2(fn.x * wn, fn.y * wn), ts = make_double2(fs.x * ws, fs.y * ws);
+      WS[f * fstride + (long)m * d.nh + j] = cadd(tn, ts);
+      WD[f * fstride + (long)m * d.nh + j] = csub(tn, ts);
+    }
+    __syncthreads();
+    if (Z != z) {  // keep z as the buffer the next field is loaded into
+      double2* t = z;
+      z = w;
+      w = t;
+    }
+  }
+}
+
 // conservation constant, ocean-load update and convergence ratio (solver.py:1002-1030); the last
 // block also evaluates the rotation block of the next iteration
 __global__ void __launch_bounds__(256) k_update(Pts p, Consts c) {
@@ -845,17 +909,17 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->ttlekhl, s->ttl, s->st->dsl_rot, c.rhoi, c.rhow};
     int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
     if (rc) return rc;
-    rc = launch_fft_syn<256>(s->h, StorePlain{s->dsl, 0}, 1, Pred{done, 0}, st);
-    if (rc) return rc;
-    LoadRT lr{s->dsl, s->cstar, s->tO};
-    if (first) {
-      rc = launch_fft_ana<256>(s->h, lr, 1, Pred{done, 0}, st);
-      if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
-    } else {
-      rc = launch_fft_ana<256>(s->h, lr, 2, Pred{done, 0}, st);
-      if (!rc)
-        rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
+    {
+      const int nf = first ? 1 : 2;
+      const size_t lds = 2 * (size_t)c.nlon * sizeof(double2);
+      k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl, s->cstar, s->tO,
+                                                     s->h->WS, s->h->WD, nf, Pred{done, 0});
+      SLX_LAUNCH_CHECK();
     }
+    if (first)
+      rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
+    else
+      rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
     if (rc) return rc;
     k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c);
     SLX_LAUNCH_CHECK();
