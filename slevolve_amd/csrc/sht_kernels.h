@@ -163,11 +163,11 @@ __device__ __forceinline__ double leg_next(double x, double2 c, LegState& s) {
   return p;
 }
 __device__ __forceinline__ void leg_renorm(LegState& s) {
-  if (s.sc > 0 && fabs(s.pb) >= 0x1p300) {
-    s.pa *= 0x1p-600;
-    s.pb *= 0x1p-600;
-    s.sc -= 1;
-  }
+  const bool big = (s.sc > 0) && (fabs(s.pb) >= 0x1p300);
+  const double f = big ? 0x1p-600 : 1.0;
+  s.pa *= f;
+  s.pb *= f;
+  s.sc -= big ? 1 : 0;
 }
 
 // Segments of the degree range: order m, first degree l0 (segments start every S degrees from
@@ -321,11 +321,16 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, SegDev g, const d
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         double p;
-        if (il == 0 && hd) {
-          p = st[r].pb;
-        } else if (il == 1 && hd) {
-          p = s23 * xr[r] * st[r].pb;
-          st[r].pa = st[r].pb;
+        if (il == 0) {  // head: P_m itself, the state does not move
+          const double pa0 = st[r].pa, pb0 = st[r].pb;
+          const double pn = leg_next(xr[r], cc[il], st[r]);
+          p = hd ? pb0 : pn;
+          st[r].pa = hd ? pa0 : st[r].pa;
+          st[r].pb = hd ? pb0 : st[r].pb;
+        } else if (il == 1) {  // head: P_{m+1} = sqrt(2m+3) x P_m
+          const double ps = s23 * xr[r] * st[r].pb;
+          const double pn = leg_next(xr[r], cc[il], st[r]);
+          p = hd ? ps : pn;
           st[r].pb = p;
         } else {
           p = leg_next(xr[r], cc[il], st[r]);
@@ -373,7 +378,7 @@ template <int NF>
 struct SynChains {
   LegState st[4];
   double mk[4];
-  int o[4], base[4], len8[4];
+  int o[4], base[4], len8[4], zpad[4];
   bool head[4];
   double2 E[4][NF], O[4][NF];
 };
@@ -381,39 +386,48 @@ struct SynChains {
 template <int NF, bool FIRST>
 __device__ __forceinline__ void syn_block8(SynChains<NF>& ch, int t0, double x, const double* s23, const double2* cl,
                                            const double2* rc, int nlp) {
+  int ib[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // past a chain's end read the zero padding of its order (contributes nothing)
+    ib[q] = ch.o[q] * nlp + (t0 < ch.len8[q] ? ch.base[q] + t0 : ch.zpad[q]);
+  }
 #pragma unroll
   for (int il = 0; il < 8; ++il) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (t0 < ch.len8[q]) {
-        const int o = ch.o[q];
-        const int i = ch.base[q] + t0 + il;
-        double p;
-        if (FIRST && il == 0 && ch.head[q]) {
-          p = ch.st[q].pb;
-        } else if (FIRST && il == 1 && ch.head[q]) {
-          p = s23[o] * x * ch.st[q].pb;
-          ch.st[q].pa = ch.st[q].pb;
-          ch.st[q].pb = p;
-        } else {
-          p = leg_next(x, rc[o * nlp + i], ch.st[q]);
-        }
-        p *= ch.mk[q];
+      const int i = ib[q] + il;
+      double p;
+      if (FIRST && il == 0) {
+        const double pa0 = ch.st[q].pa, pb0 = ch.st[q].pb;
+        const double pn = leg_next(x, rc[i], ch.st[q]);
+        const bool h = ch.head[q];
+        p = h ? pb0 : pn;
+        ch.st[q].pa = h ? pa0 : ch.st[q].pa;
+        ch.st[q].pb = h ? pb0 : ch.st[q].pb;
+      } else if (FIRST && il == 1) {
+        const double ps = s23[ch.o[q]] * x * ch.st[q].pb;
+        const double pn = leg_next(x, rc[i], ch.st[q]);
+        p = ch.head[q] ? ps : pn;
+        ch.st[q].pb = p;
+      } else {
+        p = leg_next(x, rc[i], ch.st[q]);
+      }
+      p *= ch.mk[q];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          const double2 cv = cl[(o * NF + f) * nlp + i];
-          if ((il & 1) == 0) {
-            ch.E[q][f].x += p * cv.x;
-            ch.E[q][f].y += p * cv.y;
-          } else {
-            ch.O[q][f].x += p * cv.x;
-            ch.O[q][f].y += p * cv.y;
-          }
+      for (int f = 0; f < NF; ++f) {
+        const double2 cv = cl[i + f * nlp + ch.o[q] * (NF - 1) * nlp];
+        if ((il & 1) == 0) {
+          ch.E[q][f].x += p * cv.x;
+          ch.E[q][f].y += p * cv.y;
+        } else {
+          ch.O[q][f].x += p * cv.x;
+          ch.O[q][f].y += p * cv.y;
         }
-        if (il == 7) {
-          leg_renorm(ch.st[q]);
-          ch.mk[q] = ch.st[q].sc == 0 ? 1.0 : 0.0;
-        }
+      }
+      if (il == 7) {
+        leg_renorm(ch.st[q]);
+        ch.mk[q] = ch.st[q].sc == 0 ? 1.0 : 0.0;
       }
     }
   }
@@ -430,12 +444,16 @@ __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, SegDev g, Coef cf, dou
   const int nord = (L - p == p) ? 1 : 2;
   double2* cl = lds;                 // [2][NF][nlp]
   double2* rc = lds + 2 * NF * nlp;  // [2][nlp]
+  __shared__ double pro[32];  // scratch of the coefficient loader's prologue
+  cf.prologue(pro);
+  __syncthreads();
   for (int o = 0; o < 2; ++o) {
     const int m = mord[o], mo = d.moff[m], cnt = nl - m;
     for (int i = threadIdx.x; i < nlp; i += NT) {
       const bool v = (o < nord) && (i < cnt);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) cl[(o * NF + f) * nlp + i] = v ? cf(f, mo + i, m + i, m) : make_double2(0.0, 0.0);
+      for (int f = 0; f < NF; ++f)
+        cl[(o * NF + f) * nlp + i] = v ? cf(f, mo + i, m + i, m, pro) : make_double2(0.0, 0.0);
       rc[o * nlp + i] = v ? d.rec[mo + i] : make_double2(0.0, 0.0);
     }
   }
@@ -469,6 +487,7 @@ __global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, SegDev g, Coef cf, dou
       ch.o[q] = o;
       ch.base[q] = s * S;
       ch.len8[q] = (len + 7) & ~7;
+      ch.zpad[q] = nl - m;  // first zero entry of the order's padded rows
       ch.head[q] = act && (s == 0);
       const long ix = (long)m * nh + j, sx = (long)seg * nh + j;
       if (act && s == 0) {

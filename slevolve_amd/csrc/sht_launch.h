@@ -33,7 +33,8 @@ struct LoadPlain {
 struct CoefPlain {
   const double2* c;
   long fstride;
-  __device__ double2 operator()(int f, int idx, int, int) const { return c[f * fstride + idx]; }
+  __device__ void prologue(double*) const {}
+  __device__ double2 operator()(int f, int idx, int, int, const double*) const { return c[f * fstride + idx]; }
 };
 struct StorePlain {
   double* g;

@@ -24,6 +24,7 @@ static constexpr double EPS0 = 2.220446049250313e-16;
 static constexpr int GR_MAXACC = 3;
 static constexpr int MAX_INNER = 9999;
 static constexpr double PI_LEGACY = 3.1415926535898;
+static constexpr int PSLOT = 1024;  // block-partial slots per reduction (3 of them: 2 iteration parities + guess)
 
 // Scalar state of a run.  The status block comes first so it can be copied out as is.
 struct DevState {
@@ -114,72 +115,62 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return sh[0];
 }
 
-__device__ __forceinline__ double load_relaxed(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Last-block-done: every block posts its partial; the block that arrives last returns true
-// (and has all partials visible).  counter is reset by that block.
-__device__ bool post_partial(double v, double* partial, int* counter) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partial[blockIdx.x], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    int t = atomicAdd(counter, 1);
-    last = (t == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-__device__ double sum_partials(const double* partial, int n, double* sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v += load_relaxed(&partial[i]);
-  return block_sum(v, sh);
-}
-
 // rotation block (solver.py:910-975): products of inertia from the degree-2 load, Liouville
 // equations, centrifugal potential Lambda and the rotational sea-level / radial terms, from the
 // (2,0), (2,1) coefficients of the ocean-load increment dS_cur and the ice increment dI.
-__device__ void rotation_update(DevState* s, const Consts& c, double2 dS20, double2 dS21, double2 dI20,
-                                double2 dI21) {
-  if (!c.tpw) return;
+struct RotOut {
+  double il[9], mm[3], lam[18], dil[9], dm[3], dlam[6], dsl[6], rr[6];
+};
+
+__device__ void rotation_eval(const DevState* s, const Consts& c, double2 dS20, double2 dS21, double2 dI20,
+                              double2 dI21, RotOut& o) {
+  for (int q = 0; q < 9; ++q) o.il[q] = s->st.il[q];
+  for (int q = 0; q < 3; ++q) o.mm[q] = s->st.mm[q];
+  for (int q = 0; q < 18; ++q) o.lam[q] = s->st.lam[q];
   const double z20r = c.rhow * dS20.x + c.rhoi * dI20.x;
   const double z21r = c.rhow * dS21.x + c.rhoi * dI21.x;
   const double z21i = c.rhow * dS21.y + c.rhoi * dI21.y;
-  double* il = s->st.il;
-  il[8] = c.cI33 * z20r;
-  il[2] = c.cI13 * z21r;
-  il[5] = c.cI23 * z21i;
-  for (int q = 0; q < 9; ++q) s->dil[q] = il[q] - s->p_il[q];
-  double* mm = s->st.mm;
+  o.il[8] = c.cI33 * z20r;
+  o.il[2] = c.cI13 * z21r;
+  o.il[5] = c.cI23 * z21i;
+  for (int q = 0; q < 9; ++q) o.dil[q] = o.il[q] - s->p_il[q];
   for (int i = 0; i < 2; ++i)
-    mm[i] = c.alpha * (c.inv_CA * (c.one_kE2 * il[3 * i + 2] + s->sum_il[3 * i + 2]) + c.inv_kf * s->sum_m[i]);
-  mm[2] = -c.inv_C * (c.one_kE2 * il[8] + s->sum_il[8]);
-  for (int q = 0; q < 3; ++q) s->dm[q] = mm[q] - s->p_m[q];
-  const double m1 = mm[0], m2 = mm[1], m3 = mm[2];
-  double* lam = s->st.lam;  // [3][3] complex, (re, im) interleaved
-  lam[0] = c.K0 * ((m1 * m1 + m2 * m2 + m3 * m3) + 2.0 * m3);
-  lam[1] = 0.0;
-  lam[12] = c.K20 * (m1 * m1 + m2 * m2 - 2.0 * m3 * m3 - 4.0 * m3);
-  lam[13] = 0.0;
-  lam[14] = -c.K21 * (m1 * (1.0 + m3));
-  lam[15] = -c.K21 * (-m2 * (1.0 + m3));
-  lam[16] = c.K22 * (m2 * m2 - m1 * m1);
-  lam[17] = c.K22 * (2.0 * m1 * m2);
-  for (int q = 0; q < 6; ++q) s->dlam[q] = lam[12 + q] - s->p_lam[q];
+    o.mm[i] = c.alpha * (c.inv_CA * (c.one_kE2 * o.il[3 * i + 2] + s->sum_il[3 * i + 2]) + c.inv_kf * s->sum_m[i]);
+  o.mm[2] = -c.inv_C * (c.one_kE2 * o.il[8] + s->sum_il[8]);
+  for (int q = 0; q < 3; ++q) o.dm[q] = o.mm[q] - s->p_m[q];
+  const double m1 = o.mm[0], m2 = o.mm[1], m3 = o.mm[2];
+  o.lam[0] = c.K0 * ((m1 * m1 + m2 * m2 + m3 * m3) + 2.0 * m3);
+  o.lam[1] = 0.0;
+  o.lam[12] = c.K20 * (m1 * m1 + m2 * m2 - 2.0 * m3 * m3 - 4.0 * m3);
+  o.lam[13] = 0.0;
+  o.lam[14] = -c.K21 * (m1 * (1.0 + m3));
+  o.lam[15] = -c.K21 * (-m2 * (1.0 + m3));
+  o.lam[16] = c.K22 * (m2 * m2 - m1 * m1);
+  o.lam[17] = c.K22 * (2.0 * m1 * m2);
+  for (int q = 0; q < 6; ++q) o.dlam[q] = o.lam[12 + q] - s->p_lam[q];
   const double ig = 1.0 / c.gacc;
   for (int q = 0; q < 6; ++q) {
-    const double tot = s->p_lam[q] + s->dlam[q];
-    s->dsl_rot[q] = (c.ekhTE * tot + s->sum_tsl[q]) * ig;
-    s->rr_rot[q] = c.calcRG ? (c.hTE2 * tot + s->sum_trr[q]) * ig : 0.0;
+    const double tot = s->p_lam[q] + o.dlam[q];
+    o.dsl[q] = (c.ekhTE * tot + s->sum_tsl[q]) * ig;
+    o.rr[q] = c.calcRG ? (c.hTE2 * tot + s->sum_trr[q]) * ig : 0.0;
   }
 }
 
-__device__ __forceinline__ double2 load2_relaxed(const double2* p) {
-  const double* q = reinterpret_cast<const double*>(p);
-  return make_double2(load_relaxed(q), load_relaxed(q + 1));
+__device__ void rotation_store(DevState* s, const RotOut& o) {
+  for (int q = 0; q < 9; ++q) {
+    s->st.il[q] = o.il[q];
+    s->dil[q] = o.dil[q];
+  }
+  for (int q = 0; q < 3; ++q) {
+    s->st.mm[q] = o.mm[q];
+    s->dm[q] = o.dm[q];
+  }
+  for (int q = 0; q < 18; ++q) s->st.lam[q] = o.lam[q];
+  for (int q = 0; q < 6; ++q) {
+    s->dlam[q] = o.dlam[q];
+    s->dsl_rot[q] = o.dsl[q];
+    s->rr_rot[q] = o.rr[q];
+  }
 }
 
 // ---------------------------------------------------------------------- set-up (step 0)
@@ -205,18 +196,18 @@ __global__ void k_init_grid(Pts p, Consts c, const double* topo0, const double* 
 }
 
 // ---------------------------------------------------------------------- step begin
-// degree-2 rotational states (solver.py:407-421), one thread per state component (r_il 9,
-// r_m 3, r_lambda 6 reals), and the per-step resets
-__global__ void k_begin(Pts p, Consts c) {
-  if (blockIdx.x != 0) return;
+// degree-2 rotational states (solver.py:407-421): block q < 18 advances state component q
+// (r_il 9, r_m 3, r_lambda 6 reals) for every mode, one lane per mode, and sums the weighted
+// states over the modes in a fixed order; block 18 does the per-step resets.
+__global__ void __launch_bounds__(64) k_begin(Pts p, Consts c) {
   DevState* s = p.st;
-  const int q = threadIdx.x;
+  const int q = blockIdx.x, lane = threadIdx.x;
   if (q < 18) {
     const int row = q < 9 ? 0 : (q < 12 ? 1 : 2);
     const double pv = q < 9 ? s->p_il[q] : (q < 12 ? s->p_m[q - 9] : s->p_lam[q - 12]);
     double sum1 = 0.0, sum2 = 0.0;
     if (c.L >= 2) {
-      for (int k = 0; k < c.nmod2; ++k) {
+      for (int k = lane; k < c.nmod2; k += 64) {
         const double d = p.decay[k * c.nl + 2], o = 1.0 - d;
         double* r = p.r2 + 18 * k + q;
         const double rv = d * *r + o * pv;
@@ -225,21 +216,27 @@ __global__ void k_begin(Pts p, Consts c) {
         if (row == 2) sum2 += p.a2[3 * c.kmax + k] * rv;
       }
     }
-    if (q < 9) {
-      s->sum_il[q] = sum1;
-      s->dil[q] = 0.0;
-    } else if (q < 12) {
-      s->sum_m[q - 9] = sum1;
-      s->dm[q - 9] = 0.0;
-    } else {
-      s->sum_tsl[q - 12] = sum1;
-      s->sum_trr[q - 12] = sum2;
-      s->dsl_rot[q - 12] = 0.0;
-      s->rr_rot[q - 12] = 0.0;
-      s->dlam[q - 12] = 0.0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      sum1 += __shfl_xor(sum1, off, 64);
+      sum2 += __shfl_xor(sum2, off, 64);
     }
-  }
-  if (q == 32) {
+    if (lane == 0) {
+      if (q < 9) {
+        s->sum_il[q] = sum1;
+        s->dil[q] = 0.0;
+      } else if (q < 12) {
+        s->sum_m[q - 9] = sum1;
+        s->dm[q - 9] = 0.0;
+      } else {
+        s->sum_tsl[q - 12] = sum1;
+        s->sum_trr[q - 12] = sum2;
+        s->dsl_rot[q - 12] = 0.0;
+        s->rr_rot[q - 12] = 0.0;
+        s->dlam[q - 12] = 0.0;
+      }
+    }
+  } else if (lane == 0) {
     s->st.numiter = 0;
     s->st.done = 0;
     s->st.failed = 0;
@@ -308,34 +305,45 @@ __global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
     a += cabs_d(ld);
   }
   const double bs = block_sum(a, sh);
-  if (post_partial(bs, p.partial, &p.st->counter)) {
-    const double tot = sum_partials(p.partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      p.st->so = tot;
-      p.st->counter = 0;
-      rotation_update(p.st, c, load2_relaxed(p.dS_cur + c.idx20), load2_relaxed(p.dS_cur + c.idx21),
-                      load2_relaxed(p.dI + c.idx20), load2_relaxed(p.dI + c.idx21));
-    }
-  }
+  if (threadIdx.x == 0) p.partial[2 * PSLOT + blockIdx.x] = bs;  // sum |load_inc| of the guess
 }
 
 // ---------------------------------------------------------------------- inner iteration
 // eq. B18 coefficient, assembled while the synthesis stages its coefficients:
-// w1[l] ((rho_i dI + rho_w dS_prev) + rho_w load_inc) + w2[l] resp (+ rot at l = 2, m <= 2)
+// w1[l] ((rho_i dI + rho_w dS_prev) + rho_w load_inc) + w2[l] resp (+ rotation at l = 2, m <= 2).
+// With eval set (sea-level synthesis of an inner iteration) every block evaluates the rotation
+// block in its prologue from dS_cur and dI (identical arithmetic everywhere) and block 0
+// publishes it; otherwise (radial synthesis after the loop) the stored rr terms are used.
 struct CoefB18 {
-  const double2 *dI, *dSp, *ld, *resp;
+  const double2 *dI, *dSp, *ld, *resp, *dS;
   const double *w1, *w2;
-  const double* rot;  // 6 doubles: (re, im) of (2,0), (2,1), (2,2)
-  double rhoi, rhow;
-  __device__ double2 operator()(int, int idx, int l, int m) const {
+  DevState* st;
+  Consts c;
+  int eval;
+  __device__ void prologue(double* pro) const {
+    if (threadIdx.x != 0) return;
+    if (eval) {
+      RotOut o;
+      if (c.tpw) {
+        rotation_eval(st, c, dS[c.idx20], dS[c.idx21], dI[c.idx20], dI[c.idx21], o);
+        if (blockIdx.x == 0 && blockIdx.y == 0) rotation_store(st, o);
+      } else {
+        for (int q = 0; q < 6; ++q) o.dsl[q] = 0.0;
+      }
+      for (int q = 0; q < 6; ++q) pro[q] = o.dsl[q];
+    } else {
+      for (int q = 0; q < 6; ++q) pro[q] = c.tpw ? st->rr_rot[q] : 0.0;
+    }
+  }
+  __device__ double2 operator()(int, int idx, int l, int m, const double* pro) const {
     const double2 a = dI[idx], b = dSp[idx], q = ld[idx], v = resp[idx];
-    double ar = (a.x * rhoi + b.x * rhow) + q.x * rhow;
-    double ai = (a.y * rhoi + b.y * rhow) + q.y * rhow;
+    const double ar = (a.x * c.rhoi + b.x * c.rhow) + q.x * c.rhow;
+    const double ai = (a.y * c.rhoi + b.y * c.rhow) + q.y * c.rhow;
     const double u = w1[l], t = w2[l];
     double2 r = make_double2(ar * u + v.x * t, ai * u + v.y * t);
     if (l == 2 && m <= 2) {
-      r.x += rot[2 * m];
-      r.y += rot[2 * m + 1];
+      r.x += pro[2 * m];
+      r.y += pro[2 * m + 1];
     }
     return r;
   }
@@ -416,15 +424,18 @@ __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __re
   }
 }
 
-// conservation constant, ocean-load update and convergence ratio (solver.py:1002-1030); the last
-// block also evaluates the rotation block of the next iteration
-__global__ void __launch_bounds__(256) k_update(Pts p, Consts c) {
+// conservation constant and ocean-load update (solver.py:1002-1015); per-block partial sums of
+// |load_inc| for the convergence ratio, which the refresh kernel forms
+__device__ __forceinline__ double conserv_of(const Pts& p, const Consts& c) {
+  return (1.0 / p.cstar_lm[0].x) * (-(c.rhoi / c.rhow) * p.dI[0].x - p.rt_lm[0].x);
+}
+
+__global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it) {
   __shared__ double sh[256];
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
   const int n = c.ncoef;
-  const double c00 = p.cstar_lm[0].x;
-  const double conserv = (1.0 / c00) * (-(c.rhoi / c.rhow) * p.dI[0].x - p.rt_lm[0].x);
+  const double conserv = conserv_of(p, c);
   double a = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double2 sp = p.p_ocean[i], rt = p.rt_lm[i], cs = p.cstar_lm[i];
@@ -434,46 +445,54 @@ __global__ void __launch_bounds__(256) k_update(Pts p, Consts c) {
     a += cabs_d(nw);
   }
   const double bs = block_sum(a, sh);
-  if (post_partial(bs, p.partial, &s->counter)) {
-    const double sa = sum_partials(p.partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      const double so = s->so;
-      double xi;
-      if (fabs(sa - so) < EPS0 && fabs(so) < EPS0)
-        xi = 0.0;
-      else if (fabs(so) < EPS0)
-        xi = fabs((sa - so) / (EPS0 * 10));
-      else
-        xi = fabs((sa - so) / so);
-      s->sa = sa;
-      s->so = sa;
-      s->st.xi = xi;
-      s->st.conserv = conserv;
-      s->st.numiter += 1;
-      if (xi <= c.eps1) {
-        s->st.done = 1;
-      } else if (s->st.numiter >= MAX_INNER) {
-        s->st.failed = 1;
-        s->st.done = 1;
-      } else {
-        rotation_update(s, c, load2_relaxed(p.dS_cur + c.idx20), load2_relaxed(p.dS_cur + c.idx21),
-                        load2_relaxed(p.dI + c.idx20), load2_relaxed(p.dI + c.idx21));
-      }
-      s->counter = 0;
-    }
-  }
+  if (threadIdx.x == 0) p.partial[(it & 1) * PSLOT + blockIdx.x] = bs;
 }
 
 // topography / ocean-function re-guess and the grounded-ice flip count (solver.py:1032-1040,661-684).
 // it = iteration index within the step: the counters of this iteration live in slot it & 1.
-__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it) {
+// Every block first forms xi = |sum|new| - sum|old|| / sum|old| (solver.py:1023-1030) from the
+// update's block partials in a fixed order (sum|old| = the previous iteration's partials, or the
+// guess's), so that all blocks take the same decision; block 0 publishes it.
+__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nbu, int nbg) {
+  __shared__ double sh[256];
   __shared__ int shf;
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nbu; b += blockDim.x) v += p.partial[(it & 1) * PSLOT + b];
+  const double sa = block_sum(v, sh);
+  __syncthreads();
+  v = 0.0;
+  if (it == 0) {
+    for (int b = threadIdx.x; b < nbg; b += blockDim.x) v += p.partial[2 * PSLOT + b];
+  } else {
+    for (int b = threadIdx.x; b < nbu; b += blockDim.x) v += p.partial[((it - 1) & 1) * PSLOT + b];
+  }
+  const double so = block_sum(v, sh);
+  double xi;
+  if (fabs(sa - so) < EPS0 && fabs(so) < EPS0)
+    xi = 0.0;
+  else if (fabs(so) < EPS0)
+    xi = fabs((sa - so) / (EPS0 * 10));
+  else
+    xi = fabs((sa - so) / so);
+  const double conserv = conserv_of(p, c);
+  const bool conv = xi <= c.eps1;
+  const bool fail = !conv && it + 1 >= MAX_INNER;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s->st.numiter = it + 1;
+    s->st.xi = xi;
+    s->st.conserv = conserv;
+    s->sa = sa;
+    s->so = so;
+    if (conv || fail) s->st.done = 1;
+    if (fail) s->st.failed = 1;
+  }
+  if (conv || fail) return;
   if (threadIdx.x == 0) shf = 0;
   __syncthreads();
   const long n = c.npts;
-  const double cadd = s->st.conserv * c.K00;
+  const double cadd = conserv * c.K00;
   const bool regs = c.checkmarine && s->nacc2[it & 1] < GR_MAXACC;
   int flips = 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -526,13 +545,6 @@ __global__ void k_refresh2(Pts p, Consts c, int it) {
 __global__ void k_dice(Pts p, Consts c) {
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done || !*(volatile int*)&s->ice_changed) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    auto dI_at = [&](int k) {
-      const double2 ist = p.icestar_lm[k], old = p.oldice[k], pi = p.p_ice[k];
-      return make_double2(pi.x + (ist.x - old.x), pi.y + (ist.y - old.y));
-    };
-    rotation_update(s, c, p.dS_cur[c.idx20], p.dS_cur[c.idx21], dI_at(c.idx20), dI_at(c.idx21));
-  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.ncoef) return;
   const double2 ist = p.icestar_lm[i], old = p.oldice[i], pi = p.p_ice[i];
@@ -689,10 +701,10 @@ static int grid_blocks(long n) {
   long b = (n + 255) / 256;
   return (int)(b > 2048 ? 2048 : b);
 }
-// blocks of the kernels that end in a last-block reduction: few, so the arrival counter is cheap
+// blocks of the spectral kernels that post block partials (<= PSLOT)
 static int red_blocks(long n) {
-  long b = (n + 1023) / 1024;
-  return (int)(b > 128 ? 128 : (b < 1 ? 1 : b));
+  long b = (n + 511) / 512;
+  return (int)(b > 512 ? 512 : (b < 1 ? 1 : b));
 }
 
 extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const slx_outputs* o, slx_solver** out) {
@@ -744,7 +756,7 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   c.idx22 = h->h_moff[2];
   s->extrap = p->extrap;
   const size_t G = (size_t)c.npts * sizeof(double), S = (size_t)h->ncoef * sizeof(double2);
-  const size_t nblk_max = std::max<size_t>(4096, (h->ncoef + 255) / 256 + 1);
+  const size_t nblk_max = 3 * PSLOT;
   SLX_HIP(hipMalloc(&s->st, sizeof(DevState)));
   SLX_HIP(hipMemset(s->st, 0, sizeof(DevState)));
   SLX_HIP(hipMalloc(&s->partial, nblk_max * sizeof(double)));
@@ -871,7 +883,7 @@ extern "C" int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, 
   const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
   Pts p = make_pts(s, ice_cur, oldice);
   const Consts& c = s->c;
-  k_begin<<<1, 64, 0, st>>>(p, c);
+  k_begin<<<19, 64, 0, st>>>(p, c);
   SLX_LAUNCH_CHECK();
   int rc = launch_modal(s->h, c.kmax, s->nmod, s->decay, s->omd, s->a_sl, s->a_rr, s->R, s->p_load, s->visc,
                         s->rad, st);
@@ -906,7 +918,7 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     const bool first = (s->iters_enqueued == 0);
     const int it_step = s->iters_enqueued;
     s->iters_enqueued += 1;
-    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->ttlekhl, s->ttl, s->st->dsl_rot, c.rhoi, c.rhow};
+    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->st, c, 1};
     int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
     if (rc) return rc;
     {
@@ -921,9 +933,9 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     else
       rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
     if (rc) return rc;
-    k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c);
+    k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
-    k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step);
+    k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
     SLX_LAUNCH_CHECK();
     k_refresh2<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
@@ -945,7 +957,7 @@ extern "C" int slx_step_finish(slx_solver* s, void* stream) {
   const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
   Pts p = make_pts(s, s->ice_cur, oldice);
   if (c.calcRG) {
-    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->rad, s->ttlhe, s->ttl, s->st->rr_rot, c.rhoi, c.rhow};
+    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->rad, s->dS_cur, s->ttlhe, s->ttl, s->st, c, 0};
     const Pred fin{&s->st->st.done, 1, &s->st->st.finished, 0};
     int rc = launch_leg_syn<1>(s->h, cb, fin, st);
     if (rc) return rc;
