@@ -41,6 +41,20 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a 256 MiB device copy),
+    or None when no summary covers this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.txt")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(w, ice, budget):
     """The FP64 NumPy oracle on this host: set-up, then as many steps as fit in `budget` seconds
     (at least 1) of the same workload; steps/s over the timed steps."""
@@ -143,8 +157,9 @@ def main():
     torch.cuda.synchronize()
     t_kernel = ev0.elapsed_time(ev1) / 1e3 / reps
     flops = 2.0 * 2 * sht.ncoef * w.nlat  # 4 flops per (l, m, ring pair) per field, 2 fields
+    traffic, traffic_src = measured_traffic(w.name, "leg_ana_2field")
     roof = {"bound": "mfma", "achieved": flops / t_kernel / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-            "frac": flops / t_kernel / PEAK_FP64, "traffic": None,
+            "frac": flops / t_kernel / PEAK_FP64, "traffic": traffic, "traffic_source": traffic_src,
             "kernel": "k_leg_ana<2,R,2> (per-order FP64 Legendre analysis contraction, 2 fields)",
             "kernel_us": t_kernel * 1e6, "flops_per_launch": flops}
     out = {

@@ -109,13 +109,11 @@ int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field 
   dim3 grid((h->lmax + 2) / 2, (h->nh + NT - 1) / NT);
   size_t lds = (size_t)2 * (NF + 1) * (h->lmax + 9) * sizeof(double2);
   const long off = (long)field * h->nm * h->nh;
-  if (lds > 65536) {  // gfx950: up to 160 KB of LDS per workgroup when the kernel asks for it
-    static bool raised = false;
-    if (!raised) {
-      SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leg_syn<NF, NT, Coef>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-      raised = true;
-    }
+  static size_t lds_allowed = 65536;  // gfx950: up to 160 KB per workgroup when the kernel asks
+  if (lds > lds_allowed) {
+    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leg_syn<NF, NT, Coef>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_allowed = lds;
   }
   k_leg_syn<NF, NT, Coef><<<grid, NT, lds, s>>>(dev_of(h), seg_of(h), cf, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();

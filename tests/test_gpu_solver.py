@@ -116,3 +116,34 @@ def test_lmax512_workload_vs_oracle():
         print(f"[lmax512] step {k}", flush=True)
         e = compare(a.step(w.ice(k)), b.step(w.ice(k)))
         print(f"[lmax512] step {k} errors {e}", flush=True)
+
+
+def test_lmax1023_workload_vs_oracle_fixture():
+    """The 1024 x 2048, lmax 1023 workload (0.5 kyr steps, 30 modes), two steps, against the oracle's
+    stored outputs (every 16th row/column of tgrid and R, full-grid sums and sums of squares, the
+    rotation vector and the degree-0 outputs); the oracle needs about 45 s per step at this size."""
+    import os
+
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_lmax1023.npz"))
+    w = syn.workload("lmax1023")
+    a = gpu_solver(w.nlat, w.lmax, w.love_text, w.topo, w.ice(0), dt=w.dt, epsilon1=1e-5, inner="extrap")
+    s = int(f["sub"])
+    npts = w.nlat * w.nlon
+    for k in range(f["numiter"].size):
+        print(f"[lmax1023] step {k + 1}", flush=True)
+        r = a.step(w.ice(k + 1))
+        assert r["numiter"] == int(f["numiter"][k])
+        for name in ("tgrid", "R"):
+            g = r[name].cpu().numpy()
+            err = float(np.abs(g[::s, ::s] - f[name + "_sub"][k]).max())
+            assert err < 1e-9, (name, k, err)
+            # full-grid sums: a 1e-9 m discrepancy anywhere moves these by at most 1e-9 npts
+            assert abs(g.sum() - f[name + "_sum"][k]) < 1e-9 * npts + 1e-12 * abs(f[name + "_sum"][k])
+            ss = f[name + "_ss"][k]
+            assert abs((g * g).sum() - ss) < 1e-12 * ss + 2e-9 * np.sqrt(ss * npts)
+        assert np.allclose(r["mm"], f["mm"][k], rtol=1e-9, atol=1e-20)
+        for key in ("ocean_area", "oceanBeta_area", "ice_volume"):
+            assert abs(r[key] - f[key][k]) <= 1e-12 * abs(f[key][k]) + 1e-3, key
+        for key in ("gmslc_ocn", "gmslc_ocnBeta"):
+            assert abs(r[key] - f[key][k]) < 1e-9, key
+        print(f"[lmax1023] step {k + 1} ok", flush=True)
