@@ -1,5 +1,6 @@
 // Gauss-Legendre transform handle and its C-ABI (declared in include/slx.h).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "sht_launch.h"
@@ -110,12 +111,13 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   h->h_moff.resize(nm + 1);
   for (int m = 0; m <= nm; ++m) h->h_moff[m] = slx_moff(m, L);
   std::vector<double2> rec(h->ncoef);
-  std::vector<int> l_of(h->ncoef), jb(nm, 0);
+  std::vector<int> l_of(h->ncoef), m_of(h->ncoef), jb(nm, 0);
   for (int m = 0; m < nm; ++m) {
     s2m3[m] = std::sqrt((double)(2 * m + 3));
     for (int l = m; l <= L; ++l) {
       const int idx = h->h_moff[m] + (l - m);
       l_of[idx] = l;
+      m_of[idx] = m;
       if (l >= m + 2) {
         double e = std::sqrt((double)(l * l - m * m) / (double)(4 * l * l - 1));
         double ep = std::sqrt((double)((l - 1) * (l - 1) - m * m) / (double)(4 * (l - 1) * (l - 1) - 1));
@@ -144,19 +146,35 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   ALLOC_COPY(h->s2m3, s2m3);
   ALLOC_COPY(h->moff, h->h_moff);
   ALLOC_COPY(h->l_of, l_of);
+  ALLOC_COPY(h->m_of, m_of);
   ALLOC_COPY(h->jb, jb);
   ALLOC_COPY(h->tw, tw);
-  // degree segments: one analysis block per (order, segment); S (a multiple of 8) is the
-  // largest of 256, 128, 64, 32 that still gives >= 2048 waves (with one wave per segment)
-  h->ana_nw = (nh > 256) ? 2 : 1;
+  // degree segments.  With nlat <= 1024 the analysis runs on the matrix cores with 32-degree
+  // segments in groups of 4 per block; otherwise one block per (order, segment) with S (a multiple
+  // of 8) the largest of 256, 128, 64, 32 that still gives >= 2048 waves.  Synthesis chains are
+  // syn_seg = a multiple of S degrees long, about 8 chains per order pair.
   auto count = [&](int S) {
     long n = 0;
     for (int m = 0; m < nm; ++m) n += (L - m + S) / S;
     return n;
   };
-  int S = 256;
-  while (S > 32 && count(S) * h->ana_nw < 2048) S /= 2;
+  int S;
+  h->ana_mfma = (nh <= 512) ? 1 : 0;
+  if (h->ana_mfma) {
+    S = MF_S;
+  } else {
+    h->ana_nw = (nh > 256) ? 2 : 1;
+    S = 256;
+    while (S > 32 && count(S) * h->ana_nw < 2048) S /= 2;
+  }
   h->seg_len = S;
+  // synthesis variant: matrix cores when the segments are the matrix-core ones, unless the
+  // SLX_SYN_VARIANT environment variable asks for the scalar-fed VALU kernel ("scalar")
+  {
+    const char* v = std::getenv("SLX_SYN_VARIANT");
+    h->syn_mfma = (h->ana_mfma && !(v && std::strcmp(v, "scalar") == 0)) ? 1 : 0;
+  }
+  h->syn_seg = S * std::max(1, (int)std::lround((L + 2) / (8.0 * S)));
   std::vector<int> seg_m, seg_l0, first(nm);
   for (int m = 0; m < nm; ++m) {
     first[m] = (int)seg_m.size();
@@ -166,6 +184,17 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     }
   }
   h->nseg = (int)seg_m.size();
+  {
+    std::vector<int> gm, gs;
+    for (int m = 0; m < nm; ++m)
+      for (int s0 = 0; s0 < (L - m + S) / S; s0 += 4) {
+        gm.push_back(m);
+        gs.push_back(s0);
+      }
+    h->ngrp = (int)gm.size();
+    ALLOC_COPY(h->grp_m, gm);
+    ALLOC_COPY(h->grp_s0, gs);
+  }
   ALLOC_COPY(h->seg_m, seg_m);
   ALLOC_COPY(h->seg_l0, seg_l0);
   ALLOC_COPY(h->seg_first, first);
@@ -186,10 +215,11 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   SLX_HIP(hipMalloc(&h->WD, wsz));
   SLX_HIP(hipMalloc(&h->FN, wsz));
   SLX_HIP(hipMalloc(&h->FSo, wsz));
+  SLX_HIP(hipMalloc(&h->CS, (size_t)max_fields * h->ncoef * sizeof(double2)));
   SLX_HIP(hipMemset(h->WS, 0, wsz));
   SLX_HIP(hipMemset(h->WD, 0, wsz));
-  // autotune: waves per segment of the 2-field analysis contraction (1 or 2, rings per lane
-  // <= 4), timed on the device with the handle's own shapes
+  // autotune of the 2-field analysis contraction, timed on the device with the handle's own
+  // shapes: rings per lane (2 or 4) on the matrix-core path, waves per segment (1 or 2) otherwise
   {
     double2* tmp = nullptr;
     SLX_HIP(hipMalloc(&tmp, 2 * (size_t)h->ncoef * sizeof(double2)));
@@ -197,10 +227,16 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     SLX_HIP(hipEventCreate(&e0));
     SLX_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_nw = h->ana_nw;
-    for (int nw = 1; nw <= 2; ++nw) {
-      if ((nh + 64 * nw - 1) / (64 * nw) > 4) continue;
-      h->ana_nw = nw;
+    int best_v = -1;
+    for (int v = 1; v <= 2; ++v) {
+      if (h->ana_mfma) {
+        const int r = 2 * v;
+        if ((nh + 16 * r - 1) / (16 * r) > 8) continue;
+        h->ana_r = r;
+      } else {
+        if ((nh + 64 * v - 1) / (64 * v) > 4) continue;
+        h->ana_nw = v;
+      }
       OutPtrs<2> o2{{tmp, tmp + h->ncoef}};
       int rc = launch_leg_ana<2>(h, h->WS, h->WD, o2, Pred{}, nullptr);
       if (rc) return rc;
@@ -215,10 +251,15 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       SLX_HIP(hipEventElapsedTime(&ms, e0, e1));
       if (ms < best) {
         best = ms;
-        best_nw = nw;
+        best_v = v;
       }
     }
-    h->ana_nw = best_nw;
+    if (best_v > 0) {
+      if (h->ana_mfma)
+        h->ana_r = 2 * best_v;
+      else
+        h->ana_nw = best_v;
+    }
     SLX_HIP(hipEventDestroy(e0));
     SLX_HIP(hipEventDestroy(e1));
     SLX_HIP(hipFree(tmp));
@@ -231,7 +272,7 @@ extern "C" int slx_sht_destroy(slx_sht* h) {
   if (!h) return 0;
   void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->tw,
                   h->WS, h->WD, h->FN, h->FSo, h->seg_m, h->seg_l0, h->seg_first, h->seed_a, h->seed_b,
-                  h->seed_s};
+                  h->seed_s, h->m_of, h->grp_m, h->grp_s0, h->CS};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;

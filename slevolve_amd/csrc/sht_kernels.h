@@ -110,6 +110,8 @@ struct ShtDev {
   const int* moff;
   const int* jb;
   const double2* tw;
+  const int* l_of;  // degree of each packed index
+  const int* m_of;  // order of each packed index
 };
 
 // -------------------------------------------------------------------------------------------
@@ -175,6 +177,8 @@ __device__ __forceinline__ void leg_renorm(LegState& s) {
 // produced once on the device by k_leg_seed with the same arithmetic.
 struct SegDev {
   int nseg, S;
+  int Ssyn;  // degrees per synthesis chain (a multiple of S); chain s of order m starts at seed
+             // first[m] + s Ssyn / S
   const int* seg_m;
   const int* seg_l0;
   const double* sa;  // [nseg][nh] P~_{l0-2}
@@ -569,6 +573,507 @@ __global__ void __launch_bounds__(NT) k_fft_syn(ShtDev d, const double2* __restr
     double2 v = z[i];
     st(f, rn + i, v.x);
     st(f, rs + i, v.y);
+  }
+}
+
+
+// -------------------------------------------------------------------------------------------
+// Analysis contraction on the matrix cores: v_mfma_f64_4x4x4_4b_f64 (four independent 4x4x4
+// blocks per wave, FP64 in and out).  Lane roles of that instruction (established on the device
+// with a one-hot probe): block b = (lane >> 2) & 3; the A operand A[i][k] comes from lane
+// 16 k + 4 b + i, B[k][j] from lane 16 k + 4 b + j, and D[i][j] is held by lane 16 i + 4 b + j.
+//   rows i    = four consecutive degree segments ("slots") of one order m, S degrees apart;
+//   K index k = four northern rings;  columns j = field x (re, im).
+// A lane walks the recurrence of ring k through segment i, so at degree step t the four rows hold
+// the degrees l0_i + t, whose l - m all have the parity of t (S is a multiple of 8): the folded
+// Fourier value of the lane's ring for that parity (S or D) is the B operand, and the matrix core
+// does the sum over rings.  Each lane owns R rings (R MFMAs per degree step into the same
+// accumulator); the S accumulators of a segment group are reduced over the 4 MFMA blocks (lane
+// bits 2 and 3) by a two-level reduce-scatter and over the waves in LDS, in a fixed order.
+// Block = one group of 4 segments of one order; consecutive groups (same order, same Fourier
+// inputs) are given to the same XCD (the hardware deals block b to XCD b % 8).
+struct GrpDev {
+  int ngrp;
+  const int* m;   // order of the group
+  const int* s0;  // first segment (index within the order) of the group
+};
+
+constexpr int MF_S = 32;  // degrees per segment for the matrix-core analysis
+
+// acc[v] summed over the lanes that differ in bits 2 and 3 (the 4 MFMA blocks), reduce-scatter:
+// afterwards acc[v] (v < 8) holds the sum of value 16 ((lane >> 3) & 1) + 8 ((lane >> 2) & 1) + v.
+__device__ __forceinline__ void mf_block_reduce(double (&acc)[32], int lane) {
+  {
+    const bool up = (lane & 8) != 0;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const double send = up ? acc[v] : acc[v + 16];
+      const double keep = up ? acc[v + 16] : acc[v];
+      acc[v] = keep + __shfl_xor(send, 8, 64);
+    }
+  }
+  {
+    const bool up = (lane & 4) != 0;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const double send = up ? acc[v] : acc[v + 8];
+      const double keep = up ? acc[v + 8] : acc[v];
+      acc[v] = keep + __shfl_xor(send, 4, 64);
+    }
+  }
+}
+
+template <int NF, int R>
+__global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS,
+                                                    const double2* __restrict__ WD, OutPtrs<NF> out, Pred pr) {
+  static_assert(NF == 1 || NF == 2, "one or two fields per launch");
+  if (!pr.active()) return;
+  constexpr int S = MF_S;
+  extern __shared__ double red[];  // [NW][8][64] wave partials
+  __shared__ double2 rcs[4 * S];   // recurrence coefficients of the 4 slots
+  const int per = (gr.ngrp + 7) >> 3;
+  const int gi = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (gi >= gr.ngrp) return;
+  const int m = gr.m[gi], s0 = gr.s0[gi];
+  const int L = d.lmax, nh = d.nh, mo = d.moff[m];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const int nsm = (L - m + S) / S;  // segments of order m
+  for (int i = threadIdx.x; i < 4 * S; i += blockDim.x) {
+    const int off = (s0 + i / S) * S + i % S;  // l - m
+    const bool use = off >= 2 && off <= L - m;
+    rcs[i] = use ? d.rec[mo + off] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const int blk = (lane >> 2) & 3, slot = lane & 3, kk = lane >> 4;
+  const int segi = s0 + slot;
+  const bool sval = segi < nsm;
+  const bool head = sval && segi == 0;
+  const long sx = (long)(g.first[m] + (sval ? segi : 0)) * nh;
+  const long fstride = (long)d.nm * nh;
+  const int col = slot;  // B column of this lane (same lane bits as its A row)
+  const bool colok = (col >> 1) < NF;
+  const int fb = colok ? (col >> 1) : 0;
+  double xr[R], mk[R], wv[2][R];
+  LegState st[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = (wave * R + r) * 16 + 4 * blk + kk;  // ring of this lane (A and B roles alike)
+    const bool ok = j < nh;
+    const int jj = ok ? j : 0;
+    xr[r] = d.x[jj];
+    const double a0 = head ? 0.0 : g.sa[sx + jj];
+    const double b0 = head ? d.pmm[(long)m * nh + jj] : g.sb[sx + jj];
+    const int c0 = head ? d.pms[(long)m * nh + jj] : g.ss[sx + jj];
+    const bool on = ok && sval;
+    st[r] = LegState{on ? a0 : 0.0, on ? b0 : 0.0, on ? c0 : 0};
+    mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+    const double2 vs = WS[fb * fstride + (long)m * nh + jj], vd = WD[fb * fstride + (long)m * nh + jj];
+    const bool use = ok && colok;
+    wv[0][r] = use ? ((col & 1) ? vs.y : vs.x) : 0.0;
+    wv[1][r] = use ? ((col & 1) ? vd.y : vd.x) : 0.0;
+  }
+  const double s23 = d.s2m3[m];
+  double acc[S];
+#pragma unroll
+  for (int t = 0; t < S; ++t) acc[t] = 0.0;
+  double2 cc[8];  // recurrence coefficients of the current 8 degrees, read ahead from LDS
+#pragma unroll
+  for (int t = 0; t < S; ++t) {
+    if ((t & 7) == 0) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cc[u] = rcs[slot * S + t + u];
+    }
+    const double2 c = cc[t & 7];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double p;
+      if (t == 0) {  // head: P_m itself, the state does not move
+        const double pa0 = st[r].pa, pb0 = st[r].pb;
+        const double pn = leg_next(xr[r], c, st[r]);
+        p = head ? pb0 : pn;
+        st[r].pa = head ? pa0 : st[r].pa;
+        st[r].pb = head ? pb0 : st[r].pb;
+      } else if (t == 1) {  // head: P_{m+1} = sqrt(2m+3) x P_m
+        const double ps = s23 * xr[r] * st[r].pb;
+        const double pn = leg_next(xr[r], c, st[r]);
+        p = head ? ps : pn;
+        st[r].pb = p;
+      } else {
+        p = leg_next(xr[r], c, st[r]);
+      }
+      acc[t] = __builtin_amdgcn_mfma_f64_4x4x4f64(p * mk[r], wv[t & 1][r], acc[t], 0, 0, 0);
+      if ((t & 7) == 7) {
+        leg_renorm(st[r]);
+        mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+      }
+    }
+  }
+  mf_block_reduce(acc, lane);
+  auto emit = [&](int v, int ln, double val) {
+    const int t = 16 * ((ln >> 3) & 1) + 8 * ((ln >> 2) & 1) + v;
+    const int row = ln >> 4, cj = ln & 3;
+    const int off = (s0 + row) * S + t;
+    if ((s0 + row) < nsm && off <= L - m && (cj >> 1) < NF)
+      reinterpret_cast<double*>(out.p[cj >> 1])[2 * (long)(mo + off) + (cj & 1)] = val / SLX_SQRT2_LEGACY;
+  };
+  if (NW == 1) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) emit(v, lane, acc[v]);
+    return;
+  }
+#pragma unroll
+  for (int v = 0; v < 8; ++v) red[(wave * 8 + v) * 64 + lane] = acc[v];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 512; e += blockDim.x) {
+    const int v = e >> 6, ln = e & 63;
+    double sum = red[v * 64 + ln];
+    for (int w = 1; w < NW; ++w) sum += red[(w * 8 + v) * 64 + ln];
+    emit(v, ln, sum);
+  }
+}
+
+// -------------------------------------------------------------------------------------------
+// Synthesis contraction with wave-uniform degree streams.  Block = (order pair m1 = p, m2 = L - p)
+// x 64 northern rings, 4 waves.  The chains (Ssyn-degree segments, seeded on the device) of the
+// two orders are dealt to the waves round-robin, two at a time per wave; every lane owns one ring.
+// All lanes of a wave step through the same degree, so the recurrence coefficients and the
+// spectral coefficients are uniform and come through scalar loads; past a chain's end both read
+// as zero.  E (even l - m) and O (odd) are summed per chain, per wave, then over the 4 waves in
+// LDS in a fixed order; FN = 1.41421356 (E + O) (north), FS = 1.41421356 (E - O) (south),
+// written [f][m][j].  Coefficients: cf[f cstride + packed index].
+template <int NF, int CQ, bool FIRST>
+__device__ __forceinline__ void syn_step8(LegState (&st)[CQ], double (&mk)[CQ], const bool (&head)[CQ],
+                                          const int (&gi)[CQ], const int (&cnt)[CQ], int t0, double x,
+                                          const double (&s23)[CQ], const double2* __restrict__ rec,
+                                          const double2* __restrict__ cf, long cstride, double2 (&Ec)[CQ][NF],
+                                          double2 (&Oc)[CQ][NF]) {
+#pragma unroll
+  for (int il = 0; il < 8; ++il) {
+#pragma unroll
+    for (int q = 0; q < CQ; ++q) {
+      const int t = t0 + il;
+      const bool v = t < cnt[q];
+      const int idx = v ? gi[q] + t : 0;
+      const double2 c = v ? rec[idx] : make_double2(0.0, 0.0);
+      double p;
+      if (FIRST && il == 0) {
+        const double pa0 = st[q].pa, pb0 = st[q].pb;
+        const double pn = leg_next(x, c, st[q]);
+        p = head[q] ? pb0 : pn;
+        st[q].pa = head[q] ? pa0 : st[q].pa;
+        st[q].pb = head[q] ? pb0 : st[q].pb;
+      } else if (FIRST && il == 1) {
+        const double ps = s23[q] * x * st[q].pb;
+        const double pn = leg_next(x, c, st[q]);
+        p = head[q] ? ps : pn;
+        st[q].pb = p;
+      } else {
+        p = leg_next(x, c, st[q]);
+      }
+      p *= mk[q];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double2 cv = v ? cf[f * cstride + idx] : make_double2(0.0, 0.0);
+        if ((il & 1) == 0) {
+          Ec[q][f].x += p * cv.x;
+          Ec[q][f].y += p * cv.y;
+        } else {
+          Oc[q][f].x += p * cv.x;
+          Oc[q][f].y += p * cv.y;
+        }
+      }
+      if (il == 7) {
+        leg_renorm(st[q]);
+        mk[q] = st[q].sc == 0 ? 1.0 : 0.0;
+      }
+    }
+  }
+}
+
+template <int NF, int CQ>
+__global__ void __launch_bounds__(256) k_leg_syn_s(ShtDev d, SegDev g, const double2* __restrict__ rec,
+                                                   const double2* __restrict__ cf, long cstride,
+                                                   double2* __restrict__ FN, double2* __restrict__ FS, Pred pr) {
+  if (!pr.active()) return;
+  __shared__ double2 part[4][2][NF][2][64];
+  const int L = d.lmax, nl = L + 1, S = g.Ssyn, ratio = g.Ssyn / g.S, nh = d.nh;
+  const int p = blockIdx.x;
+  const int mord[2] = {p, L - p};
+  const int nord = (L - p == p) ? 1 : 2;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int j = blockIdx.y * 64 + lane;
+  const bool ok = j < nh;
+  const int jj = ok ? j : 0;
+  const double x = d.x[jj];
+  const double s23[2] = {d.s2m3[mord[0]], d.s2m3[mord[1]]};
+  const int ns0 = (nl - mord[0] + S - 1) / S;
+  const int ns1 = nord == 2 ? (nl - mord[1] + S - 1) / S : 0;
+  const int C = ns0 + ns1;
+  double2 E[2][NF], O[2][NF];
+#pragma unroll
+  for (int o = 0; o < 2; ++o)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) E[o][f] = O[o][f] = make_double2(0.0, 0.0);
+  for (int c0 = CQ * wave; c0 < C; c0 += 4 * CQ) {
+    LegState st[CQ];
+    double mk[CQ], s23c[CQ];
+    bool head[CQ];
+    int gi[CQ], cnt[CQ], oo[CQ];
+    double2 Ec[CQ][NF], Oc[CQ][NF];
+#pragma unroll
+    for (int q = 0; q < CQ; ++q) {
+      const int c = c0 + q;
+      const bool act = c < C;
+      const int o = (c < ns0) ? 0 : 1;
+      const int s = (c < ns0) ? c : c - ns0;
+      const int m = mord[o];
+      oo[q] = o;
+      s23c[q] = s23[o];
+      cnt[q] = act ? min(S, nl - m - s * S) : 0;
+      gi[q] = d.moff[m] + s * S;
+      head[q] = act && s == 0;
+      const long ix = (long)m * nh + jj;
+      const long sx = (long)(g.first[m] + s * ratio) * nh + jj;
+      if (act && s == 0) {
+        st[q] = LegState{0.0, d.pmm[ix], d.pms[ix]};
+      } else if (act) {
+        st[q] = LegState{g.sa[sx], g.sb[sx], g.ss[sx]};
+      } else {
+        st[q] = LegState{0.0, 0.0, 0};
+      }
+      if (!ok) st[q] = LegState{0.0, 0.0, 0};
+      mk[q] = st[q].sc == 0 ? 1.0 : 0.0;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) Ec[q][f] = Oc[q][f] = make_double2(0.0, 0.0);
+    }
+    int tmax = 0;
+#pragma unroll
+    for (int q = 0; q < CQ; ++q) tmax = max(tmax, cnt[q]);
+    syn_step8<NF, CQ, true>(st, mk, head, gi, cnt, 0, x, s23c, rec, cf, cstride, Ec, Oc);
+    for (int t0 = 8; t0 < tmax; t0 += 8)
+      syn_step8<NF, CQ, false>(st, mk, head, gi, cnt, t0, x, s23c, rec, cf, cstride, Ec, Oc);
+#pragma unroll
+    for (int q = 0; q < CQ; ++q) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        if (oo[q] == 0) {
+          E[0][f] = cadd(E[0][f], Ec[q][f]);
+          O[0][f] = cadd(O[0][f], Oc[q][f]);
+        } else {
+          E[1][f] = cadd(E[1][f], Ec[q][f]);
+          O[1][f] = cadd(O[1][f], Oc[q][f]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < 2; ++o)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      part[wave][o][f][0][lane] = E[o][f];
+      part[wave][o][f][1][lane] = O[o][f];
+    }
+  __syncthreads();
+  if (wave >= nord || !ok) return;
+  const int o = wave, m = mord[o];
+  const long fstride = (long)d.nm * nh;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    double2 e = part[0][o][f][0][lane], q = part[0][o][f][1][lane];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      e = cadd(e, part[w][o][f][0][lane]);
+      q = cadd(q, part[w][o][f][1][lane]);
+    }
+    const long idx = f * fstride + (long)m * nh + j;
+    FN[idx] = make_double2(SLX_SQRT2_LEGACY * (e.x + q.x), SLX_SQRT2_LEGACY * (e.y + q.y));
+    FS[idx] = make_double2(SLX_SQRT2_LEGACY * (e.x - q.x), SLX_SQRT2_LEGACY * (e.y - q.y));
+  }
+}
+
+// Spectral coefficients assembled by a loader (Coef: prologue(double*) per block, then
+// double2 operator()(f, idx, l, m, pro)) into a plain packed array out[f ostride + idx].
+template <int NF, class Coef>
+__global__ void __launch_bounds__(256) k_coef_fill(ShtDev d, Coef cf, double2* __restrict__ out, long ostride,
+                                                   int ncoef, Pred pr) {
+  if (!pr.active()) return;
+  __shared__ double pro[32];
+  cf.prologue(pro);
+  __syncthreads();
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= ncoef) return;
+  const int l = d.l_of[idx], m = d.m_of[idx];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) out[f * ostride + idx] = cf(f, idx, l, m, pro);
+}
+
+
+// -------------------------------------------------------------------------------------------
+// Synthesis contraction on the matrix cores (v_mfma_f64_4x4x4_4b_f64, lane roles as for the
+// analysis above): rows i = four northern rings, K index k = four degree slots (32-degree
+// segments 4g .. 4g + 3 of one order), columns j = field x (re, im).  A lane walks the
+// recurrence of its ring through its slot; at step t the K index runs over the degrees
+// l0_k + t, all of the parity of t, so even and odd steps accumulate into separate E and O
+// outputs.  Block = (order pair p, L - p) x 16 R rings (R rings per lane, 4 waves); the segment
+// groups of both orders are dealt to the waves, whose E/O partials are summed in LDS in a fixed
+// order.  Recurrence and spectral coefficients of both orders are staged in LDS, zero past L.
+// Outputs FN = 1.41421356 (E + O), FS = 1.41421356 (E - O), [f][m][j].
+template <int NF, int R>
+__global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride,
+                                                    double2* __restrict__ FN, double2* __restrict__ FS, Pred pr) {
+  static_assert(NF == 1 || NF == 2, "one or two fields per launch");
+  if (!pr.active()) return;
+  constexpr int S = MF_S;
+  extern __shared__ double2 lsy[];
+  const int L = d.lmax, nl = L + 1, nh = d.nh;
+  const int nlp = ((nl + 4 * S - 1) / (4 * S)) * (4 * S);
+  double2* rc = lsy;                // [2][nlp]
+  double2* cl = lsy + 2 * nlp;      // [2][NF][nlp]
+  double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
+  const int p = blockIdx.x;
+  const int mord[2] = {p, L - p};
+  const int nord = (L - p == p) ? 1 : 2;
+  for (int o = 0; o < 2; ++o) {
+    const int m = mord[o], mo = d.moff[m], cnt = nl - m;
+    for (int i = threadIdx.x; i < nlp; i += 256) {
+      const bool v = (o < nord) && (i < cnt);
+      rc[o * nlp + i] = (v && i >= 2) ? d.rec[mo + i] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) cl[(o * NF + f) * nlp + i] = v ? cf[f * cstride + mo + i] : make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int blk = (lane >> 2) & 3, ri = lane & 3, slot = lane >> 4;
+  const int col = lane & 3, fb = col >> 1, comp = col & 1;
+  const bool colok = fb < NF;
+  int jr[R];
+  bool okr[R];
+  double xr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    jr[r] = blockIdx.y * 16 * R + r * 16 + 4 * blk + ri;
+    okr[r] = jr[r] < nh;
+    xr[r] = d.x[okr[r] ? jr[r] : 0];
+  }
+  const int ng0 = ((nl - mord[0] + S - 1) / S + 3) / 4;
+  const int ng1 = nord == 2 ? ((nl - mord[1] + S - 1) / S + 3) / 4 : 0;
+  double EO[2][2][R];  // [order][E, O][ring]
+#pragma unroll
+  for (int o = 0; o < 2; ++o)
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int r = 0; r < R; ++r) EO[o][e][r] = 0.0;
+  for (int it = wave; it < ng0 + ng1; it += 4) {
+    const int o = it < ng0 ? 0 : 1;
+    const int gq = it < ng0 ? it : it - ng0;
+    const int m = mord[o];
+    const int nsm = (nl - m + S - 1) / S;
+    const int sg = 4 * gq + slot;
+    const bool sval = sg < nsm;
+    const bool head = sval && sg == 0;
+    const long sx = (long)(g.first[m] + (sval ? sg : 0)) * nh;
+    LegState st[R];
+    double mk[R], aE[R], aO[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int jj = okr[r] ? jr[r] : 0;
+      const double a0 = head ? 0.0 : g.sa[sx + jj];
+      const double b0 = head ? d.pmm[(long)m * nh + jj] : g.sb[sx + jj];
+      const int c0 = head ? d.pms[(long)m * nh + jj] : g.ss[sx + jj];
+      const bool on = okr[r] && sval;
+      st[r] = LegState{on ? a0 : 0.0, on ? b0 : 0.0, on ? c0 : 0};
+      mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+      aE[r] = aO[r] = 0.0;
+    }
+    const double s23 = d.s2m3[m];
+    const double2* rco = rc + o * nlp + sg * S;
+    const double* clo = reinterpret_cast<const double*>(cl + (o * NF + (colok ? fb : 0)) * nlp + sg * S) + comp;
+    double2 cc[8];
+    double bb[8];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      if ((t & 7) == 0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          cc[u] = sval ? rco[t + u] : make_double2(0.0, 0.0);
+          bb[u] = (sval && colok) ? clo[2 * (t + u)] : 0.0;
+        }
+      }
+      const double2 c = cc[t & 7];
+      const double bv = bb[t & 7];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        double pv;
+        if (t == 0) {
+          const double pa0 = st[r].pa, pb0 = st[r].pb;
+          const double pn = leg_next(xr[r], c, st[r]);
+          pv = head ? pb0 : pn;
+          st[r].pa = head ? pa0 : st[r].pa;
+          st[r].pb = head ? pb0 : st[r].pb;
+        } else if (t == 1) {
+          const double ps = s23 * xr[r] * st[r].pb;
+          const double pn = leg_next(xr[r], c, st[r]);
+          pv = head ? ps : pn;
+          st[r].pb = pv;
+        } else {
+          pv = leg_next(xr[r], c, st[r]);
+        }
+        if ((t & 1) == 0)
+          aE[r] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv * mk[r], bv, aE[r], 0, 0, 0);
+        else
+          aO[r] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv * mk[r], bv, aO[r], 0, 0, 0);
+        if ((t & 7) == 7) {
+          leg_renorm(st[r]);
+          mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+        }
+      }
+    }
+    if (o == 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        EO[0][0][r] += aE[r];
+        EO[0][1][r] += aO[r];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        EO[1][0][r] += aE[r];
+        EO[1][1][r] += aO[r];
+      }
+    }
+  }
+  __syncthreads();  // all waves are done with the staged coefficients
+#pragma unroll
+  for (int o = 0; o < 2; ++o)
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int r = 0; r < R; ++r) part[(((wave * 2 + o) * 2 + e) * R + r) * 64 + lane] = EO[o][e][r];
+  __syncthreads();
+  if (wave >= nord) return;
+  const int o = wave, m = mord[o];
+  if (!colok) return;
+  const long fstride = (long)d.nm * nh;
+  const int rowr = lane >> 4;  // ring row of the D element this lane holds
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double e = part[(((0 * 2 + o) * 2 + 0) * R + r) * 64 + lane];
+    double q = part[(((0 * 2 + o) * 2 + 1) * R + r) * 64 + lane];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      e += part[(((w * 2 + o) * 2 + 0) * R + r) * 64 + lane];
+      q += part[(((w * 2 + o) * 2 + 1) * R + r) * 64 + lane];
+    }
+    const int j = blockIdx.y * 16 * R + r * 16 + 4 * blk + rowr;
+    if (j < nh) {
+      const long idx = fb * fstride + (long)m * nh + j;
+      reinterpret_cast<double*>(FN)[2 * idx + comp] = SLX_SQRT2_LEGACY * (e + q);
+      reinterpret_cast<double*>(FS)[2 * idx + comp] = SLX_SQRT2_LEGACY * (e - q);
+    }
   }
 }
 

@@ -1,5 +1,8 @@
 // Host-side launch helpers for the transform kernels (shared by the SHT and solver units).
 #pragma once
+#include <algorithm>
+#include <type_traits>
+
 #include "sht_kernels.h"
 
 namespace slx {
@@ -22,6 +25,8 @@ inline ShtDev dev_of(const slx_sht* h) {
   d.moff = h->moff;
   d.jb = h->jb;
   d.tw = h->tw;
+  d.l_of = h->l_of;
+  d.m_of = h->m_of;
   return d;
 }
 
@@ -47,6 +52,7 @@ inline SegDev seg_of(const slx_sht* h) {
   SegDev g;
   g.nseg = h->nseg;
   g.S = h->seg_len;
+  g.Ssyn = h->syn_seg;
   g.seg_m = h->seg_m;
   g.seg_l0 = h->seg_l0;
   g.sa = h->seed_a;
@@ -61,9 +67,32 @@ inline int ana_rings_per_lane(const slx_sht* h) {
   return r <= 1 ? 1 : r <= 2 ? 2 : 4;
 }
 
+template <int NF, int R>
+int launch_leg_ana_mf(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
+                      hipStream_t s) {
+  const int nw = (h->nh + 16 * R - 1) / (16 * R);
+  GrpDev gr{h->ngrp, h->grp_m, h->grp_s0};
+  dim3 grid(8 * ((h->ngrp + 7) / 8));
+  const size_t lds = (nw > 1) ? (size_t)nw * 8 * 64 * sizeof(double) : 0;
+  k_leg_ana_mf<NF, R><<<grid, 64 * nw, lds, s>>>(dev_of(h), seg_of(h), gr, WS, WD, out, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
 template <int NF>
 int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
                    hipStream_t s) {
+  if (h->ana_mfma) {
+    if constexpr (NF == 3) {  // two launches: fields 0, 1 and field 2
+      const long fs = (long)h->nm * h->nh;
+      int rc = launch_leg_ana<2>(h, WS, WD, OutPtrs<2>{{out.p[0], out.p[1]}}, pr, s);
+      if (rc) return rc;
+      return launch_leg_ana<1>(h, WS + 2 * fs, WD + 2 * fs, OutPtrs<1>{{out.p[2]}}, pr, s);
+    } else {
+      return h->ana_r == 4 ? launch_leg_ana_mf<NF, 4>(h, WS, WD, out, pr, s)
+                           : launch_leg_ana_mf<NF, 2>(h, WS, WD, out, pr, s);
+    }
+  }
   const ShtDev d = dev_of(h);
   const SegDev g = seg_of(h);
   const int R = ana_rings_per_lane(h);
@@ -103,19 +132,41 @@ int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
   SLX_LAUNCH_CHECK();
   return 0;
 }
+// Synthesis contraction of NF fields (FN/FS fields field .. field + NF - 1).  A CoefPlain
+// source is read in place; any other loader is first assembled into the handle's CS array.
 template <int NF, class Coef>
 int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field = 0) {
-  constexpr int NT = 256;
-  dim3 grid((h->lmax + 2) / 2, (h->nh + NT - 1) / NT);
-  size_t lds = (size_t)2 * (NF + 1) * (h->lmax + 9) * sizeof(double2);
-  const long off = (long)field * h->nm * h->nh;
-  static size_t lds_allowed = 65536;  // gfx950: up to 160 KB per workgroup when the kernel asks
-  if (lds > lds_allowed) {
-    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leg_syn<NF, NT, Coef>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    lds_allowed = lds;
+  const double2* src;
+  long cstride;
+  if constexpr (std::is_same<Coef, CoefPlain>::value) {
+    src = cf.c;
+    cstride = cf.fstride;
+  } else {
+    k_coef_fill<NF, Coef><<<(h->ncoef + 255) / 256, 256, 0, s>>>(dev_of(h), cf, h->CS, h->ncoef, h->ncoef, pr);
+    SLX_LAUNCH_CHECK();
+    src = h->CS;
+    cstride = h->ncoef;
   }
-  k_leg_syn<NF, NT, Coef><<<grid, NT, lds, s>>>(dev_of(h), seg_of(h), cf, h->FN + off, h->FSo + off, pr);
+  const long off = (long)field * h->nm * h->nh;
+  if constexpr (NF <= 2) {
+    if (h->syn_mfma) {
+      constexpr int R = 4;
+      dim3 grid((h->lmax + 2) / 2, (h->nh + 16 * R - 1) / (16 * R));
+      const int nl = h->lmax + 1, nlp = ((nl + 4 * MF_S - 1) / (4 * MF_S)) * (4 * MF_S);
+      const size_t lds = std::max((size_t)(2 + 2 * NF) * nlp * sizeof(double2), (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
+      static size_t lds_allowed = 65536;
+      if (lds > lds_allowed) {
+        SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leg_syn_mf<NF, R>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        lds_allowed = lds;
+      }
+      k_leg_syn_mf<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
+      SLX_LAUNCH_CHECK();
+      return 0;
+    }
+  }
+  dim3 grid((h->lmax + 2) / 2, (h->nh + 63) / 64);
+  k_leg_syn_s<NF, 1><<<grid, 256, 0, s>>>(dev_of(h), seg_of(h), h->rec, src, cstride, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }

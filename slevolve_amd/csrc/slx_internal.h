@@ -56,11 +56,18 @@ struct slx_sht {
   int* l_of = nullptr;    // [ncoef] degree of each packed index
   int* jb = nullptr;      // [nm] first ring (from the pole) with non-negligible P~_lm, multiple of 64
   double2* tw = nullptr;  // [nlon] exp(-2 pi i t / nlon)
+  int* m_of = nullptr;    // [ncoef] order of each packed index
   // degree segments of the contractions (order m, first degree l0) and recurrence seeds
   int nseg = 0, seg_len = 0, ana_nw = 1;
+  int syn_seg = 0;        // degrees per synthesis chain (a multiple of seg_len)
   int *seg_m = nullptr, *seg_l0 = nullptr, *seg_first = nullptr;
   double *seed_a = nullptr, *seed_b = nullptr;
   int* seed_s = nullptr;
+  // analysis on the matrix cores: groups of 4 consecutive segments of one order
+  int ana_mfma = 0, ana_r = 2, ngrp = 0;
+  int syn_mfma = 0;       // synthesis on the matrix cores (else wave-uniform scalar-fed VALU)
+  int *grp_m = nullptr, *grp_s0 = nullptr;
+  double2* CS = nullptr;  // [max_fields][ncoef] synthesis coefficients assembled by a loader
   double2* WS = nullptr;  // [max_fields][nm][nh] analysis: folded even parts
   double2* WD = nullptr;  // [max_fields][nm][nh] analysis: folded odd parts
   double2* FN = nullptr;  // [max_fields][nm][nh] synthesis: northern Fourier coefficients

@@ -15,11 +15,15 @@ def rand_coeffs(L, rng):
     return c
 
 
+@pytest.mark.parametrize("variant", ["mfma", "scalar"])
 @pytest.mark.parametrize("nlat,L", [(64, 63), (128, 63), (64, 31), (512, 511), (512, 512), (1024, 1023)])
-def test_synthesis_and_analysis_match_oracle(nlat, L):
+def test_synthesis_and_analysis_match_oracle(nlat, L, variant, monkeypatch):
+    """Both synthesis kernels (matrix-core and scalar-fed VALU; SLX_SYN_VARIANT is read when a
+    transform handle is created) and the analysis against the oracle."""
     from slevolve_amd import SHT
 
-    print(f"[sht] nlat {nlat} lmax {L}", flush=True)
+    monkeypatch.setenv("SLX_SYN_VARIANT", variant)
+    print(f"[sht] nlat {nlat} lmax {L} synthesis {variant}", flush=True)
     nlon = 2 * nlat
     rng = np.random.default_rng(nlat + L)
     gpu, ref = SHT(nlat, nlon, L), OracleSHT(nlat, nlon, L)
