@@ -1,4 +1,5 @@
 // Gauss-Legendre transform handle and its C-ABI (declared in include/slx.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -127,6 +128,21 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       }
     }
   }
+  std::vector<double> qc(h->ncoef, 0.0), qg(h->ncoef, 1.0);
+  for (int m = 0; m < nm; ++m) {
+    long double g = 1.0L;
+    for (int l = m + 2; l <= L; ++l) {
+      const int idx = h->h_moff[m] + (l - m);
+      const int ls = std::max(m + ((l - m) / MF_S) * MF_S, m + 2);
+      if (l == ls) g = 1.0L;
+      const long double lm1 = l - 1, mm = m, ll = l;
+      const long double ep = std::sqrt((lm1 * lm1 - mm * mm) / (4.0L * lm1 * lm1 - 1.0L));
+      const long double e = std::sqrt((ll * ll - mm * mm) / (4.0L * ll * ll - 1.0L));
+      qc[idx] = (double)((l == ls) ? ep : ep * ep);
+      g /= e;
+      qg[idx] = (double)g;
+    }
+  }
   std::vector<double2> tw(nlon);
   const double pi2 = 8.0 * std::atan(1.0);
   for (int t = 0; t < nlon; ++t) {
@@ -147,6 +163,8 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   ALLOC_COPY(h->moff, h->h_moff);
   ALLOC_COPY(h->l_of, l_of);
   ALLOC_COPY(h->m_of, m_of);
+  ALLOC_COPY(h->qc, qc);
+  ALLOC_COPY(h->qg, qg);
   ALLOC_COPY(h->jb, jb);
   ALLOC_COPY(h->tw, tw);
   // degree segments.  With nlat <= 1024 the analysis runs on the matrix cores with 32-degree
@@ -272,7 +290,7 @@ extern "C" int slx_sht_destroy(slx_sht* h) {
   if (!h) return 0;
   void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->tw,
                   h->WS, h->WD, h->FN, h->FSo, h->seg_m, h->seg_l0, h->seg_first, h->seed_a, h->seed_b,
-                  h->seed_s, h->m_of, h->grp_m, h->grp_s0, h->CS};
+                  h->seed_s, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;

@@ -112,6 +112,8 @@ struct ShtDev {
   const double2* tw;
   const int* l_of;  // degree of each packed index
   const int* m_of;  // order of each packed index
+  const double* qc;  // scaled recurrence coefficients (matrix-core kernels)
+  const double* qg;  // P~_l = qg_l Q_l
 };
 
 // -------------------------------------------------------------------------------------------
@@ -623,6 +625,78 @@ __device__ __forceinline__ void mf_block_reduce(double (&acc)[32], int lane) {
   }
 }
 
+// one step of the scaled recurrence: Q_l = x Q_{l-1} - c Q_{l-2}
+__device__ __forceinline__ double q_next(double x, double c, LegState& s) {
+  const double q = fma(x, s.pb, -(c * s.pa));
+  s.pa = s.pb;
+  s.pb = q;
+  return q;
+}
+// extended range of the scaled recurrence: renormalise once |P~| = |Q| g reaches 2^300
+__device__ __forceinline__ void q_renorm(LegState& s, double g) {
+  const bool big = (s.sc > 0) && (fabs(s.pb * g) >= 0x1p300);
+  const double f = big ? 0x1p-600 : 1.0;
+  s.pa *= f;
+  s.pb *= f;
+  s.sc -= big ? 1 : 0;
+}
+
+// Degree steps t of one segment group for R rings of this lane (slot = its segment row).
+// MASKED: some lane of the wave still carries an extended-range scale (values with sc > 0 are
+// masked out and the state is renormalised every 8 degrees); otherwise no lane ever will.
+// HEADG: the group holds the first segment of its order (lanes with head = true start with
+// P~_m, P~_{m+1}).  MFMAs of two consecutive steps are interleaved so that consecutive matrix
+// instructions update different accumulators.
+template <int R, bool MASKED, bool HEADG>
+__device__ __forceinline__ void ana_mf_steps(double (&acc)[MF_S], LegState (&st)[R], double (&mk)[R],
+                                             const double (&xr)[R], const double (&wv)[2][R],
+                                             const double* qcs, const double* qgs, int slot, bool head,
+                                             double s23) {
+  constexpr int S = MF_S;
+  double cc[8];
+#pragma unroll
+  for (int t = 0; t < S; t += 2) {
+    if ((t & 7) == 0) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cc[u] = qcs[slot * S + t + u];
+    }
+    double pv[2][R];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int tt = t + h;
+      const double c = cc[tt & 7];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        double p;
+        if (HEADG && tt == 0) {  // head: P_m itself, the state does not move
+          const double pa0 = st[r].pa, pb0 = st[r].pb;
+          const double pn = q_next(xr[r], c, st[r]);
+          p = head ? pb0 : pn;
+          st[r].pa = head ? pa0 : st[r].pa;
+          st[r].pb = head ? pb0 : st[r].pb;
+        } else if (HEADG && tt == 1) {  // head: P_{m+1} = sqrt(2m+3) x P_m
+          const double ps = s23 * xr[r] * st[r].pb;
+          const double pn = q_next(xr[r], c, st[r]);
+          p = head ? ps : pn;
+          st[r].pb = p;
+        } else {
+          p = q_next(xr[r], c, st[r]);
+        }
+        pv[h][r] = MASKED ? p * mk[r] : p;
+        if (MASKED && (tt & 7) == 7) {
+          q_renorm(st[r], qgs[slot * S + tt]);
+          mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      acc[t] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv[0][r], wv[0][r], acc[t], 0, 0, 0);
+      acc[t + 1] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv[1][r], wv[1][r], acc[t + 1], 0, 0, 0);
+    }
+  }
+}
+
 template <int NF, int R>
 __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS,
                                                     const double2* __restrict__ WD, OutPtrs<NF> out, Pred pr) {
@@ -630,7 +704,8 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   if (!pr.active()) return;
   constexpr int S = MF_S;
   extern __shared__ double red[];  // [NW][8][64] wave partials
-  __shared__ double2 rcs[4 * S];   // recurrence coefficients of the 4 slots
+  __shared__ double qcs[4 * S];    // scaled recurrence coefficients of the 4 slots
+  __shared__ double qgs[4 * S];    // and the factors P~ = qg Q
   const int per = (gr.ngrp + 7) >> 3;
   const int gi = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (gi >= gr.ngrp) return;
@@ -640,8 +715,9 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   const int nsm = (L - m + S) / S;  // segments of order m
   for (int i = threadIdx.x; i < 4 * S; i += blockDim.x) {
     const int off = (s0 + i / S) * S + i % S;  // l - m
-    const bool use = off >= 2 && off <= L - m;
-    rcs[i] = use ? d.rec[mo + off] : make_double2(0.0, 0.0);
+    const bool use = off <= L - m;
+    qcs[i] = use ? d.qc[mo + off] : 0.0;
+    qgs[i] = use ? d.qg[mo + off] : 0.0;
   }
   __syncthreads();
   const int blk = (lane >> 2) & 3, slot = lane & 3, kk = lane >> 4;
@@ -655,18 +731,27 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   const int fb = colok ? (col >> 1) : 0;
   double xr[R], mk[R], wv[2][R];
   LegState st[R];
+  bool anysc = false;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = (wave * R + r) * 16 + 4 * blk + kk;  // ring of this lane (A and B roles alike)
     const bool ok = j < nh;
     const int jj = ok ? j : 0;
     xr[r] = d.x[jj];
-    const double a0 = head ? 0.0 : g.sa[sx + jj];
-    const double b0 = head ? d.pmm[(long)m * nh + jj] : g.sb[sx + jj];
-    const int c0 = head ? d.pms[(long)m * nh + jj] : g.ss[sx + jj];
     const bool on = ok && sval;
-    st[r] = LegState{on ? a0 : 0.0, on ? b0 : 0.0, on ? c0 : 0};
-    mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+    double a0 = 0.0, b0 = 0.0;
+    int c0 = 0;
+    if (on && head) {
+      b0 = d.pmm[(long)m * nh + jj];
+      c0 = d.pms[(long)m * nh + jj];
+    } else if (on) {
+      a0 = g.sa[sx + jj];
+      b0 = g.sb[sx + jj];
+      c0 = g.ss[sx + jj];
+    }
+    st[r] = LegState{a0, b0, c0};
+    mk[r] = c0 == 0 ? 1.0 : 0.0;
+    anysc = anysc || (c0 > 0);
     const double2 vs = WS[fb * fstride + (long)m * nh + jj], vd = WD[fb * fstride + (long)m * nh + jj];
     const bool use = ok && colok;
     wv[0][r] = use ? ((col & 1) ? vs.y : vs.x) : 0.0;
@@ -676,37 +761,17 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   double acc[S];
 #pragma unroll
   for (int t = 0; t < S; ++t) acc[t] = 0.0;
-  double2 cc[8];  // recurrence coefficients of the current 8 degrees, read ahead from LDS
-#pragma unroll
-  for (int t = 0; t < S; ++t) {
-    if ((t & 7) == 0) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) cc[u] = rcs[slot * S + t + u];
-    }
-    const double2 c = cc[t & 7];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      double p;
-      if (t == 0) {  // head: P_m itself, the state does not move
-        const double pa0 = st[r].pa, pb0 = st[r].pb;
-        const double pn = leg_next(xr[r], c, st[r]);
-        p = head ? pb0 : pn;
-        st[r].pa = head ? pa0 : st[r].pa;
-        st[r].pb = head ? pb0 : st[r].pb;
-      } else if (t == 1) {  // head: P_{m+1} = sqrt(2m+3) x P_m
-        const double ps = s23 * xr[r] * st[r].pb;
-        const double pn = leg_next(xr[r], c, st[r]);
-        p = head ? ps : pn;
-        st[r].pb = p;
-      } else {
-        p = leg_next(xr[r], c, st[r]);
-      }
-      acc[t] = __builtin_amdgcn_mfma_f64_4x4x4f64(p * mk[r], wv[t & 1][r], acc[t], 0, 0, 0);
-      if ((t & 7) == 7) {
-        leg_renorm(st[r]);
-        mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
-      }
-    }
+  const bool masked = __any(anysc);  // wave-uniform
+  if (s0 == 0) {
+    if (masked)
+      ana_mf_steps<R, true, true>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
+    else
+      ana_mf_steps<R, false, true>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
+  } else {
+    if (masked)
+      ana_mf_steps<R, true, false>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
+    else
+      ana_mf_steps<R, false, false>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
   }
   mf_block_reduce(acc, lane);
   auto emit = [&](int v, int ln, double val) {
@@ -714,7 +779,8 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
     const int row = ln >> 4, cj = ln & 3;
     const int off = (s0 + row) * S + t;
     if ((s0 + row) < nsm && off <= L - m && (cj >> 1) < NF)
-      reinterpret_cast<double*>(out.p[cj >> 1])[2 * (long)(mo + off) + (cj & 1)] = val / SLX_SQRT2_LEGACY;
+      reinterpret_cast<double*>(out.p[cj >> 1])[2 * (long)(mo + off) + (cj & 1)] =
+          (val * qgs[row * S + t]) / SLX_SQRT2_LEGACY;
   };
   if (NW == 1) {
 #pragma unroll
@@ -919,6 +985,61 @@ __global__ void __launch_bounds__(256) k_coef_fill(ShtDev d, Coef cf, double2* _
 // groups of both orders are dealt to the waves, whose E/O partials are summed in LDS in a fixed
 // order.  Recurrence and spectral coefficients of both orders are staged in LDS, zero past L.
 // Outputs FN = 1.41421356 (E + O), FS = 1.41421356 (E - O), [f][m][j].
+// Degree steps of one segment group (synthesis).  Parities of t go to separate accumulators
+// aE (even l - m) and aO (odd); consecutive MFMAs alternate between them.
+template <int R, bool MASKED, bool HEADG>
+__device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], LegState (&st)[R],
+                                             double (&mk)[R], const double (&xr)[R], const double* qco,
+                                             const double* clo, const double* qgo, int qlim, bool colok,
+                                             bool head, double s23) {
+  constexpr int S = MF_S;
+  double cc[8], bb[8];
+#pragma unroll
+  for (int t = 0; t < S; t += 2) {
+    if ((t & 7) == 0) {  // staged rows are zero past the order's last degree
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        cc[u] = qco[t + u];
+        bb[u] = colok ? clo[2 * (t + u)] : 0.0;
+      }
+    }
+    double pv[2][R];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int tt = t + h;
+      const double c = cc[tt & 7];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        double p;
+        if (HEADG && tt == 0) {
+          const double pa0 = st[r].pa, pb0 = st[r].pb;
+          const double pn = q_next(xr[r], c, st[r]);
+          p = head ? pb0 : pn;
+          st[r].pa = head ? pa0 : st[r].pa;
+          st[r].pb = head ? pb0 : st[r].pb;
+        } else if (HEADG && tt == 1) {
+          const double ps = s23 * xr[r] * st[r].pb;
+          const double pn = q_next(xr[r], c, st[r]);
+          p = head ? ps : pn;
+          st[r].pb = p;
+        } else {
+          p = q_next(xr[r], c, st[r]);
+        }
+        pv[h][r] = MASKED ? p * mk[r] : p;
+        if (MASKED && (tt & 7) == 7) {
+          q_renorm(st[r], qgo[min(tt, qlim)]);
+          mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      aE[r] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv[0][r], bb[t & 7], aE[r], 0, 0, 0);
+      aO[r] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv[1][r], bb[(t + 1) & 7], aO[r], 0, 0, 0);
+    }
+  }
+}
+
 template <int NF, int R>
 __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride,
                                                     double2* __restrict__ FN, double2* __restrict__ FS, Pred pr) {
@@ -928,8 +1049,8 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
   extern __shared__ double2 lsy[];
   const int L = d.lmax, nl = L + 1, nh = d.nh;
   const int nlp = ((nl + 4 * S - 1) / (4 * S)) * (4 * S);
-  double2* rc = lsy;                // [2][nlp]
-  double2* cl = lsy + 2 * nlp;      // [2][NF][nlp]
+  double* qcl = reinterpret_cast<double*>(lsy);  // [2][nlp] scaled recurrence coefficients
+  double2* cl = lsy + nlp;                       // [2][NF][nlp] coefficients times qg
   double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
   const int p = blockIdx.x;
   const int mord[2] = {p, L - p};
@@ -938,9 +1059,13 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
     const int m = mord[o], mo = d.moff[m], cnt = nl - m;
     for (int i = threadIdx.x; i < nlp; i += 256) {
       const bool v = (o < nord) && (i < cnt);
-      rc[o * nlp + i] = (v && i >= 2) ? d.rec[mo + i] : make_double2(0.0, 0.0);
+      qcl[o * nlp + i] = v ? d.qc[mo + i] : 0.0;
+      const double gq = v ? d.qg[mo + i] : 0.0;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) cl[(o * NF + f) * nlp + i] = v ? cf[f * cstride + mo + i] : make_double2(0.0, 0.0);
+      for (int f = 0; f < NF; ++f) {
+        const double2 c = v ? cf[f * cstride + mo + i] : make_double2(0.0, 0.0);
+        cl[(o * NF + f) * nlp + i] = make_double2(c.x * gq, c.y * gq);
+      }
     }
   }
   __syncthreads();
@@ -970,7 +1095,7 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
   for (int it = wave; it < ng0 + ng1; it += 4) {
     const int o = it < ng0 ? 0 : 1;
     const int gq = it < ng0 ? it : it - ng0;
-    const int m = mord[o];
+    const int m = mord[o], mo = d.moff[m];
     const int nsm = (nl - m + S - 1) / S;
     const int sg = 4 * gq + slot;
     const bool sval = sg < nsm;
@@ -978,59 +1103,42 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
     const long sx = (long)(g.first[m] + (sval ? sg : 0)) * nh;
     LegState st[R];
     double mk[R], aE[R], aO[R];
+    bool anysc = false;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int jj = okr[r] ? jr[r] : 0;
-      const double a0 = head ? 0.0 : g.sa[sx + jj];
-      const double b0 = head ? d.pmm[(long)m * nh + jj] : g.sb[sx + jj];
-      const int c0 = head ? d.pms[(long)m * nh + jj] : g.ss[sx + jj];
       const bool on = okr[r] && sval;
-      st[r] = LegState{on ? a0 : 0.0, on ? b0 : 0.0, on ? c0 : 0};
-      mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
+      double a0 = 0.0, b0 = 0.0;
+      int c0 = 0;
+      if (on && head) {
+        b0 = d.pmm[(long)m * nh + jj];
+        c0 = d.pms[(long)m * nh + jj];
+      } else if (on) {
+        a0 = g.sa[sx + jj];
+        b0 = g.sb[sx + jj];
+        c0 = g.ss[sx + jj];
+      }
+      st[r] = LegState{a0, b0, c0};
+      mk[r] = c0 == 0 ? 1.0 : 0.0;
+      anysc = anysc || (c0 > 0);
       aE[r] = aO[r] = 0.0;
     }
     const double s23 = d.s2m3[m];
-    const double2* rco = rc + o * nlp + sg * S;
+    const double* qco = qcl + o * nlp + sg * S;
     const double* clo = reinterpret_cast<const double*>(cl + (o * NF + (colok ? fb : 0)) * nlp + sg * S) + comp;
-    double2 cc[8];
-    double bb[8];
-#pragma unroll
-    for (int t = 0; t < S; ++t) {
-      if ((t & 7) == 0) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          cc[u] = sval ? rco[t + u] : make_double2(0.0, 0.0);
-          bb[u] = (sval && colok) ? clo[2 * (t + u)] : 0.0;
-        }
-      }
-      const double2 c = cc[t & 7];
-      const double bv = bb[t & 7];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        double pv;
-        if (t == 0) {
-          const double pa0 = st[r].pa, pb0 = st[r].pb;
-          const double pn = leg_next(xr[r], c, st[r]);
-          pv = head ? pb0 : pn;
-          st[r].pa = head ? pa0 : st[r].pa;
-          st[r].pb = head ? pb0 : st[r].pb;
-        } else if (t == 1) {
-          const double ps = s23 * xr[r] * st[r].pb;
-          const double pn = leg_next(xr[r], c, st[r]);
-          pv = head ? ps : pn;
-          st[r].pb = pv;
-        } else {
-          pv = leg_next(xr[r], c, st[r]);
-        }
-        if ((t & 1) == 0)
-          aE[r] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv * mk[r], bv, aE[r], 0, 0, 0);
-        else
-          aO[r] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv * mk[r], bv, aO[r], 0, 0, 0);
-        if ((t & 7) == 7) {
-          leg_renorm(st[r]);
-          mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
-        }
-      }
+    const double* qgo = d.qg + mo + (sval ? sg * S : 0);  // global, read on the masked path only
+    const int qlim = sval ? L - m - sg * S : 0;           // last valid entry of qgo
+    const bool masked = __any(anysc);
+    if (gq == 0) {
+      if (masked)
+        syn_mf_steps<R, true, true>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
+      else
+        syn_mf_steps<R, false, true>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
+    } else {
+      if (masked)
+        syn_mf_steps<R, true, false>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
+      else
+        syn_mf_steps<R, false, false>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
     }
     if (o == 0) {
 #pragma unroll

@@ -27,6 +27,8 @@ inline ShtDev dev_of(const slx_sht* h) {
   d.tw = h->tw;
   d.l_of = h->l_of;
   d.m_of = h->m_of;
+  d.qc = h->qc;
+  d.qg = h->qg;
   return d;
 }
 
@@ -153,7 +155,8 @@ int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field 
       constexpr int R = 4;
       dim3 grid((h->lmax + 2) / 2, (h->nh + 16 * R - 1) / (16 * R));
       const int nl = h->lmax + 1, nlp = ((nl + 4 * MF_S - 1) / (4 * MF_S)) * (4 * MF_S);
-      const size_t lds = std::max((size_t)(2 + 2 * NF) * nlp * sizeof(double2), (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
+      const size_t lds = std::max((size_t)nlp * sizeof(double2) + (size_t)2 * NF * nlp * sizeof(double2),
+                                  (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
       static size_t lds_allowed = 65536;
       if (lds > lds_allowed) {
         SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leg_syn_mf<NF, R>),

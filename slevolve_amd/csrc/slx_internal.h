@@ -57,6 +57,12 @@ struct slx_sht {
   int* jb = nullptr;      // [nm] first ring (from the pole) with non-negligible P~_lm, multiple of 64
   double2* tw = nullptr;  // [nlon] exp(-2 pi i t / nlon)
   int* m_of = nullptr;    // [ncoef] order of each packed index
+  // scaled recurrence of the matrix-core kernels, restarted at every 32-degree segment start
+  // ls = max(l0, m + 2) from the unscaled state (P~_{ls-2}, P~_{ls-1}):
+  //   Q_l = x Q_{l-1} - qc_l Q_{l-2},  P~_l = qg_l Q_l,
+  //   qc_ls = eps_{ls-1}, qc_l = eps_{l-1}^2 (l > ls), qg_l = prod_{k = ls..l} 1 / eps_k
+  double* qc = nullptr;   // [ncoef]
+  double* qg = nullptr;   // [ncoef] (1 for l < m + 2)
   // degree segments of the contractions (order m, first degree l0) and recurrence seeds
   int nseg = 0, seg_len = 0, ana_nw = 1;
   int syn_seg = 0;        // degrees per synthesis chain (a multiple of seg_len)
