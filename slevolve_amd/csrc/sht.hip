@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "sht_launch.h"
 #include "slx.h"
@@ -186,12 +187,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     while (S > 32 && count(S) * h->ana_nw < 2048) S /= 2;
   }
   h->seg_len = S;
-  // synthesis variant: matrix cores when the segments are the matrix-core ones, unless the
-  // SLX_SYN_VARIANT environment variable asks for the scalar-fed VALU kernel ("scalar")
-  {
-    const char* v = std::getenv("SLX_SYN_VARIANT");
-    h->syn_mfma = (h->ana_mfma && !(v && std::strcmp(v, "scalar") == 0)) ? 1 : 0;
-  }
+  h->syn_mfma = h->ana_mfma;
   h->syn_seg = S * std::max(1, (int)std::lround((L + 2) / (8.0 * S)));
   std::vector<int> seg_m, seg_l0, first(nm);
   for (int m = 0; m < nm; ++m) {
@@ -202,17 +198,6 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     }
   }
   h->nseg = (int)seg_m.size();
-  {
-    std::vector<int> gm, gs;
-    for (int m = 0; m < nm; ++m)
-      for (int s0 = 0; s0 < (L - m + S) / S; s0 += 4) {
-        gm.push_back(m);
-        gs.push_back(s0);
-      }
-    h->ngrp = (int)gm.size();
-    ALLOC_COPY(h->grp_m, gm);
-    ALLOC_COPY(h->grp_s0, gs);
-  }
   ALLOC_COPY(h->seg_m, seg_m);
   ALLOC_COPY(h->seg_l0, seg_l0);
   ALLOC_COPY(h->seg_first, first);
@@ -227,6 +212,18 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     k_leg_seed<<<grid, 64>>>(dev_of(h), seg_of(h), h->seed_a, h->seed_b, h->seed_s);
     SLX_LAUNCH_CHECK();
     SLX_HIP(hipDeviceSynchronize());
+  }
+  // matrix-core analysis groups: 4 consecutive segments of one order
+  {
+    std::vector<int> gm, gs;
+    for (int m = 0; m < nm; ++m)
+      for (int s0 = 0; s0 < (L - m + S) / S; s0 += 4) {
+        gm.push_back(m);
+        gs.push_back(s0);
+      }
+    h->ngrp = (int)gm.size();
+    ALLOC_COPY(h->grp_m, gm);
+    ALLOC_COPY(h->grp_s0, gs);
   }
 #undef ALLOC_COPY
   SLX_HIP(hipMalloc(&h->WS, wsz));
@@ -277,6 +274,44 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
         h->ana_r = 2 * best_v;
       else
         h->ana_nw = best_v;
+    }
+    // synthesis kernel: scalar-fed VALU, or matrix cores with 2 or 4 rings per lane (the latter
+    // only with the matrix-core segments).  SLX_SYN_VARIANT = scalar | mfma2 | mfma4 | mfma fixes
+    // the choice (mfma: rings per lane still timed); otherwise the fastest is kept.
+    {
+      const char* v = std::getenv("SLX_SYN_VARIANT");
+      const std::string want = v ? v : "";
+      struct Cand { int mf, r; };
+      std::vector<Cand> cand;
+      if (want == "scalar" || !h->ana_mfma) cand = {{0, 4}};
+      else if (want == "mfma2") cand = {{1, 2}};
+      else if (want == "mfma4") cand = {{1, 4}};
+      else if (want == "mfma") cand = {{1, 2}, {1, 4}};
+      else cand = {{0, 4}, {1, 2}, {1, 4}};
+      float bs = 1e30f;
+      Cand bc = cand[0];
+      for (const Cand& c : cand) {
+        if (cand.size() == 1) break;
+        h->syn_mfma = c.mf;
+        h->syn_r = c.r;
+        int rc = launch_leg_syn<1>(h, CoefPlain{tmp, 0}, Pred{}, nullptr);
+        if (rc) return rc;
+        SLX_HIP(hipEventRecord(e0, nullptr));
+        for (int rep = 0; rep < 3; ++rep) {
+          rc = launch_leg_syn<1>(h, CoefPlain{tmp, 0}, Pred{}, nullptr);
+          if (rc) return rc;
+        }
+        SLX_HIP(hipEventRecord(e1, nullptr));
+        SLX_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SLX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < bs) {
+          bs = ms;
+          bc = c;
+        }
+      }
+      h->syn_mfma = bc.mf;
+      h->syn_r = bc.r;
     }
     SLX_HIP(hipEventDestroy(e0));
     SLX_HIP(hipEventDestroy(e1));

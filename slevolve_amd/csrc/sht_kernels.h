@@ -641,6 +641,22 @@ __device__ __forceinline__ void q_renorm(LegState& s, double g) {
   s.sc -= big ? 1 : 0;
 }
 
+// Start state of a segment for the matrix-core kernels, which carry no scale counts: a lane whose
+// scale count is still > 0 at the next segment start (sc_next) stays below 2^-300 through the
+// segment and starts from zero (its terms are below 2^-300 relative and are dropped, as the
+// masked recurrence drops them); a lane whose scale count reaches 0 inside the segment started it
+// with s = 1 (the functions grow by far less than 2^600 over 32 degrees), so its true value
+// P~ 2^-600 lies in [2^-900, 2^-300), a normal double: it runs unscaled, keeping those terms.
+__device__ __forceinline__ void mf_unscale(double& a, double& b, int sc, int sc_next) {
+  if (sc <= 0) return;
+  if (sc_next > 0) {
+    a = b = 0.0;
+  } else {
+    a = ldexp(a, -SLX_SCALE_K * sc);
+    b = ldexp(b, -SLX_SCALE_K * sc);
+  }
+}
+
 // Degree steps t of one segment group for R rings of this lane (slot = its segment row).
 // MASKED: some lane of the wave still carries an extended-range scale (values with sc > 0 are
 // masked out and the state is renormalised every 8 degrees); otherwise no lane ever will.
@@ -731,7 +747,6 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   const int fb = colok ? (col >> 1) : 0;
   double xr[R], mk[R], wv[2][R];
   LegState st[R];
-  bool anysc = false;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = (wave * R + r) * 16 + 4 * blk + kk;  // ring of this lane (A and B roles alike)
@@ -749,9 +764,9 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
       b0 = g.sb[sx + jj];
       c0 = g.ss[sx + jj];
     }
-    st[r] = LegState{a0, b0, c0};
-    mk[r] = c0 == 0 ? 1.0 : 0.0;
-    anysc = anysc || (c0 > 0);
+    mf_unscale(a0, b0, c0, segi + 1 < nsm ? g.ss[sx + nh + jj] : 0);
+    st[r] = LegState{a0, b0, 0};
+    mk[r] = 1.0;
     const double2 vs = WS[fb * fstride + (long)m * nh + jj], vd = WD[fb * fstride + (long)m * nh + jj];
     const bool use = ok && colok;
     wv[0][r] = use ? ((col & 1) ? vs.y : vs.x) : 0.0;
@@ -761,18 +776,10 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   double acc[S];
 #pragma unroll
   for (int t = 0; t < S; ++t) acc[t] = 0.0;
-  const bool masked = __any(anysc);  // wave-uniform
-  if (s0 == 0) {
-    if (masked)
-      ana_mf_steps<R, true, true>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
-    else
-      ana_mf_steps<R, false, true>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
-  } else {
-    if (masked)
-      ana_mf_steps<R, true, false>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
-    else
-      ana_mf_steps<R, false, false>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
-  }
+  if (s0 == 0)
+    ana_mf_steps<R, false, true>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
+  else
+    ana_mf_steps<R, false, false>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
   mf_block_reduce(acc, lane);
   auto emit = [&](int v, int ln, double val) {
     const int t = 16 * ((ln >> 3) & 1) + 8 * ((ln >> 2) & 1) + v;
@@ -1103,7 +1110,6 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
     const long sx = (long)(g.first[m] + (sval ? sg : 0)) * nh;
     LegState st[R];
     double mk[R], aE[R], aO[R];
-    bool anysc = false;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int jj = okr[r] ? jr[r] : 0;
@@ -1118,9 +1124,9 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
         b0 = g.sb[sx + jj];
         c0 = g.ss[sx + jj];
       }
-      st[r] = LegState{a0, b0, c0};
-      mk[r] = c0 == 0 ? 1.0 : 0.0;
-      anysc = anysc || (c0 > 0);
+      mf_unscale(a0, b0, c0, sg + 1 < nsm ? g.ss[sx + nh + jj] : 0);
+      st[r] = LegState{a0, b0, 0};
+      mk[r] = 1.0;
       aE[r] = aO[r] = 0.0;
     }
     const double s23 = d.s2m3[m];
@@ -1128,18 +1134,10 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
     const double* clo = reinterpret_cast<const double*>(cl + (o * NF + (colok ? fb : 0)) * nlp + sg * S) + comp;
     const double* qgo = d.qg + mo + (sval ? sg * S : 0);  // global, read on the masked path only
     const int qlim = sval ? L - m - sg * S : 0;           // last valid entry of qgo
-    const bool masked = __any(anysc);
-    if (gq == 0) {
-      if (masked)
-        syn_mf_steps<R, true, true>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
-      else
-        syn_mf_steps<R, false, true>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
-    } else {
-      if (masked)
-        syn_mf_steps<R, true, false>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
-      else
-        syn_mf_steps<R, false, false>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
-    }
+    if (gq == 0)
+      syn_mf_steps<R, false, true>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
+    else
+      syn_mf_steps<R, false, false>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
     if (o == 0) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {

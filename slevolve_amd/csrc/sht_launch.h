@@ -134,6 +134,23 @@ int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
   SLX_LAUNCH_CHECK();
   return 0;
 }
+template <int NF, int R>
+int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long off, Pred pr, hipStream_t s) {
+  dim3 grid((h->lmax + 2) / 2, (h->nh + 16 * R - 1) / (16 * R));
+  const int nl = h->lmax + 1, nlp = ((nl + 4 * MF_S - 1) / (4 * MF_S)) * (4 * MF_S);
+  const size_t lds = std::max((size_t)nlp * sizeof(double2) + (size_t)2 * NF * nlp * sizeof(double2),
+                              (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
+  static size_t lds_allowed = 65536;
+  if (lds > lds_allowed) {
+    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leg_syn_mf<NF, R>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_allowed = lds;
+  }
+  k_leg_syn_mf<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
 // Synthesis contraction of NF fields (FN/FS fields field .. field + NF - 1).  A CoefPlain
 // source is read in place; any other loader is first assembled into the handle's CS array.
 template <int NF, class Coef>
@@ -151,22 +168,8 @@ int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field 
   }
   const long off = (long)field * h->nm * h->nh;
   if constexpr (NF <= 2) {
-    if (h->syn_mfma) {
-      constexpr int R = 4;
-      dim3 grid((h->lmax + 2) / 2, (h->nh + 16 * R - 1) / (16 * R));
-      const int nl = h->lmax + 1, nlp = ((nl + 4 * MF_S - 1) / (4 * MF_S)) * (4 * MF_S);
-      const size_t lds = std::max((size_t)nlp * sizeof(double2) + (size_t)2 * NF * nlp * sizeof(double2),
-                                  (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
-      static size_t lds_allowed = 65536;
-      if (lds > lds_allowed) {
-        SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leg_syn_mf<NF, R>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        lds_allowed = lds;
-      }
-      k_leg_syn_mf<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
-      SLX_LAUNCH_CHECK();
-      return 0;
-    }
+    if (h->syn_mfma) return h->syn_r == 2 ? launch_leg_syn_mf<NF, 2>(h, src, cstride, off, pr, s)
+                                          : launch_leg_syn_mf<NF, 4>(h, src, cstride, off, pr, s);
   }
   dim3 grid((h->lmax + 2) / 2, (h->nh + 63) / 64);
   k_leg_syn_s<NF, 1><<<grid, 256, 0, s>>>(dev_of(h), seg_of(h), h->rec, src, cstride, h->FN + off, h->FSo + off, pr);

@@ -72,6 +72,7 @@ struct slx_sht {
   // analysis on the matrix cores: groups of 4 consecutive segments of one order
   int ana_mfma = 0, ana_r = 2, ngrp = 0;
   int syn_mfma = 0;       // synthesis on the matrix cores (else wave-uniform scalar-fed VALU)
+  int syn_r = 4;          // rings per lane of the matrix-core synthesis (2 or 4)
   int *grp_m = nullptr, *grp_s0 = nullptr;
   double2* CS = nullptr;  // [max_fields][ncoef] synthesis coefficients assembled by a loader
   double2* WS = nullptr;  // [max_fields][nm][nh] analysis: folded even parts

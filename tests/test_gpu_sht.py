@@ -15,7 +15,7 @@ def rand_coeffs(L, rng):
     return c
 
 
-@pytest.mark.parametrize("variant", ["mfma", "scalar"])
+@pytest.mark.parametrize("variant", ["mfma2", "mfma4", "scalar"])
 @pytest.mark.parametrize("nlat,L", [(64, 63), (128, 63), (64, 31), (512, 511), (512, 512), (1024, 1023)])
 def test_synthesis_and_analysis_match_oracle(nlat, L, variant, monkeypatch):
     """Both synthesis kernels (matrix-core and scalar-fed VALU; SLX_SYN_VARIANT is read when a
