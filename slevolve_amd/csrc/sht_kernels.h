@@ -372,176 +372,6 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, SegDev g, const d
 }
 
 // -------------------------------------------------------------------------------------------
-// Synthesis contraction.  Block (order pair m1 = p, m2 = L - p; ring block of NT rings): the two
-// orders' coefficients (through the Coef loader, which may assemble them from several spectral
-// arrays) and recurrence coefficients are staged in LDS, zero-padded by 8 past the last degree;
-// each thread owns one northern ring and walks the degree segments of both orders, four segments
-// interleaved (independent recurrences from their seeds), in blocks of 8 degrees so that parity
-// and renormalisation points are compile-time; E (even l - m) and O (odd l - m) are accumulated
-// per order in a fixed order.  Outputs FN = 1.41421356 (E + O) (north), FS = 1.41421356 (E - O)
-// (south), [f][m][j].  Coef: double2 operator()(int f, int idx, int l, int m) const.
-template <int NF>
-struct SynChains {
-  LegState st[4];
-  double mk[4];
-  int o[4], base[4], len8[4], zpad[4];
-  bool head[4];
-  double2 E[4][NF], O[4][NF];
-};
-
-template <int NF, bool FIRST>
-__device__ __forceinline__ void syn_block8(SynChains<NF>& ch, int t0, double x, const double* s23, const double2* cl,
-                                           const double2* rc, int nlp) {
-  int ib[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    // past a chain's end read the zero padding of its order (contributes nothing)
-    ib[q] = ch.o[q] * nlp + (t0 < ch.len8[q] ? ch.base[q] + t0 : ch.zpad[q]);
-  }
-#pragma unroll
-  for (int il = 0; il < 8; ++il) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = ib[q] + il;
-      double p;
-      if (FIRST && il == 0) {
-        const double pa0 = ch.st[q].pa, pb0 = ch.st[q].pb;
-        const double pn = leg_next(x, rc[i], ch.st[q]);
-        const bool h = ch.head[q];
-        p = h ? pb0 : pn;
-        ch.st[q].pa = h ? pa0 : ch.st[q].pa;
-        ch.st[q].pb = h ? pb0 : ch.st[q].pb;
-      } else if (FIRST && il == 1) {
-        const double ps = s23[ch.o[q]] * x * ch.st[q].pb;
-        const double pn = leg_next(x, rc[i], ch.st[q]);
-        p = ch.head[q] ? ps : pn;
-        ch.st[q].pb = p;
-      } else {
-        p = leg_next(x, rc[i], ch.st[q]);
-      }
-      p *= ch.mk[q];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const double2 cv = cl[i + f * nlp + ch.o[q] * (NF - 1) * nlp];
-        if ((il & 1) == 0) {
-          ch.E[q][f].x += p * cv.x;
-          ch.E[q][f].y += p * cv.y;
-        } else {
-          ch.O[q][f].x += p * cv.x;
-          ch.O[q][f].y += p * cv.y;
-        }
-      }
-      if (il == 7) {
-        leg_renorm(ch.st[q]);
-        ch.mk[q] = ch.st[q].sc == 0 ? 1.0 : 0.0;
-      }
-    }
-  }
-}
-
-template <int NF, int NT, class Coef>
-__global__ void __launch_bounds__(NT) k_leg_syn(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN,
-                                                double2* __restrict__ FS, Pred pr) {
-  if (!pr.active()) return;
-  extern __shared__ double2 lds[];
-  const int L = d.lmax, nl = L + 1, S = g.S, nlp = nl + 8;
-  const int p = blockIdx.x;
-  const int mord[2] = {p, L - p};
-  const int nord = (L - p == p) ? 1 : 2;
-  double2* cl = lds;                 // [2][NF][nlp]
-  double2* rc = lds + 2 * NF * nlp;  // [2][nlp]
-  __shared__ double pro[32];  // scratch of the coefficient loader's prologue
-  cf.prologue(pro);
-  __syncthreads();
-  for (int o = 0; o < 2; ++o) {
-    const int m = mord[o], mo = d.moff[m], cnt = nl - m;
-    for (int i = threadIdx.x; i < nlp; i += NT) {
-      const bool v = (o < nord) && (i < cnt);
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
-        cl[(o * NF + f) * nlp + i] = v ? cf(f, mo + i, m + i, m, pro) : make_double2(0.0, 0.0);
-      rc[o * nlp + i] = v ? d.rec[mo + i] : make_double2(0.0, 0.0);
-    }
-  }
-  __syncthreads();
-  const int j = blockIdx.y * NT + threadIdx.x;
-  if (j >= d.nh) return;
-  const double x = d.x[j];
-  const int nh = d.nh;
-  const double s23[2] = {d.s2m3[mord[0]], d.s2m3[mord[1]]};
-  double2 E[2][NF], O[2][NF];
-#pragma unroll
-  for (int o = 0; o < 2; ++o)
-#pragma unroll
-    for (int f = 0; f < NF; ++f) E[o][f] = O[o][f] = make_double2(0.0, 0.0);
-  const int ns0 = (nl - mord[0] + S - 1) / S;
-  const int ns1 = nord == 2 ? (nl - mord[1] + S - 1) / S : 0;
-  const int C = ns0 + ns1;
-  for (int c0 = 0; c0 < C; c0 += 4) {
-    SynChains<NF> ch;
-    int tmax = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = c0 + q;
-      const bool act = c < C;
-      const int o = (c < ns0) ? 0 : 1;
-      const int s = (c < ns0) ? c : c - ns0;
-      const int m = mord[o];
-      const int seg = act ? g.first[m] + s : 0;
-      const int l0 = m + s * S;
-      const int len = act ? min(S, nl - l0) : 0;
-      ch.o[q] = o;
-      ch.base[q] = s * S;
-      ch.len8[q] = (len + 7) & ~7;
-      ch.zpad[q] = nl - m;  // first zero entry of the order's padded rows
-      ch.head[q] = act && (s == 0);
-      const long ix = (long)m * nh + j, sx = (long)seg * nh + j;
-      if (act && s == 0) {
-        ch.st[q] = LegState{0.0, d.pmm[ix], d.pms[ix]};
-      } else if (act) {
-        ch.st[q] = LegState{g.sa[sx], g.sb[sx], g.ss[sx]};
-      } else {
-        ch.st[q] = LegState{0.0, 0.0, 0};
-      }
-      ch.mk[q] = ch.st[q].sc == 0 ? 1.0 : 0.0;
-      tmax = max(tmax, ch.len8[q]);
-#pragma unroll
-      for (int f = 0; f < NF; ++f) ch.E[q][f] = ch.O[q][f] = make_double2(0.0, 0.0);
-    }
-    syn_block8<NF, true>(ch, 0, x, s23, cl, rc, nlp);
-    for (int t0 = 8; t0 < tmax; t0 += 8) syn_block8<NF, false>(ch, t0, x, s23, cl, rc, nlp);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (ch.len8[q] > 0) {
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          if (ch.o[q] == 0) {
-            E[0][f] = cadd(E[0][f], ch.E[q][f]);
-            O[0][f] = cadd(O[0][f], ch.O[q][f]);
-          } else {
-            E[1][f] = cadd(E[1][f], ch.E[q][f]);
-            O[1][f] = cadd(O[1][f], ch.O[q][f]);
-          }
-        }
-      }
-    }
-  }
-  const long fstride = (long)d.nm * nh;
-#pragma unroll
-  for (int o = 0; o < 2; ++o) {
-    if (o < nord) {
-      const int m = mord[o];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const long idx = f * fstride + (long)m * nh + j;
-        FN[idx] = make_double2(SLX_SQRT2_LEGACY * (E[o][f].x + O[o][f].x), SLX_SQRT2_LEGACY * (E[o][f].y + O[o][f].y));
-        FS[idx] = make_double2(SLX_SQRT2_LEGACY * (E[o][f].x - O[o][f].x), SLX_SQRT2_LEGACY * (E[o][f].y - O[o][f].y));
-      }
-    }
-  }
-}
-
-// -------------------------------------------------------------------------------------------
 // Synthesis back end: one block per (northern ring j, field f).  Builds the Hermitian spectra of
 // rows j and nlat-1-j (Im of m = 0 and of the Nyquist bin dropped, as irfft does), packs them as
 // Z = Xn + i Xs, inverse-transforms once and hands (row, column, value) to the Store functor:
@@ -632,15 +462,6 @@ __device__ __forceinline__ double q_next(double x, double c, LegState& s) {
   s.pb = q;
   return q;
 }
-// extended range of the scaled recurrence: renormalise once |P~| = |Q| g reaches 2^300
-__device__ __forceinline__ void q_renorm(LegState& s, double g) {
-  const bool big = (s.sc > 0) && (fabs(s.pb * g) >= 0x1p300);
-  const double f = big ? 0x1p-600 : 1.0;
-  s.pa *= f;
-  s.pb *= f;
-  s.sc -= big ? 1 : 0;
-}
-
 // Start state of a segment for the matrix-core kernels, which carry no scale counts: a lane whose
 // scale count is still > 0 at the next segment start (sc_next) stays below 2^-300 through the
 // segment and starts from zero (its terms are below 2^-300 relative and are dropped, as the
@@ -658,15 +479,12 @@ __device__ __forceinline__ void mf_unscale(double& a, double& b, int sc, int sc_
 }
 
 // Degree steps t of one segment group for R rings of this lane (slot = its segment row).
-// MASKED: some lane of the wave still carries an extended-range scale (values with sc > 0 are
-// masked out and the state is renormalised every 8 degrees); otherwise no lane ever will.
 // HEADG: the group holds the first segment of its order (lanes with head = true start with
 // P~_m, P~_{m+1}).  MFMAs of two consecutive steps are interleaved so that consecutive matrix
 // instructions update different accumulators.
-template <int R, bool MASKED, bool HEADG>
-__device__ __forceinline__ void ana_mf_steps(double (&acc)[MF_S], LegState (&st)[R], double (&mk)[R],
-                                             const double (&xr)[R], const double (&wv)[2][R],
-                                             const double* qcs, const double* qgs, int slot, bool head,
+template <int R, bool HEADG>
+__device__ __forceinline__ void ana_mf_steps(double (&acc)[MF_S], LegState (&st)[R], const double (&xr)[R],
+                                             const double (&wv)[2][R], const double* qcs, int slot, bool head,
                                              double s23) {
   constexpr int S = MF_S;
   double cc[8];
@@ -698,11 +516,7 @@ __device__ __forceinline__ void ana_mf_steps(double (&acc)[MF_S], LegState (&st)
         } else {
           p = q_next(xr[r], c, st[r]);
         }
-        pv[h][r] = MASKED ? p * mk[r] : p;
-        if (MASKED && (tt & 7) == 7) {
-          q_renorm(st[r], qgs[slot * S + tt]);
-          mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
-        }
+        pv[h][r] = p;
       }
     }
 #pragma unroll
@@ -745,7 +559,7 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   const int col = slot;  // B column of this lane (same lane bits as its A row)
   const bool colok = (col >> 1) < NF;
   const int fb = colok ? (col >> 1) : 0;
-  double xr[R], mk[R], wv[2][R];
+  double xr[R], wv[2][R];
   LegState st[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -766,7 +580,6 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
     }
     mf_unscale(a0, b0, c0, segi + 1 < nsm ? g.ss[sx + nh + jj] : 0);
     st[r] = LegState{a0, b0, 0};
-    mk[r] = 1.0;
     const double2 vs = WS[fb * fstride + (long)m * nh + jj], vd = WD[fb * fstride + (long)m * nh + jj];
     const bool use = ok && colok;
     wv[0][r] = use ? ((col & 1) ? vs.y : vs.x) : 0.0;
@@ -777,9 +590,9 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
 #pragma unroll
   for (int t = 0; t < S; ++t) acc[t] = 0.0;
   if (s0 == 0)
-    ana_mf_steps<R, false, true>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
+    ana_mf_steps<R, true>(acc, st, xr, wv, qcs, slot, head, s23);
   else
-    ana_mf_steps<R, false, false>(acc, st, mk, xr, wv, qcs, qgs, slot, head, s23);
+    ana_mf_steps<R, false>(acc, st, xr, wv, qcs, slot, head, s23);
   mf_block_reduce(acc, lane);
   auto emit = [&](int v, int ln, double val) {
     const int t = 16 * ((ln >> 3) & 1) + 8 * ((ln >> 2) & 1) + v;
@@ -994,11 +807,10 @@ __global__ void __launch_bounds__(256) k_coef_fill(ShtDev d, Coef cf, double2* _
 // Outputs FN = 1.41421356 (E + O), FS = 1.41421356 (E - O), [f][m][j].
 // Degree steps of one segment group (synthesis).  Parities of t go to separate accumulators
 // aE (even l - m) and aO (odd); consecutive MFMAs alternate between them.
-template <int R, bool MASKED, bool HEADG>
+template <int R, bool HEADG>
 __device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], LegState (&st)[R],
-                                             double (&mk)[R], const double (&xr)[R], const double* qco,
-                                             const double* clo, const double* qgo, int qlim, bool colok,
-                                             bool head, double s23) {
+                                             const double (&xr)[R], const double* qco, const double* clo,
+                                             bool colok, bool head, double s23) {
   constexpr int S = MF_S;
   double cc[8], bb[8];
 #pragma unroll
@@ -1032,11 +844,7 @@ __device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], L
         } else {
           p = q_next(xr[r], c, st[r]);
         }
-        pv[h][r] = MASKED ? p * mk[r] : p;
-        if (MASKED && (tt & 7) == 7) {
-          q_renorm(st[r], qgo[min(tt, qlim)]);
-          mk[r] = st[r].sc == 0 ? 1.0 : 0.0;
-        }
+        pv[h][r] = p;
       }
     }
 #pragma unroll
@@ -1109,7 +917,7 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
     const bool head = sval && sg == 0;
     const long sx = (long)(g.first[m] + (sval ? sg : 0)) * nh;
     LegState st[R];
-    double mk[R], aE[R], aO[R];
+    double aE[R], aO[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int jj = okr[r] ? jr[r] : 0;
@@ -1126,18 +934,15 @@ __global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const do
       }
       mf_unscale(a0, b0, c0, sg + 1 < nsm ? g.ss[sx + nh + jj] : 0);
       st[r] = LegState{a0, b0, 0};
-      mk[r] = 1.0;
       aE[r] = aO[r] = 0.0;
     }
     const double s23 = d.s2m3[m];
     const double* qco = qcl + o * nlp + sg * S;
     const double* clo = reinterpret_cast<const double*>(cl + (o * NF + (colok ? fb : 0)) * nlp + sg * S) + comp;
-    const double* qgo = d.qg + mo + (sval ? sg * S : 0);  // global, read on the masked path only
-    const int qlim = sval ? L - m - sg * S : 0;           // last valid entry of qgo
     if (gq == 0)
-      syn_mf_steps<R, false, true>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
+      syn_mf_steps<R, true>(aE, aO, st, xr, qco, clo, colok, head, s23);
     else
-      syn_mf_steps<R, false, false>(aE, aO, st, mk, xr, qco, clo, qgo, qlim, colok, head, s23);
+      syn_mf_steps<R, false>(aE, aO, st, xr, qco, clo, colok, head, s23);
     if (o == 0) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
