@@ -25,8 +25,9 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
 // -------------------------------------------------------------------------------------------
 // Complex FFT of length N = 2^log2N held in LDS (a), Stockham autosort with ping-pong buffer b:
 // one radix-2 stage first when log2N is odd, then radix-4 stages.  Forward uses
-// exp(-2 pi i k n / N), inverse exp(+...), both unnormalised.  Returns the buffer holding the
-// result.  Every thread of the block must call it.
+// exp(-2 pi i k n / N), inverse exp(+...), both unnormalised; tw = exp(-2 pi i t / N), t < N (the
+// callers stage it in LDS).  Returns the buffer holding the result.  Every thread of the block
+// must call it.
 template <int NT>
 __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ tw, int N, int log2N,
                             bool inverse) {
@@ -132,11 +133,15 @@ __global__ void __launch_bounds__(NT) k_fft_ana(ShtDev d, Loader ld, double2* __
   const int N = d.nlon;
   double2* a = smem;
   double2* b = smem + N;
+  double2* tws = smem + 2 * N;  // twiddles, staged once per block
   const int j = blockIdx.x, f = blockIdx.y, js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
-  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(ld(f, rn + i), ld(f, rs + i));
+  for (int i = threadIdx.x; i < N; i += NT) {
+    a[i] = make_double2(ld(f, rn + i), ld(f, rs + i));
+    tws[i] = d.tw[i];
+  }
   __syncthreads();
-  double2* Z = fft_lds<NT>(a, b, d.tw, N, d.log2n, false);
+  double2* Z = fft_lds<NT>(a, b, tws, N, d.log2n, false);
   const double wn = d.wn[j], ws = d.ws[j];
   const long base = (long)f * d.nm * d.nh;
   for (int m = threadIdx.x; m <= d.lmax; m += NT) {
@@ -385,9 +390,13 @@ __global__ void __launch_bounds__(NT) k_fft_syn(ShtDev d, const double2* __restr
   const int N = d.nlon, L = d.lmax;
   double2* a = smem;
   double2* b = smem + N;
+  double2* tws = smem + 2 * N;  // twiddles, staged once per block
   const int j = blockIdx.x, f = blockIdx.y, js = d.nlat - 1 - j;
   const long base = (long)f * d.nm * d.nh;
-  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(0.0, 0.0);
+  for (int i = threadIdx.x; i < N; i += NT) {
+    a[i] = make_double2(0.0, 0.0);
+    tws[i] = d.tw[i];
+  }
   __syncthreads();
   for (int m = threadIdx.x; m <= L; m += NT) {
     double2 fn = FN[base + (long)m * d.nh + j], fs = FS[base + (long)m * d.nh + j];
@@ -399,7 +408,7 @@ __global__ void __launch_bounds__(NT) k_fft_syn(ShtDev d, const double2* __restr
     }
   }
   __syncthreads();
-  double2* z = fft_lds<NT>(a, b, d.tw, N, d.log2n, true);
+  double2* z = fft_lds<NT>(a, b, tws, N, d.log2n, true);
   const long rn = (long)j * N, rs = (long)js * N;
   for (int i = threadIdx.x; i < N; i += NT) {
     double2 v = z[i];
@@ -778,15 +787,18 @@ __global__ void __launch_bounds__(256) k_leg_syn_s(ShtDev d, SegDev g, const dou
   }
 }
 
-// Spectral coefficients assembled by a loader (Coef: prologue(double*) per block, then
-// double2 operator()(f, idx, l, m, pro)) into a plain packed array out[f ostride + idx].
+// Spectral coefficients assembled by a loader (Coef: prologue(double*), run by the blocks for
+// which prologue_needed(first index, end index) holds, then double2 operator()(f, idx, l, m, pro))
+// into a plain packed array out[f ostride + idx].
 template <int NF, class Coef>
 __global__ void __launch_bounds__(256) k_coef_fill(ShtDev d, Coef cf, double2* __restrict__ out, long ostride,
                                                    int ncoef, Pred pr) {
   if (!pr.active()) return;
   __shared__ double pro[32];
-  cf.prologue(pro);
-  __syncthreads();
+  if (cf.prologue_needed(blockIdx.x * 256, blockIdx.x * 256 + 256)) {  // block-uniform
+    cf.prologue(pro);
+    __syncthreads();
+  }
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= ncoef) return;
   const int l = d.l_of[idx], m = d.m_of[idx];

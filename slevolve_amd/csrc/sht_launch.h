@@ -40,6 +40,7 @@ struct LoadPlain {
 struct CoefPlain {
   const double2* c;
   long fstride;
+  __device__ bool prologue_needed(int, int) const { return false; }
   __device__ void prologue(double*) const {}
   __device__ double2 operator()(int f, int idx, int, int, const double*) const { return c[f * fstride + idx]; }
 };
@@ -121,7 +122,13 @@ namespace slx {
 template <int NT, class Loader>
 int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {
   dim3 grid(h->nh, nf);
-  size_t lds = 2 * (size_t)h->nlon * sizeof(double2);
+  const size_t lds = 3 * (size_t)h->nlon * sizeof(double2);
+  static size_t lds_allowed = 65536;
+  if (lds > lds_allowed) {
+    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_ana<NT, Loader>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_allowed = lds;
+  }
   k_fft_ana<NT, Loader><<<grid, NT, lds, s>>>(dev_of(h), ld, h->WS, h->WD, pr);
   SLX_LAUNCH_CHECK();
   return 0;
@@ -129,7 +136,13 @@ int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) 
 template <int NT, class Store>
 int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
   dim3 grid(h->nh, nf);
-  size_t lds = 2 * (size_t)h->nlon * sizeof(double2);
+  const size_t lds = 3 * (size_t)h->nlon * sizeof(double2);
+  static size_t lds_allowed = 65536;
+  if (lds > lds_allowed) {
+    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_syn<NT, Store>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_allowed = lds;
+  }
   k_fft_syn<NT, Store><<<grid, NT, lds, s>>>(dev_of(h), h->FN, h->FSo, st, pr);
   SLX_LAUNCH_CHECK();
   return 0;

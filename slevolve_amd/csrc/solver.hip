@@ -320,6 +320,12 @@ struct CoefB18 {
   DevState* st;
   Consts c;
   int eval;
+  // only the block that stores the rotation (block 0) and those holding the degree-2, m <= 2
+  // coefficients need it
+  __device__ bool prologue_needed(int lo, int hi) const {
+    auto in = [&](int i) { return i >= lo && i < hi; };
+    return lo == 0 || in(c.idx20) || in(c.idx21) || in(c.idx22);
+  }
   __device__ void prologue(double* pro) const {
     if (threadIdx.x != 0) return;
     if (eval) {
@@ -376,9 +382,13 @@ __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __re
   const int N = d.nlon, L = d.lmax;
   double2* a = smem;
   double2* b = smem + N;
+  double2* tws = smem + 2 * N;  // twiddles, staged once per block
   const int j = blockIdx.x, js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
-  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(0.0, 0.0);
+  for (int i = threadIdx.x; i < N; i += NT) {
+    a[i] = make_double2(0.0, 0.0);
+    tws[i] = d.tw[i];
+  }
   __syncthreads();
   for (int m = threadIdx.x; m <= L; m += NT) {
     const double2 fn = FN[(long)m * d.nh + j], fs = FS[(long)m * d.nh + j];
@@ -390,7 +400,7 @@ __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __re
     }
   }
   __syncthreads();
-  double2* z = fft_lds<NT>(a, b, d.tw, N, d.log2n, true);
+  double2* z = fft_lds<NT>(a, b, tws, N, d.log2n, true);
   double2* w = (z == a) ? b : a;
   for (int i = threadIdx.x; i < N; i += NT) {
     const double2 v = z[i];
@@ -406,7 +416,7 @@ __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __re
       for (int i = threadIdx.x; i < N; i += NT) z[i] = make_double2(cstar[rn + i], cstar[rs + i]);
       __syncthreads();
     }
-    double2* Z = fft_lds<NT>(z, w, d.tw, N, d.log2n, false);
+    double2* Z = fft_lds<NT>(z, w, tws, N, d.log2n, false);
     for (int m = threadIdx.x; m <= L; m += NT) {
       const double2 zm = Z[m], zc = Z[(N - m) & (N - 1)];
       const double2 fn = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y - zc.y));
@@ -923,7 +933,13 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     if (rc) return rc;
     {
       const int nf = first ? 1 : 2;
-      const size_t lds = 2 * (size_t)c.nlon * sizeof(double2);
+      const size_t lds = 3 * (size_t)c.nlon * sizeof(double2);
+      static size_t lds_allowed = 65536;
+      if (lds > lds_allowed) {
+        SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_synana<256>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        lds_allowed = lds;
+      }
       k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl, s->cstar, s->tO,
                                                      s->h->WS, s->h->WD, nf, Pred{done, 0});
       SLX_LAUNCH_CHECK();
