@@ -33,7 +33,7 @@ int slx_abi_version(void);
  * sht_ducc.py:70-132 (DuccSHT), reached through make_sht (sht_ducc.py:206-217).  Builds the
  * Gauss-Legendre nodes/weights (gaulw, sht.py:65-101), the sectoral P~_mm and recurrence
  * coefficients (legend, sht.py:105-144) and FFT twiddles, and work buffers for up to
- * max_fields (1..4) fields per call.  nlat even, nlon a power of two, lmax <= nlon/2. */
+ * max_fields (1..3) fields per call.  nlat even, nlon a power of two, lmax <= nlon/2. */
 int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_sht** out);
 int slx_sht_destroy(slx_sht* h);
 /* dims (host, 5 ints or NULL) = {nlat, nlon, lmax, ncoef, max_fields}; gaulats/weights (host,
@@ -122,6 +122,14 @@ int slx_solver_set_dt(slx_solver* s, double dt_years);
  * ice / ocean analyses and the first guess (eustatic, or extrapolated with the step lengths
  * dt0 = this step, dt1, dt2 = the two previous ones when the solver extrapolates). */
 int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, double dt1, double dt2, void* stream);
+/* slx_step_begin with the "warm" start across outer passes of OptSeaLevelSolver(reuse="warm")
+ * (solver_opt.py:400-402,517-528): before the ice/ocean block the ocean function becomes
+ * warm_ocean (device grid, 0/1: the converged ocean function this step had in the previous
+ * pass), and the first guess of the ocean-load increment becomes warm_dS (device, packed
+ * [ncoef] complex: the converged increment of that pass, fields.load_inc after its finish), with
+ * deltaS_cur = p_ocean + warm_dS.  Both pointers NULL = slx_step_begin. */
+int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double dt0, double dt1, double dt2,
+                        const void* warm_dS, const double* warm_ocean, void* stream);
 /* Enqueue n inner iterations; those after convergence return at once on the device. */
 int slx_step_iterate(slx_solver* s, int n, void* stream);
 /* Radial displacement, commit, topography / ocean function and degree-0 diagnostics.  These

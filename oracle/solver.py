@@ -59,10 +59,20 @@ def convergence_ratio(sa, so):
 
 
 class SeaLevelSolver:
+    """The step of solver.py:521-1144 with the inner starts of solver_opt.py ("off", "extrap") and
+    the "warm" cross-pass reuse of solver_opt.py:272-286: with reuse="warm" and a caller-owned
+    reuse_store dict (keyed by step number), step n starts from the ocean function and the
+    converged load increment that step n stored in the previous outer pass
+    (solver_opt.py:400-402,517-528) and stores its own at the end (solver_opt.py:932-941)."""
+
     def __init__(self, sht, love, topo0, ice0, *, norder, dt=1.0, tpw=True, checkmarine=True,
-                 calcRG=True, epsilon1=1.0e-5, inner="off", planet=None):
+                 calcRG=True, epsilon1=1.0e-5, inner="off", planet=None, reuse="off", reuse_store=None):
         if inner not in ("off", "extrap"):
             raise ValueError("inner must be 'off' or 'extrap'")
+        if reuse not in ("off", "warm"):
+            raise ValueError("reuse must be 'off' or 'warm'")
+        self.reuse = reuse
+        self.reuse_store = reuse_store if reuse_store is not None else {}
         self.sht, self.love = sht, love
         self.L = norder
         self.nl = norder + 1
@@ -126,6 +136,9 @@ class SeaLevelSolver:
         visc, rad, sum_il, sum_m, sum_tsl, sum_trr = st.advance_and_read()
         dS_prev = st.p_ocean.copy()
         old_il, old_m, lam_prev2 = st.p_il.copy(), st.p_m.copy(), st.p_lambda.copy()
+        rs = self.reuse_store.get(self.iter) if self.reuse != "off" else None
+        if rs is not None:
+            self.cxy = rs["cxy"].astype(np.float64)  # converged mask of this step, previous pass
 
         # ---- ice: grounded projection on the previous converged topography
         if self.checkmarine:
@@ -154,6 +167,9 @@ class SeaLevelSolver:
         if self.inner == "extrap" and len(self.hist) >= 2:
             (h1, dt1), (h2, dt2) = self.hist[0], self.hist[1]
             load_inc = extrap_guess(h1, h2, self.dt, dt1, dt2)
+            dS_cur = dS_prev + load_inc
+        if rs is not None:  # warm start: the stored converged increment replaces the guess
+            load_inc = rs["dS"].copy()
             dS_cur = dS_prev + load_inc
 
         kTE2, hTE2, kE2 = self.love.kTE[2], self.love.hTE[2], self.love.ke[2]
@@ -261,6 +277,8 @@ class SeaLevelSolver:
         topo = self.tinit - deltasl
         self.cxy = (topo < 0.0).astype(np.float64)
         self.topo_prev = topo.copy()
+        if self.reuse != "off":
+            self.reuse_store[self.iter] = {"dS": load_inc.copy(), "cxy": self.cxy != 0.0}
         area = 4 * PI * p.radius**2
         slc = deltasl * self.cxy
         gmslc_ocn = (sht.analysis(slc)[0, 0] / sht.analysis(self.cxy)[0, 0]).real

@@ -62,6 +62,7 @@ SIGNATURES = {
     "slx_solver_init": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "slx_solver_set_dt": (c_int, [c_void_p, c_double]),
     "slx_step_begin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_double, c_void_p]),
+    "slx_step_begin_warm": (c_int, [c_void_p, c_void_p, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
     "slx_step_iterate": (c_int, [c_void_p, c_int, c_void_p]),
     "slx_step_finish": (c_int, [c_void_p, c_void_p]),
     "slx_solver_status": (c_int, [c_void_p, C.POINTER(SlxStatus), c_void_p]),

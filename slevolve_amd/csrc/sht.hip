@@ -233,8 +233,9 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   SLX_HIP(hipMalloc(&h->CS, (size_t)max_fields * h->ncoef * sizeof(double2)));
   SLX_HIP(hipMemset(h->WS, 0, wsz));
   SLX_HIP(hipMemset(h->WD, 0, wsz));
-  // autotune of the 2-field analysis contraction, timed on the device with the handle's own
-  // shapes: rings per lane (2 or 4) on the matrix-core path, waves per segment (1 or 2) otherwise
+  // autotune of the analysis contraction (2 fields, or 1 for a 1-field handle), timed on the
+  // device with the handle's own shapes: rings per lane (2 or 4) on the matrix-core path, waves
+  // per segment (1 or 2) otherwise
   {
     double2* tmp = nullptr;
     SLX_HIP(hipMalloc(&tmp, 2 * (size_t)h->ncoef * sizeof(double2)));
@@ -252,12 +253,16 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
         if ((nh + 64 * v - 1) / (64 * v) > 4) continue;
         h->ana_nw = v;
       }
-      OutPtrs<2> o2{{tmp, tmp + h->ncoef}};
-      int rc = launch_leg_ana<2>(h, h->WS, h->WD, o2, Pred{}, nullptr);
+      // as many fields as the work buffers hold (a 1-field handle has no second field to read)
+      auto tune = [&]() {
+        return h->max_fields >= 2 ? launch_leg_ana<2>(h, h->WS, h->WD, OutPtrs<2>{{tmp, tmp + h->ncoef}}, Pred{}, nullptr)
+                                  : launch_leg_ana<1>(h, h->WS, h->WD, OutPtrs<1>{{tmp}}, Pred{}, nullptr);
+      };
+      int rc = tune();
       if (rc) return rc;
       SLX_HIP(hipEventRecord(e0, nullptr));
       for (int rep = 0; rep < 3; ++rep) {
-        rc = launch_leg_ana<2>(h, h->WS, h->WD, o2, Pred{}, nullptr);
+        rc = tune();
         if (rc) return rc;
       }
       SLX_HIP(hipEventRecord(e1, nullptr));

@@ -270,6 +270,7 @@ __global__ void k_ice_grid(Pts p, Consts c) {
 struct GuessArgs {
   int extrap_ready, uniform;
   double c1, c2;
+  const double2* warm;  // warm start: replaces the guess (deltaS_cur = p_ocean + warm)
 };
 __global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
   __shared__ double sh[256];
@@ -300,6 +301,11 @@ __global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
       const double2 q = make_double2(p.cstar_lm[i].x * scl, p.cstar_lm[i].y * scl);
       ld = make_double2(q.x * e.x - q.y * e.y, q.x * e.y + q.y * e.x);
       p.dS_cur[i] = make_double2(0.0, 0.0);
+    }
+    if (g.warm) {
+      ld = g.warm[i];
+      const double2 sp = p.p_ocean[i];
+      p.dS_cur[i] = make_double2(sp.x + ld.x, sp.y + ld.y);
     }
     p.load_inc[i] = ld;
     a += cabs_d(ld);
@@ -882,7 +888,14 @@ extern "C" int slx_solver_init(slx_solver* s, const double* topo0, const double*
 }
 
 extern "C" int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, double dt1, double dt2, void* stream) {
+  return slx_step_begin_warm(s, ice_cur, dt0, dt1, dt2, nullptr, nullptr, stream);
+}
+
+extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double dt0, double dt1, double dt2,
+                                   const void* warm_dS, const double* warm_ocean, void* stream) {
   SLX_CHECK(s && ice_cur, "slx_step_begin: null argument");
+  SLX_CHECK((warm_dS == nullptr) == (warm_ocean == nullptr),
+            "slx_step_begin_warm: give both warm_dS and warm_ocean, or neither");
   SLX_CHECK(s->dt > 0.0, "slx_solver_set_dt must be called before the first step");
   hipStream_t st = (hipStream_t)stream;
   SLX_CHECK(s->closed, "slx_step_begin: the previous step has not finished (check slx_solver_status)");
@@ -898,6 +911,8 @@ extern "C" int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, 
   int rc = launch_modal(s->h, c.kmax, s->nmod, s->decay, s->omd, s->a_sl, s->a_rr, s->R, s->p_load, s->visc,
                         s->rad, st);
   if (rc) return rc;
+  if (warm_ocean)
+    SLX_HIP(hipMemcpyAsync(s->cxy, warm_ocean, (size_t)c.npts * sizeof(double), hipMemcpyDeviceToDevice, st));
   k_ice_grid<<<grid_blocks(c.npts), 256, 0, st>>>(p, c);
   SLX_LAUNCH_CHECK();
   rc = launch_fft_ana<256>(s->h, LoadPtrs<2>{{s->icestar, s->cstar}}, 2, Pred{}, st);
@@ -910,6 +925,7 @@ extern "C" int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, 
   const double w = (dt0 + dt1) / (dt1 + dt2);
   g.c1 = dt0 / dt1 * (1.0 + w);
   g.c2 = dt0 / dt2 * w;
+  g.warm = static_cast<const double2*>(warm_dS);
   k_guess<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, g);
   SLX_LAUNCH_CHECK();
   return 0;

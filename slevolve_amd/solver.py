@@ -8,9 +8,10 @@ reads one small status block back, and returns the reference's result dictionary
 it are device tensors that the solver overwrites at the next step (the reference also returns
 its reused buffers).
 
-Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap".
+Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap", and
+the "warm" cross-pass reuse of OptSeaLevelSolver (per-step converged states kept on the device).
 The second-order ramp, analytic rates, horizontal read-out, axial-rotation variants, Anderson
-mixing and the outer-iteration reuse store of the reference are not provided and raise.
+mixing and the other reuse modes of the reference are not provided and raise.
 """
 from __future__ import annotations
 
@@ -192,6 +193,8 @@ class SeaLevelSolver:
         self._dS = torch.empty((self.nl, self.nl), dtype=torch.complex128, device=dev)
         self._spec_tmp = torch.empty(sht.ncoef, dtype=torch.complex128, device=dev)
         self._guess_iters = 4
+        self._reuse, self._reuse_store = "off", {}
+        self.reuse_stats = {"warm_steps": 0}
         self.prof = {}
         self.state = _StateView(self)
         self.il = np.zeros((3, 3))
@@ -235,7 +238,13 @@ class SeaLevelSolver:
             ice.copy_(D.as_device(icexy_cur, torch.float64, self.dev, (self.nlat, self.nlon)))
         h = self._s_handle
         sp = D.stream_ptr()
-        _lib.call("slx_step_begin", h, ice.data_ptr(), self.dt, self._dt_hist[0], self._dt_hist[1], sp)
+        rs = self._reuse_store.get(self.iter) if self._reuse != "off" else None
+        if rs is not None:  # warm start from the previous outer pass (solver_opt.py:400-402,517-528)
+            _lib.call("slx_step_begin_warm", h, ice.data_ptr(), self.dt, self._dt_hist[0], self._dt_hist[1],
+                      rs["dS"].data_ptr(), rs["cxy"].data_ptr(), sp)
+            self.reuse_stats["warm_steps"] += 1
+        else:
+            _lib.call("slx_step_begin", h, ice.data_ptr(), self.dt, self._dt_hist[0], self._dt_hist[1], sp)
         n = self._guess_iters
         total = 0
         while True:
@@ -257,6 +266,14 @@ class SeaLevelSolver:
         self.lam = lam[..., 0] + 1j * lam[..., 1]
         _copy_dev(self._spec_tmp, self._fields.dS_converged, self.sht.ncoef * 16)
         self.sht.to_dense(self._spec_tmp, self._dS)
+        if self._reuse != "off":  # store this step's converged state (solver_opt.py:932-941)
+            ent = self._reuse_store.get(self.iter)
+            if ent is None:
+                ent = self._reuse_store[self.iter] = {
+                    "dS": torch.empty(self.sht.ncoef, dtype=torch.complex128, device=self.dev),
+                    "cxy": torch.empty_like(self.cxy)}
+            _copy_dev(ent["dS"], self._fields.load_inc, self.sht.ncoef * 16)
+            ent["cxy"].copy_(self.cxy)
         wall = time.perf_counter() - t_all
         if profile:
             self.prof = {"total": wall}
@@ -293,10 +310,15 @@ class OptSeaLevelSolver(SeaLevelSolver):
                  anderson_keep=False, reuse="off", reuse_store=None, **kw):
         if inner == "anderson":
             raise NotImplementedError("Anderson mixing is not provided")
-        if reuse != "off":
-            raise NotImplementedError("the cross-pass reuse store is not provided")
+        if reuse not in ("off", "warm"):
+            raise NotImplementedError(f"reuse={reuse!r} is not provided (off | warm)")
         self.opt = set(opt)
         super().__init__(*args, inner=inner, **kw)
+        # cross-pass reuse (solver_opt.py:272-286): the caller-owned store keeps, per step, the
+        # converged load increment (packed, device) and ocean function (device grid)
+        self._reuse = reuse
+        self._reuse_store = reuse_store if reuse_store is not None else {}
+        self.reuse_store = self._reuse_store
         self.inner_stats = {}
 
 
