@@ -9,8 +9,10 @@ radial displacement, commit).  All ice grids are uploaded before the timed regio
 then K timed steps bracketed by device synchronisation.  Prints one JSON line.
 
 Also reported: the roofline of the dominant kernel (the per-order Legendre analysis contraction),
-timed live with HIP events on the stream it is launched on, and the CPU oracle's steps/s on a
-bounded sample of the same workload on this host.
+timed live with HIP events on the stream it is launched on, with the HBM traffic per launch from
+the committed rocprofv3 counter summary; the CPU oracle's steps/s on a bounded sample of the same
+workload on this host; and, in a default run, the 1024 x 2048, lmax 1023 grid (240 x 0.5 kyr,
+three ice caps) in FP64 under "also".
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lmax512|lmax1023|lmax63]
 """
@@ -84,6 +86,49 @@ def cpu_baseline(w, ice, budget):
                       f"{os.environ.get('OMP_NUM_THREADS', 'default')}), {el:.1f} s"}
 
 
+def run_workload(w, warmup, steps, dev):
+    """W warm-up steps, then K timed steps of workload w with every ice grid resident in HBM,
+    bracketed by device synchronisation.  Returns (steps/s, seconds, K, numiter list, transform)."""
+    import numpy as np
+    import torch
+
+    from slevolve_amd import SHT, LoveNumbers, OptSeaLevelSolver
+
+    steps = steps if steps is not None else w.nsteps - warmup
+    nsteps = warmup + steps
+    if nsteps > w.nsteps:
+        raise SystemExit(f"warmup + steps = {nsteps} exceeds the workload's {w.nsteps} steps")
+    t0 = time.perf_counter()
+    ice_dev = torch.empty((nsteps + 1, w.nlat, w.nlon), dtype=torch.float64, device=dev)
+    for k in range(nsteps + 1):
+        ice_dev[k].copy_(torch.from_numpy(w.ice(k)))
+    print(f"[bench] {w.name}: inputs on the device ({ice_dev.numel() * 8 / 1e9:.2f} GB, "
+          f"{time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    sht = SHT(w.nlat, w.nlon, w.lmax, device=dev)
+    love = LoveNumbers(w.love_text, w.lmax, text=True)
+    slv = OptSeaLevelSolver(sht, love, w.topo, w.ice(0), norder=w.lmax, dt=w.dt, tpw=True, checkmarine=True,
+                            calcRG=True, epsilon1=1e-5, inner="extrap")
+    for k in range(1, warmup + 1):
+        slv.step(ice_dev[k])
+    torch.cuda.synchronize()
+    numiter = []
+    t0 = time.perf_counter()
+    for k in range(warmup + 1, nsteps + 1):
+        numiter.append(slv.step(ice_dev[k])["numiter"])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(f"[bench] {w.name}: {steps} steps in {el:.3f} s, numiter mean {np.mean(numiter):.2f}", file=sys.stderr,
+          flush=True)
+    del slv, ice_dev
+    torch.cuda.synchronize()
+    return steps / el, el, steps, numiter, sht
+
+
+def describe(w):
+    return (f"{w.name}: {w.nlat}x{w.nlon} GL grid, lmax {w.lmax}, {w.nmodes} modes/degree, "
+            f"{w.nsteps} x {w.dt:g} yr, FP64, inner extrap, tpw+checkmarine on, eps1 1e-5")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -93,8 +138,7 @@ def main():
     import numpy as np
     import torch
 
-    from slevolve_amd import SHT, LoveNumbers, OptSeaLevelSolver
-    from slevolve_amd import _lib
+    from slevolve_amd import SHT
     from slevolve_amd import synthetic as syn
 
     w = syn.workload(a.workload)
@@ -113,33 +157,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"kernels_only": True, "copy_bytes": src.numel() * 8}))
         return
-    steps = a.steps if a.steps is not None else w.nsteps - a.warmup
-    nsteps = a.warmup + steps
-    if nsteps > w.nsteps:
-        raise SystemExit(f"warmup + steps = {nsteps} exceeds the workload's {w.nsteps} steps")
     dev = torch.device("cuda", 0)
-    t0 = time.perf_counter()
-    ice_dev = torch.empty((nsteps + 1, w.nlat, w.nlon), dtype=torch.float64, device=dev)
-    for k in range(nsteps + 1):
-        ice_dev[k].copy_(torch.from_numpy(w.ice(k)))
-    print(f"[bench] {w.name}: inputs on the device ({ice_dev.numel() * 8 / 1e9:.2f} GB, "
-          f"{time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    sht = SHT(w.nlat, w.nlon, w.lmax, device=dev)
-    love = LoveNumbers(w.love_text, w.lmax, text=True)
-    slv = OptSeaLevelSolver(sht, love, w.topo, w.ice(0), norder=w.lmax, dt=w.dt, tpw=True, checkmarine=True,
-                            calcRG=True, epsilon1=1e-5, inner="extrap")
-    for k in range(1, a.warmup + 1):
-        slv.step(ice_dev[k])
-    torch.cuda.synchronize()
-    numiter = []
-    t0 = time.perf_counter()
-    for k in range(a.warmup + 1, nsteps + 1):
-        numiter.append(slv.step(ice_dev[k])["numiter"])
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    sps = steps / el
-    print(f"[bench] {steps} steps in {el:.3f} s, numiter mean {np.mean(numiter):.2f}", file=sys.stderr, flush=True)
-
+    sps, el, steps, numiter, sht = run_workload(w, a.warmup, a.steps, dev)
     # ---- dominant kernel: the per-order analysis contraction of a 2-field sweep, timed with HIP
     # events on the current stream (the stream the library launches on)
     grids = torch.from_numpy(np.stack([w.topo, w.ice(1)])).to(dev)
@@ -160,20 +179,27 @@ def main():
     traffic, traffic_src = measured_traffic(w.name, "leg_ana_2field")
     roof = {"bound": "mfma", "achieved": flops / t_kernel / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": flops / t_kernel / PEAK_FP64, "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": "k_leg_ana<2,R,2> (per-order FP64 Legendre analysis contraction, 2 fields)",
+            "kernel": "k_leg_ana_mf<2,R> (per-order FP64 Legendre analysis contraction on the matrix cores, 2 fields)",
             "kernel_us": t_kernel * 1e6, "flops_per_launch": flops}
     out = {
         "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
         "ms_per_step": 1e3 * el / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{w.name}: {w.nlat}x{w.nlon} GL grid, lmax {w.lmax}, {w.nmodes} modes/degree, "
-                               f"{w.nsteps} x {w.dt:g} yr, FP64, inner extrap, tpw+checkmarine on, eps1 1e-5",
-                   "numiter_mean": float(np.mean(numiter)), "numiter_total": int(np.sum(numiter)),
-                   "ice": w.note},
+        "config": {"workload": describe(w), "numiter_mean": float(np.mean(numiter)),
+                   "numiter_total": int(np.sum(numiter)), "ice": w.note},
         "roofline": roof,
+        "mixed": "not provided: on this GPU FP32 vector math runs at the FP64 rate and the contraction is "
+                 "bound by its FP64 recurrence, so reduced-precision sweeps would not be cheaper",
     }
-    del slv
+    del sht
     torch.cuda.synchronize()
+    if a.steps is None and a.workload == "lmax512":
+        # the metric's second grid (lmax 1023, 1024 x 2048, 240 x 0.5 kyr), FP64, every step but 5
+        w3 = syn.workload("lmax1023")
+        sps3, el3, steps3, nit3, sht3 = run_workload(w3, 5, None, dev)
+        del sht3
+        out["also"] = [{"workload": describe(w3), "value": sps3, "unit": "steps/s", "steps": steps3, "warmup": 5,
+                        "ms_per_step": 1e3 * el3 / steps3, "numiter_mean": float(np.mean(nit3)), "ice": w3.note}]
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, w.ice, a.cpu_seconds)
     print(json.dumps(out))
