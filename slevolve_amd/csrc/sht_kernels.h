@@ -25,9 +25,22 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
 // -------------------------------------------------------------------------------------------
 // Complex FFT of length N = 2^log2N held in LDS (a), Stockham autosort with ping-pong buffer b:
 // one radix-2 stage first when log2N is odd, then radix-4 stages.  Forward uses
-// exp(-2 pi i k n / N), inverse exp(+...), both unnormalised; tw = exp(-2 pi i t / N), t < N (the
-// callers stage it in LDS).  Returns the buffer holding the result.  Every thread of the block
-// must call it.
+// exp(-2 pi i k n / N), inverse exp(+...), both unnormalised.  tw4 = exp(-2 pi i t / N) for
+// t < N/4 (the callers stage it in LDS); the other quadrants follow by exact rotations:
+// exp(-2 pi i (t + q N/4) / N) = tw4[t] (-i)^q.  Returns the buffer holding the result.  Every
+// thread of the block must call it.
+__device__ __forceinline__ double2 tw_quarter(const double2* tw4, int u, int log2N) {
+  const int q = u >> (log2N - 2), r = u & ((1 << (log2N - 2)) - 1);
+  const double2 w = tw4[r];
+  return q == 0 ? w : q == 1 ? make_double2(w.y, -w.x) : q == 2 ? make_double2(-w.x, -w.y) : make_double2(-w.y, w.x);
+}
+
+// stage the quarter-period twiddle table (N/4 entries) in LDS
+template <int NT>
+__device__ __forceinline__ void stage_tw4(double2* tw4, const double2* __restrict__ tw, int N) {
+  for (int i = threadIdx.x; i < N / 4; i += NT) tw4[i] = tw[i];
+}
+
 template <int NT>
 __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ tw, int N, int log2N,
                             bool inverse) {
@@ -50,15 +63,16 @@ __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ 
       const int k = j & (Ns - 1);
       double2 v0 = a[j], v1 = a[j + q], v2 = a[j + 2 * q], v3 = a[j + 3 * q];
       if (Ns > 1) {
-        const int st = k * tstride;
+        const int st = k * tstride;  // < N/4
+        const double2 w1 = tw[st], w2 = tw_quarter(tw, 2 * st, log2N), w3 = tw_quarter(tw, 3 * st, log2N);
         if (!inverse) {
-          v1 = cmul(v1, tw[st]);
-          v2 = cmul(v2, tw[2 * st]);
-          v3 = cmul(v3, tw[3 * st]);
+          v1 = cmul(v1, w1);
+          v2 = cmul(v2, w2);
+          v3 = cmul(v3, w3);
         } else {
-          v1 = cmulc(v1, tw[st]);
-          v2 = cmulc(v2, tw[2 * st]);
-          v3 = cmulc(v3, tw[3 * st]);
+          v1 = cmulc(v1, w1);
+          v2 = cmulc(v2, w2);
+          v3 = cmulc(v3, w3);
         }
       }
       double2 s02 = cadd(v0, v2), d02 = csub(v0, v2), s13 = cadd(v1, v3), d13 = csub(v1, v3);
@@ -133,13 +147,11 @@ __global__ void __launch_bounds__(NT) k_fft_ana(ShtDev d, Loader ld, double2* __
   const int N = d.nlon;
   double2* a = smem;
   double2* b = smem + N;
-  double2* tws = smem + 2 * N;  // twiddles, staged once per block
+  double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
   const int j = blockIdx.x, f = blockIdx.y, js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
-  for (int i = threadIdx.x; i < N; i += NT) {
-    a[i] = make_double2(ld(f, rn + i), ld(f, rs + i));
-    tws[i] = d.tw[i];
-  }
+  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(ld(f, rn + i), ld(f, rs + i));
+  stage_tw4<NT>(tws, d.tw, N);
   __syncthreads();
   double2* Z = fft_lds<NT>(a, b, tws, N, d.log2n, false);
   const double wn = d.wn[j], ws = d.ws[j];
@@ -390,13 +402,11 @@ __global__ void __launch_bounds__(NT) k_fft_syn(ShtDev d, const double2* __restr
   const int N = d.nlon, L = d.lmax;
   double2* a = smem;
   double2* b = smem + N;
-  double2* tws = smem + 2 * N;  // twiddles, staged once per block
+  double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
   const int j = blockIdx.x, f = blockIdx.y, js = d.nlat - 1 - j;
   const long base = (long)f * d.nm * d.nh;
-  for (int i = threadIdx.x; i < N; i += NT) {
-    a[i] = make_double2(0.0, 0.0);
-    tws[i] = d.tw[i];
-  }
+  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(0.0, 0.0);
+  stage_tw4<NT>(tws, d.tw, N);
   __syncthreads();
   for (int m = threadIdx.x; m <= L; m += NT) {
     double2 fn = FN[base + (long)m * d.nh + j], fs = FS[base + (long)m * d.nh + j];

@@ -122,7 +122,7 @@ namespace slx {
 template <int NT, class Loader>
 int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {
   dim3 grid(h->nh, nf);
-  const size_t lds = 3 * (size_t)h->nlon * sizeof(double2);
+  const size_t lds = (2 * (size_t)h->nlon + h->nlon / 4) * sizeof(double2);
   static size_t lds_allowed = 65536;
   if (lds > lds_allowed) {
     SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_ana<NT, Loader>),
@@ -136,7 +136,7 @@ int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) 
 template <int NT, class Store>
 int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
   dim3 grid(h->nh, nf);
-  const size_t lds = 3 * (size_t)h->nlon * sizeof(double2);
+  const size_t lds = (2 * (size_t)h->nlon + h->nlon / 4) * sizeof(double2);
   static size_t lds_allowed = 65536;
   if (lds > lds_allowed) {
     SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_syn<NT, Store>),

@@ -388,13 +388,11 @@ __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __re
   const int N = d.nlon, L = d.lmax;
   double2* a = smem;
   double2* b = smem + N;
-  double2* tws = smem + 2 * N;  // twiddles, staged once per block
+  double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
   const int j = blockIdx.x, js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
-  for (int i = threadIdx.x; i < N; i += NT) {
-    a[i] = make_double2(0.0, 0.0);
-    tws[i] = d.tw[i];
-  }
+  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(0.0, 0.0);
+  stage_tw4<NT>(tws, d.tw, N);
   __syncthreads();
   for (int m = threadIdx.x; m <= L; m += NT) {
     const double2 fn = FN[(long)m * d.nh + j], fs = FS[(long)m * d.nh + j];
@@ -949,7 +947,7 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     if (rc) return rc;
     {
       const int nf = first ? 1 : 2;
-      const size_t lds = 3 * (size_t)c.nlon * sizeof(double2);
+      const size_t lds = (2 * (size_t)c.nlon + c.nlon / 4) * sizeof(double2);
       static size_t lds_allowed = 65536;
       if (lds > lds_allowed) {
         SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_synana<256>),
