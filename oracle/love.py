@@ -4,7 +4,8 @@ Restates ``LoveNumbers`` (``python/slevolve/solver.py:114-225``): Fortran list-d
 (every READ starts a new record, surplus items of the last record are dropped), per degree
 l = 1..norder the groups ``l K`` / ``s_1..s_K`` / elastic line / fluid line / h, l, k residues
 and, for l >= 2, the tidal elastic / fluid lines and residues.  l- and k-type numbers are stored
-times the degree in the file and divided by l here.
+times the degree in the file and divided by l here.  The tangential (l-type) numbers feed only the
+horizontal read-out.
 """
 from __future__ import annotations
 
@@ -69,14 +70,20 @@ class LoveNumbers:
             res[key][:, 1:] /= deg
         self.he = np.zeros(nl)
         self.ke = np.zeros(nl)
+        self.le = np.zeros(nl)
         self.hTE = np.zeros(nl)
         self.kTE = np.zeros(nl)
+        self.lTE = np.zeros(nl)
         self.he[1:] = el[0, 1:]
         self.ke[1:] = el[2, 1:] / deg
+        self.le[1:] = el[1, 1:] / deg  # tangential numbers (solver.py:202-208)
         self.hTE[1:] = tel[0, 1:]
         self.kTE[1:] = tel[2, 1:] / deg
+        self.lTE[1:] = tel[1, 1:] / deg
         self.r = res["h"]
         self.rprime = res["k"]
+        self.rl = res["l"]
         self.rT = res["th"]
         self.rprimeT = res["tk"]
+        self.rlT = res["tl"]
         self.max_modes = int(self.nmod.max())

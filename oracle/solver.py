@@ -3,11 +3,14 @@
 Restates the reference's exact-memory solver:
 
 * ``SeaLevelSolver.__init__`` step-0 set-up (``python/slevolve/solver.py:521-605``);
-* ``SeaLevelSolver.step`` with ``ramp`` off, ``axial_rotation`` off, no rates / horizontal
-  read-out (solver.py:814-1144): modal advance, grounded-ice projection on the previous converged
-  topography, eustatic first guess, the inner fixed point (rotation, eq. B18, synthesis, rO,
-  analysis, conservation constant, ocean-load update, convergence ratio, ocean-function and
-  grounded-ice re-guess), radial displacement, commit and the degree-0 diagnostics;
+* ``SeaLevelSolver.step`` with ``axial_rotation`` off (solver.py:814-1144): modal advance,
+  grounded-ice projection on the previous converged topography, eustatic first guess, the inner
+  fixed point (rotation, eq. B18, synthesis, rO, analysis, conservation constant, ocean-load
+  update, convergence ratio, ocean-function and grounded-ice re-guess), radial displacement,
+  commit and the degree-0 diagnostics;
+* the optional second-order load placement ``ramp`` (solver.py:904-988,1065-1068), the horizontal
+  read-out ``horizontal`` (solver.py:687-720) and the analytic rates ``step(rates=True)``
+  (solver.py:722-806);
 * ``inner="extrap"``: the extrapolated inner start of ``OptSeaLevelSolver``
   (``solver_opt.py:148-165,504-527,817-822``).
 
@@ -66,7 +69,8 @@ class SeaLevelSolver:
     (solver_opt.py:400-402,517-528) and stores its own at the end (solver_opt.py:932-941)."""
 
     def __init__(self, sht, love, topo0, ice0, *, norder, dt=1.0, tpw=True, checkmarine=True,
-                 calcRG=True, epsilon1=1.0e-5, inner="off", planet=None, reuse="off", reuse_store=None):
+                 calcRG=True, epsilon1=1.0e-5, inner="off", planet=None, reuse="off", reuse_store=None,
+                 ramp=False):
         if inner not in ("off", "extrap"):
             raise ValueError("inner must be 'off' or 'extrap'")
         if reuse not in ("off", "warm"):
@@ -94,6 +98,9 @@ class SeaLevelSolver:
         self.oceanBeta_area0 = sht.analysis(self.cstar0)[0, 0].real * area
         self.ice_volume0 = sht.analysis(g0)[0, 0].real * area
         self.state = ExactMemory(love, norder, self.dt)
+        self.ramp = bool(ramp)
+        self.state.ramp = self.ramp
+        self.horizontal = False  # set True for the horizontal read-out (result key "horiz")
         nl = self.nl
         ell = np.arange(nl, dtype=np.float64)
         ttl = np.zeros(nl)
@@ -102,9 +109,12 @@ class SeaLevelSolver:
         ekhl[1:] = 1 + love.ke[1:nl] - love.he[1:nl]
         he = np.zeros(nl)
         he[1:] = love.he[1:nl]
+        le = np.zeros(nl)
+        le[1:] = love.le[1:nl]
         self.ttl = ttl[:, None]
         self.ttlekhl = (ttl * ekhl)[:, None]
         self.ttlhe = (ttl * he)[:, None]
+        self.ttlle = (ttl * le)[:, None]
         self.il = np.zeros((3, 3))
         self.mm = np.zeros(3)
         self.lam = np.zeros((3, 3), dtype=np.complex128)
@@ -128,7 +138,97 @@ class SeaLevelSolver:
             self.state.set_dt(self.dt)
 
     # ------------------------------------------------------------------------------------
-    def step(self, ice_cur, dt=None):
+    def horizontal_lm(self, dI, dS_prev, load_inc, dice, rot_tot):
+        """Psi_lm = T_l [l^E_l A_lm + sum_k (l_k/s_k) R_k,lm] (+ the ramp's same-step term and the
+        degree-2 rotation with the tidal l numbers), U convention (solver.py:687-720)."""
+        p, st = self.p, self.state
+        if st.a_hl is None:
+            st.horizontal_weights(self.love)
+        visc_h, thl = st.read_horizontal()
+        A = p.rhoi * dI + p.rhow * dS_prev
+        A += p.rhow * load_inc
+        hz = self.ttlle * A + self.ttl * visc_h
+        if self.ramp:
+            c_hl = (st.a_hl * st.omphi).sum(0)
+            hz += (self.ttl[:, 0] * c_hl)[:, None] * (p.rhoi * dice + p.rhow * load_inc)
+        hz[0, :] = 0.0
+        if self.tpw and rot_tot is not None and self.L >= 2:
+            if self.ramp:
+                c_thl = float(st.omphi[:st.nmod2, 2] @ st.a_thl[:st.nmod2])
+                thl = thl + c_thl * (rot_tot - st.p_lambda)
+            hz[2, 0:3] += (self.love.lTE[2] * rot_tot + thl) / p.gacc
+        return hz
+
+    def _rotation_rate(self, old_il, old_m, lam_prev2):
+        """il-dot, m-dot (Liouville equations differentiated, the ramp's current-increment term as
+        s_k (m - r_m,k)), Lambda-dot by the chain rule, and the rotational sea-level / radial rates
+        at degree 2 (solver.py:722-767).  U convention."""
+        p, st = self.p, self.state
+        h = float(self.dt)
+        dil = self.il - old_il
+        dm = self.mm - old_m
+        tot = self.lam[2, 0:3].copy()
+        r_il, r_m, r_la, sk = st.read_rate_rotation(dil, dm, tot - lam_prev2)
+        n2 = st.nmod2
+        kTE2, hTE2, kE2 = self.love.kTE[2], self.love.hTE[2], self.love.ke[2]
+        ild = dil / h
+        sild = np.zeros((3, 3))
+        smd = np.zeros(3)
+        tsld = np.zeros(3, dtype=np.complex128)
+        trrd = np.zeros(3, dtype=np.complex128)
+        for k in range(n2):
+            sild += (st.a_il[k] * sk[k]) * (self.il - r_il[k])
+            smd += (st.a_m[k] * sk[k]) * (self.mm - r_m[k])
+            tsld += (st.a_tsl[k] * sk[k]) * (tot - r_la[k])
+            trrd += (st.a_trr[k] * sk[k]) * (tot - r_la[k])
+        alpha = 1 / (1 - kTE2 / p.kf)
+        md = np.zeros(3)
+        for i in (0, 1):
+            md[i] = alpha * ((1 / (p.moiC - p.moiA)) * ((1 + kE2) * ild[i, 2] + sild[i, 2]) + smd[i] / p.kf)
+        md[2] = -(1.0 / p.moiC) * ((1 + kE2) * ild[2, 2] + sild[2, 2])
+        m1, m2, m3 = self.mm
+        d1, d2, d3 = md
+        r2w2 = p.radius**2 * p.omega**2
+        lad = np.zeros((3, 3), dtype=np.complex128)
+        lad[0, 0] = (r2w2 / 3.0) * (2 * m1 * d1 + 2 * m2 * d2 + 2 * m3 * d3 + 2.0 * d3)
+        lad[2, 0] = (r2w2 / (6.0 * math.sqrt(5.0))) * (2 * m1 * d1 + 2 * m2 * d2 - 4 * m3 * d3 - 4 * d3)
+        lad[2, 1] = -(r2w2 / math.sqrt(30.0)) * ((d1 - 1j * d2) * (1.0 + m3) + (m1 - 1j * m2) * d3)
+        lad[2, 2] = (r2w2 / math.sqrt(5.0 * 24.0)) * (2 * m2 * d2 - 2 * m1 * d1 + 2.0 * 1j * (d1 * m2 + m1 * d2))
+        rate_sl = ((1 + kTE2 - hTE2) * lad[2, 0:3] + tsld) / p.gacc
+        rate_rr = (hTE2 * lad[2, 0:3] + trrd) / p.gacc
+        return rate_sl, rate_rr, {"ildot": ild, "mdot": md, "lambdadot": lad}
+
+    def _rate_spectra(self, cur, old_il, old_m, lam_prev2):
+        """dSL/dt (degrees >= 1) and dR/dt spectra at the end of the step, before the commit
+        (solver.py:769-790)."""
+        h = float(self.dt)
+        rv_sl, rv_rr = self.state.read_rate(cur)
+        dsl = self.ttlekhl * cur / h + self.ttl * rv_sl
+        drr = self.ttlhe * cur / h + self.ttl * rv_rr
+        dsl[0, :] = 0.0
+        drr[0, :] = 0.0
+        rot = None
+        if self.tpw and self.L >= 2:
+            rs, rr, rot = self._rotation_rate(old_il, old_m, lam_prev2)
+            dsl[2, 0:3] += rs
+            drr[2, 0:3] += rr
+        return dsl, drr, rot
+
+    def _rate_grids(self, dsl, drr, dice00, rot):
+        """Rate grids; degree 0 of dSL/dt from the conservation constant on the end-of-step ocean
+        function C* = C beta (solver.py:792-806)."""
+        p, sht = self.p, self.sht
+        g = sht.synthesis(dsl)
+        cst = self.cxy * self.beta
+        a = sht.analysis(g * cst)[0, 0].real
+        c00 = sht.analysis(cst)[0, 0].real
+        dsl[0, 0] = (-(p.rhoi / p.rhow) * dice00.real / float(self.dt) - a) / c00
+        out = {"dSLdt": sht.synthesis(dsl), "dRdt": sht.synthesis(drr), "dSLdt_lm": dsl, "dRdt_lm": drr}
+        if rot is not None:
+            out.update(rot)
+        return out
+
+    def step(self, ice_cur, dt=None, rates=False):
         self.set_step_dt(dt)
         p, sht, st = self.p, self.sht, self.state
         self.iter += 1
@@ -191,12 +291,23 @@ class SeaLevelSolver:
                 self.il[0, 2] = (-c4 / math.sqrt(30.0)) * z21.real
                 self.il[1, 2] = (c4 / math.sqrt(30.0)) * (-1j * z21).real
                 dil = self.il - old_il
-                # Liouville equations (solver.py:940-946)
-                for i in (0, 1):
-                    self.mm[i] = (1 / (1 - kTE2 / p.kf)) * (
-                        (1 / (p.moiC - p.moiA)) * ((1 + kE2) * self.il[i, 2] + sum_il[i, 2])
-                        + (1 / p.kf) * sum_m[i])
-                self.mm[2] = -(1.0 / p.moiC) * ((1 + kE2) * self.il[2, 2] + sum_il[2, 2])
+                if self.ramp:
+                    # current increments carry 1 - phi of their viscous response: implicit in m
+                    # (solver.py:922-933)
+                    sil_r = sum_il + st.c_il * dil
+                    al = 1 / (1 - kTE2 / p.kf)
+                    for i in (0, 1):
+                        self.mm[i] = al * (
+                            (1 / (p.moiC - p.moiA)) * ((1 + kE2) * self.il[i, 2] + sil_r[i, 2])
+                            + (1 / p.kf) * (sum_m[i] - st.c_m * old_m[i])) / (1 - al * st.c_m / p.kf)
+                    self.mm[2] = -(1.0 / p.moiC) * ((1 + kE2) * self.il[2, 2] + sil_r[2, 2])
+                else:
+                    # Liouville equations (solver.py:940-946)
+                    for i in (0, 1):
+                        self.mm[i] = (1 / (1 - kTE2 / p.kf)) * (
+                            (1 / (p.moiC - p.moiA)) * ((1 + kE2) * self.il[i, 2] + sum_il[i, 2])
+                            + (1 / p.kf) * sum_m[i])
+                    self.mm[2] = -(1.0 / p.moiC) * ((1 + kE2) * self.il[2, 2] + sum_il[2, 2])
                 dm = self.mm - old_m
                 m1, m2, m3 = self.mm
                 self.lam[0, 0] = (r2w2 / 3.0) * ((m1**2 + m2**2 + m3**2) + 2.0 * m3)
@@ -205,12 +316,18 @@ class SeaLevelSolver:
                 self.lam[2, 2] = (r2w2 / math.sqrt(5.0 * 24.0)) * (m2**2 - m1**2 + 2.0 * 1j * m1 * m2)
                 dlam = self.lam[2, 0:3] - lam_prev2
                 tot = lam_prev2 + dlam
-                dsl_rot = ((1 + kTE2 - hTE2) * tot + sum_tsl) / p.gacc
-                rr_rot = (hTE2 * tot + sum_trr) / p.gacc if self.calcRG else np.zeros(3, complex)
+                tsl_v, trr_v = sum_tsl, sum_trr
+                if self.ramp:
+                    tsl_v = sum_tsl + st.c_tsl * dlam
+                    trr_v = sum_trr + st.c_trr * dlam
+                dsl_rot = ((1 + kTE2 - hTE2) * tot + tsl_v) / p.gacc
+                rr_rot = (hTE2 * tot + trr_v) / p.gacc if self.calcRG else np.zeros(3, complex)
 
             # ---- eq. B18 and the grid round trip
             A = (p.rhoi * dI + p.rhow * dS_prev) + p.rhow * load_inc
             sl_inc = A * self.ttlekhl + visc * self.ttl
+            if self.ramp:  # same-step viscous weight of the current increment (solver.py:986-989)
+                sl_inc += (self.ttl[:, 0] * st.c_sl)[:, None] * (p.rhoi * dice + p.rhow * load_inc)
             if self.tpw:
                 sl_inc[2, 0:3] += dsl_rot
             dslxy = sht.synthesis(sl_inc)
@@ -259,13 +376,23 @@ class SeaLevelSolver:
         if self.calcRG:
             A = (p.rhoi * dI + p.rhow * dS_prev) + p.rhow * load_inc
             rr_lm = A * self.ttlhe + rad * self.ttl
+            if self.ramp:
+                rr_lm += (self.ttl[:, 0] * st.c_rr)[:, None] * (p.rhoi * dice + p.rhow * load_inc)
             if self.tpw:
                 rr_lm[2, 0:3] += rr_rot
             rrxy = sht.synthesis(rr_lm)
 
+        hz = None
+        if self.horizontal:
+            hz = self.horizontal_lm(dI, dS_prev, load_inc, dice, lam_prev2 + dlam if self.tpw else None)
+
         # ---- commit (row l = 0 of the load excluded, solver.py:1080-1095)
         cur = p.rhoi * dice + p.rhow * load_inc
         cur[0, :] = 0.0
+        rate_sp = None
+        if rates:
+            rate_sp = self._rate_spectra(cur, old_il, old_m, lam_prev2)
+            dice00 = complex(dice[0, 0])
         if self.tpw:
             st.commit(cur, dil, dm, dlam)
         else:
@@ -287,6 +414,7 @@ class SeaLevelSolver:
         cstar00 = sht.analysis(cst)[0, 0].real
         gmslc_beta = (sht.analysis(slc * self.beta)[0, 0].real) / cstar00
         self.ice_prev = ice_cur
+        rate_out = self._rate_grids(*rate_sp[:2], dice00, rate_sp[2]) if rates else None
         return {
             "iter": self.iter,
             "numiter": numiter,
@@ -295,6 +423,8 @@ class SeaLevelSolver:
             "beta": self.beta.copy(),
             "R": rrxy,
             "G": (deltasl + rrxy) if rrxy is not None else None,
+            "horiz": hz,
+            "rates": rate_out,
             "dS_converged": dS_cur,
             "ocean_area": clm00 * area,
             "oceanBeta_area": cstar00 * area,
