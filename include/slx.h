@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SLX_ABI_VERSION 1
+#define SLX_ABI_VERSION 2
 
 typedef struct slx_sht slx_sht;
 typedef struct slx_solver slx_solver;
@@ -69,9 +69,9 @@ int slx_modal_advance_read(const slx_sht* h, int kmax, const int* nmod, const do
 
 /* ------------------------------------------------------------------- the step ------------
  * Replaces SeaLevelSolver (solver.py:521-1144) and OptSeaLevelSolver with inner = "off" or
- * "extrap" (solver_opt.py:369-979), default step placement, axial rotation "off".  All state
- * is device-resident; the inner fixed point runs as predicated launches that stop on a
- * device convergence flag. */
+ * "extrap" (solver_opt.py:369-979), step or second-order ("ramp") load placement, axial
+ * rotation "off".  All state is device-resident; the inner fixed point runs as predicated
+ * launches that stop on a device convergence flag. */
 typedef struct slx_solver_params {
   int lmax, kmax, nmod2;
   int tpw, checkmarine, calcRG, extrap; /* booleans */
@@ -84,6 +84,12 @@ typedef struct slx_solver_params {
   const double* a_rr;                               /* host [kmax][lmax+1] r/s */
   const double* a_il, *a_m, *a_tsl, *a_trr;         /* host [kmax] degree-2 rotational weights */
   const double* ttl, *ttlekhl, *ttlhe;              /* host [lmax+1] T_l, T_l E_l, T_l h_l */
+  int ramp;          /* second-order load placement, SeaLevelSolver(ramp=True) (solver.py:535,549-551) */
+  /* horizontal read-out (solver.py:424-434,687-720); a_hl NULL = not available */
+  const double* a_hl;                               /* host [kmax][lmax+1] l_k/s_k */
+  const double* a_thl;                              /* host [kmax] tidal l_k/s_k at degree 2 */
+  const double* ttlle;                              /* host [lmax+1] T_l l^E_l */
+  double lTE2;                                      /* tidal elastic l number at degree 2 */
 } slx_solver_params;
 
 typedef struct slx_status {
@@ -116,7 +122,8 @@ int slx_solver_destroy(slx_solver* s);
 /* Step-0 set-up (solver.py:554-605): tinit, C0, marine-checked I*0, beta0, C*0 and the three
  * degree-0 areas.  topo0, ice0: device grids. */
 int slx_solver_init(slx_solver* s, const double* topo0, const double* ice0, void* stream);
-/* Step length (years) of the following steps: decay = exp(-s dt / 1000) (solver.py:331-346). */
+/* Step length (years) of the following steps: decay = exp(-s dt / 1000), the ramp factors
+ * 1 - phi(s dt) and their sums (solver.py:331-367). */
 int slx_solver_set_dt(slx_solver* s, double dt_years);
 /* Start a step with the ice grid ice_cur (device): modal advance, grounded-ice projection,
  * ice / ocean analyses and the first guess (eustatic, or extrapolated with the step lengths
@@ -136,6 +143,21 @@ int slx_step_iterate(slx_solver* s, int n, void* stream);
  * launches run only once the inner loop has converged and only once per step, so the host may
  * enqueue them right after a batch of iterations and test status.finished afterwards. */
 int slx_step_finish(slx_solver* s, void* stream);
+/* Optional read-outs of a step, computed on the same state by the finish (caller-owned device
+ * buffers, NULL = not wanted).  Rates (solver.py:722-806, step(rates=True)): the analytic left
+ * time derivatives at the end of the step with the current increment accruing linearly over it;
+ * dSLdt, dRdt [nlat][nlon] m/yr, their packed [ncoef] complex spectra (dSLdt_lm degree 0 from the
+ * conservation constant on the end-of-step C*), and rot_rates[30] = il-dot (3x3), m-dot (3),
+ * Lambda-dot (3x3 complex, (re, im)); give all five or none.  horiz_lm: packed [ncoef] complex
+ * Psi_lm of the horizontal read-out (solver.py:687-720, result key "horiz"; needs a_hl). */
+typedef struct slx_readouts {
+  double* dSLdt, *dRdt;
+  void* dSLdt_lm, *dRdt_lm;
+  double* rot_rates;
+  void* horiz_lm;
+} slx_readouts;
+/* slx_step_finish with read-outs (r NULL = slx_step_finish). */
+int slx_step_finish_readouts(slx_solver* s, const slx_readouts* r, void* stream);
 /* Synchronous: copies the status block to *out (host) after the stream's pending work. */
 int slx_solver_status(slx_solver* s, slx_status* out, void* stream);
 int slx_solver_fields(slx_solver* s, slx_fields* out);

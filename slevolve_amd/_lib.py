@@ -21,7 +21,8 @@ class SlxSolverParams(C.Structure):
                 ("kTE2", c_double), ("hTE2", c_double), ("kE2", c_double),
                 ("nmod", P_int), ("s", P_double), ("a_sl", P_double), ("a_rr", P_double),
                 ("a_il", P_double), ("a_m", P_double), ("a_tsl", P_double), ("a_trr", P_double),
-                ("ttl", P_double), ("ttlekhl", P_double), ("ttlhe", P_double)]
+                ("ttl", P_double), ("ttlekhl", P_double), ("ttlhe", P_double),
+                ("ramp", c_int), ("a_hl", P_double), ("a_thl", P_double), ("ttlle", P_double), ("lTE2", c_double)]
 
 
 class SlxStatus(C.Structure):
@@ -41,6 +42,10 @@ class SlxFields(C.Structure):
 
 class SlxOutputs(C.Structure):
     _fields_ = [(n, c_void_p) for n in ("tgrid", "ocean", "beta", "R", "G")]
+
+
+class SlxReadouts(C.Structure):
+    _fields_ = [(n, c_void_p) for n in ("dSLdt", "dRdt", "dSLdt_lm", "dRdt_lm", "rot_rates", "horiz_lm")]
 
 
 # name -> (restype, argtypes)
@@ -65,6 +70,7 @@ SIGNATURES = {
     "slx_step_begin_warm": (c_int, [c_void_p, c_void_p, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
     "slx_step_iterate": (c_int, [c_void_p, c_int, c_void_p]),
     "slx_step_finish": (c_int, [c_void_p, c_void_p]),
+    "slx_step_finish_readouts": (c_int, [c_void_p, C.POINTER(SlxReadouts), c_void_p]),
     "slx_solver_status": (c_int, [c_void_p, C.POINTER(SlxStatus), c_void_p]),
     "slx_solver_fields": (c_int, [c_void_p, C.POINTER(SlxFields)]),
 }

@@ -1,6 +1,7 @@
-// Device-resident sea-level time step (FP64).  Restates python/slevolve/solver.py:521-1144 and
-// the extrapolated start of solver_opt.py:148-165,504-527 with the grid and spectral algebra fused
-// into the transform passes:
+// Device-resident sea-level time step (FP64).  Restates python/slevolve/solver.py:521-1144 (step
+// or second-order load placement; the optional rate and horizontal read-outs of solver.py:687-806)
+// and the extrapolated start of solver_opt.py:148-165,504-527 with the grid and spectral algebra
+// fused into the transform passes:
 //   * rO - tO is analysed as one field (its analysis is all the update uses, solver.py:1002-1015);
 //   * the ocean function C* is analysed in the same sweep as rO - tO of the next iteration;
 //   * deltasl = dslxy + conserv K00 (one synthesis less per iteration, solver_opt.py:28-32);
@@ -36,6 +37,7 @@ struct DevState {
   double dil[9], dm[3], dlam[6];
   double sum_il[9], sum_m[3], sum_tsl[6], sum_trr[6];
   double dsl_rot[6], rr_rot[6];
+  double rate_sl[6], rate_rr[6], hz_rot[6], rate00;  // read-outs of the finishing step
 };
 
 // Constants evaluated on the host once, in the reference's operation order.
@@ -44,9 +46,12 @@ struct Consts {
   double cI33, cI13, cI23;                   // inertia factors (solver.py:915-920)
   double alpha, inv_CA, one_kE2, inv_kf, inv_C;
   double K0, K20, K21, K22;                  // Lambda factors (solver.py:955-959)
-  double ekhTE, hTE2;
+  double ekhTE, hTE2, lTE2;
   int L, nl, ncoef, npts, nlat, nlon, kmax, nmod2, tpw, checkmarine, calcRG;
   int idx20, idx21, idx22;
+  int ramp;                                  // second-order load placement
+  double dt;                                 // length of the current step (years)
+  double c_il, c_m, c_tsl, c_trr, c_thl;     // sum_k (1 - phi_k) a_k at degree 2 (solver.py:362-367)
 };
 
 struct slx_solver {
@@ -66,6 +71,14 @@ struct slx_solver {
   double *a2 = nullptr;       // [4][kmax] a_il, a_m, a_tsl, a_trr
   double *r2 = nullptr;       // [kmax][18] degree-2 states r_il(9), r_m(3), r_lambda(6)
   double *ttl = nullptr, *ttlekhl = nullptr, *ttlhe = nullptr;
+  // second-order placement and read-outs: 1 - phi(s dt) [kmax][nl]; rate weights a s / 1000
+  // [kmax][nl]; horizontal weights a_hl [kmax][nl], a_thl [kmax]; T_l l^E_l [nl]; per-degree
+  // [7][nl]: T c_sl, T c_rr, ws_sl, ws_rr, wsphi_sl, wsphi_rr, T c_hl
+  double *omphi = nullptr, *w_sl = nullptr, *w_rr = nullptr, *a_hl = nullptr, *a_thl = nullptr, *ttlle = nullptr,
+         *deg = nullptr;
+  int has_horiz = 0;
+  std::vector<double> h_a_sl, h_a_rr, h_a2, h_a_hl, h_a_thl, h_ttl;  // host copies for the ramp sums
+  std::vector<int> h_nmod;
   // grids
   double *tinit, *cxy0, *beta0, *cstar0, *cxy, *beta, *cstar, *tO, *icestar, *dsl, *rr, *G, *topo, *topo_prev,
       *ice_prev, *tmp;
@@ -94,6 +107,10 @@ struct Pts {  // all device pointers a kernel may need, passed by value
   const double *ttl, *ttlekhl, *ttlhe, *decay, *a2;
   double* r2;
   const int* l_of;
+  const int* m_of;
+  const int* nmod;
+  double2* R;
+  const double *s, *omphi, *w_sl, *w_rr, *a_hl, *a_thl, *ttlle, *deg;
 };
 
 __device__ __forceinline__ double keep_factor(double ice, double topo, double rhow, double rhoi) {
@@ -134,9 +151,22 @@ __device__ void rotation_eval(const DevState* s, const Consts& c, double2 dS20, 
   o.il[2] = c.cI13 * z21r;
   o.il[5] = c.cI23 * z21i;
   for (int q = 0; q < 9; ++q) o.dil[q] = o.il[q] - s->p_il[q];
-  for (int i = 0; i < 2; ++i)
-    o.mm[i] = c.alpha * (c.inv_CA * (c.one_kE2 * o.il[3 * i + 2] + s->sum_il[3 * i + 2]) + c.inv_kf * s->sum_m[i]);
-  o.mm[2] = -c.inv_C * (c.one_kE2 * o.il[8] + s->sum_il[8]);
+  if (c.ramp) {
+    // the current increments carry 1 - phi of their viscous response: implicit in m
+    // (solver.py:922-933)
+    double silr[9];
+    for (int q = 0; q < 9; ++q) silr[q] = s->sum_il[q] + c.c_il * o.dil[q];
+    const double den = 1.0 - c.alpha * c.c_m * c.inv_kf;
+    for (int i = 0; i < 2; ++i)
+      o.mm[i] = c.alpha *
+                (c.inv_CA * (c.one_kE2 * o.il[3 * i + 2] + silr[3 * i + 2]) + c.inv_kf * (s->sum_m[i] - c.c_m * s->p_m[i])) /
+                den;
+    o.mm[2] = -c.inv_C * (c.one_kE2 * o.il[8] + silr[8]);
+  } else {
+    for (int i = 0; i < 2; ++i)
+      o.mm[i] = c.alpha * (c.inv_CA * (c.one_kE2 * o.il[3 * i + 2] + s->sum_il[3 * i + 2]) + c.inv_kf * s->sum_m[i]);
+    o.mm[2] = -c.inv_C * (c.one_kE2 * o.il[8] + s->sum_il[8]);
+  }
   for (int q = 0; q < 3; ++q) o.dm[q] = o.mm[q] - s->p_m[q];
   const double m1 = o.mm[0], m2 = o.mm[1], m3 = o.mm[2];
   o.lam[0] = c.K0 * ((m1 * m1 + m2 * m2 + m3 * m3) + 2.0 * m3);
@@ -151,8 +181,13 @@ __device__ void rotation_eval(const DevState* s, const Consts& c, double2 dS20, 
   const double ig = 1.0 / c.gacc;
   for (int q = 0; q < 6; ++q) {
     const double tot = s->p_lam[q] + o.dlam[q];
-    o.dsl[q] = (c.ekhTE * tot + s->sum_tsl[q]) * ig;
-    o.rr[q] = c.calcRG ? (c.hTE2 * tot + s->sum_trr[q]) * ig : 0.0;
+    double tsl = s->sum_tsl[q], trr = s->sum_trr[q];
+    if (c.ramp) {
+      tsl += c.c_tsl * o.dlam[q];
+      trr += c.c_trr * o.dlam[q];
+    }
+    o.dsl[q] = (c.ekhTE * tot + tsl) * ig;
+    o.rr[q] = c.calcRG ? (c.hTE2 * tot + trr) * ig : 0.0;
   }
 }
 
@@ -316,13 +351,17 @@ __global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
 
 // ---------------------------------------------------------------------- inner iteration
 // eq. B18 coefficient, assembled while the synthesis stages its coefficients:
-// w1[l] ((rho_i dI + rho_w dS_prev) + rho_w load_inc) + w2[l] resp (+ rotation at l = 2, m <= 2).
+// w1[l] ((rho_i dI + rho_w dS_prev) + rho_w load_inc) + w2[l] resp
+// (+ w3[l] (rho_i dice + rho_w load_inc) with the ramp, solver.py:986-989,1065-1068)
+// (+ rotation at l = 2, m <= 2).
 // With eval set (sea-level synthesis of an inner iteration) every block evaluates the rotation
 // block in its prologue from dS_cur and dI (identical arithmetic everywhere) and block 0
 // publishes it; otherwise (radial synthesis after the loop) the stored rr terms are used.
 struct CoefB18 {
   const double2 *dI, *dSp, *ld, *resp, *dS;
   const double *w1, *w2;
+  const double2* dice;
+  const double* w3;
   DevState* st;
   Consts c;
   int eval;
@@ -353,6 +392,12 @@ struct CoefB18 {
     const double ai = (a.y * c.rhoi + b.y * c.rhow) + q.y * c.rhow;
     const double u = w1[l], t = w2[l];
     double2 r = make_double2(ar * u + v.x * t, ai * u + v.y * t);
+    if (c.ramp) {
+      const double2 d = dice[idx];
+      const double w = w3[l];
+      r.x += w * (c.rhoi * d.x + c.rhow * q.x);
+      r.y += w * (c.rhoi * d.y + c.rhow * q.y);
+    }
     if (l == 2 && m <= 2) {
       r.x += pro[2 * m];
       r.y += pro[2 * m + 1];
@@ -581,6 +626,16 @@ __global__ void k_commit(Pts p, Consts c, int extrap) {
   const double2 d = p.dice[i], q = p.load_inc[i];
   double2 cur = make_double2(d.x * c.rhoi + q.x * c.rhow, d.y * c.rhoi + q.y * c.rhow);
   if (l == 0) cur = make_double2(0.0, 0.0);
+  if (c.ramp) {
+    // the increment already carries 1 - phi_k of its response in every mode (solver.py:486-498)
+#pragma clang fp contract(off)
+    const int nk = p.nmod[l] < c.kmax ? p.nmod[l] : c.kmax;
+    for (int k = 0; k < nk; ++k) {
+      const double o = p.omphi[k * c.nl + l];
+      const double2 r = p.R[(long)k * c.ncoef + i];
+      p.R[(long)k * c.ncoef + i] = make_double2(r.x + o * cur.x, r.y + o * cur.y);
+    }
+  }
   const double2 pl = p.p_load[i];
   p.p_load[i] = make_double2(pl.x + cur.x, pl.y + cur.y);
   p.p_ice[i] = p.dI[i];
@@ -589,13 +644,14 @@ __global__ void k_commit(Pts p, Consts c, int extrap) {
   if (extrap) p.h2[i] = q;  // becomes the newest history entry (the host swaps h1/h2)
 }
 
-// final topography, ocean function, G and the row sums of the degree-0 diagnostics (one block per row)
-__global__ void __launch_bounds__(256) k_post_grid(Pts p, Consts c) {
+// final topography, ocean function, G and the row sums of the degree-0 diagnostics (one block per
+// row); with g0 (the dSL/dt grid without its degree 0) also the row sums of g0 C* on the final C*
+__global__ void __launch_bounds__(256) k_post_grid(Pts p, Consts c, const double* g0) {
   if (!finish_active(p.st)) return;
   __shared__ double sh[256];
   const int j = blockIdx.x;
   const double cadd = p.st->st.conserv * c.K00;
-  double a = 0.0, b = 0.0, e = 0.0, f = 0.0;
+  double a = 0.0, b = 0.0, e = 0.0, f = 0.0, r5 = 0.0;
   for (int i = threadIdx.x; i < c.nlon; i += blockDim.x) {
     const long k = (long)j * c.nlon + i;
     const double dsl = p.dsl[k] + cadd;
@@ -610,6 +666,7 @@ __global__ void __launch_bounds__(256) k_post_grid(Pts p, Consts c) {
     b += o;
     e += sl * bt;
     f += o * bt;
+    if (g0) r5 += g0[k] * (o * bt);
   }
   double r;
   r = block_sum(a, sh);
@@ -623,21 +680,43 @@ __global__ void __launch_bounds__(256) k_post_grid(Pts p, Consts c) {
   __syncthreads();
   r = block_sum(f, sh);
   if (threadIdx.x == 0) p.rows[3 * c.nlat + j] = r;
+  if (g0) {
+    __syncthreads();
+    r = block_sum(r5, sh);
+    if (threadIdx.x == 0) p.rows[4 * c.nlat + j] = r;
+  }
 }
 
-// degree-0 diagnostics and the rotational commit (solver.py:1060-1110; solver_opt.py:525-547)
-__global__ void __launch_bounds__(256) k_finish(Pts p, Consts c) {
+// degree-0 diagnostics and the rotational commit (solver.py:1060-1110; solver_opt.py:525-547);
+// with the ramp the degree-2 states take their share of the increments (solver.py:490-498).
+// With rate read-outs (rates != 0) the degree-0 coefficient of dSL/dt from the conservation
+// constant on the end-of-step C* (solver.py:792-806) goes to rate00 and dsl_lm[0]; close = 0
+// leaves status.finished to k_close.
+__global__ void __launch_bounds__(256) k_finish(Pts p, Consts c, int rates, double2* dsl_lm, int close) {
   __shared__ double sh[256];
   DevState* s = p.st;
   if (!finish_active(s)) return;
-  double q[4];
-  for (int k = 0; k < 4; ++k) {
+  double q[5];
+  const int nq = rates ? 5 : 4;
+  for (int k = 0; k < nq; ++k) {
     double v = 0.0;
     for (int j = threadIdx.x; j < c.nlat; j += blockDim.x) v += p.w00[j] * p.rows[k * c.nlat + j];
     q[k] = block_sum(v, sh);
     __syncthreads();
   }
+  if (c.ramp && c.tpw) {
+    for (int idx = threadIdx.x; idx < 18 * c.nmod2; idx += blockDim.x) {
+      const int k = idx / 18, e = idx % 18;
+      const double d = e < 9 ? s->dil[e] : (e < 12 ? s->dm[e - 9] : s->dlam[e - 12]);
+      p.r2[idx] += p.omphi[k * c.nl + 2] * d;
+    }
+  }
   if (threadIdx.x != 0) return;
+  if (rates) {
+    const double d00 = ((-(c.rhoi / c.rhow) * p.dice[0].x) / c.dt - q[4]) / q[3];
+    s->rate00 = d00;
+    dsl_lm[0] = make_double2(d00, 0.0);
+  }
   s->st.gmslc_ocn = q[0] / q[1];
   s->st.gmslc_ocnBeta = q[2] / q[3];
   s->st.ocean_area = q[1] * c.area;
@@ -648,7 +727,172 @@ __global__ void __launch_bounds__(256) k_finish(Pts p, Consts c) {
     for (int k = 0; k < 3; ++k) s->p_m[k] += s->dm[k];
     for (int k = 0; k < 6; ++k) s->p_lam[k] += s->dlam[k];
   }
-  s->st.finished = 1;
+  if (close) s->st.finished = 1;
+}
+
+// ---------------------------------------------------------------------- read-outs
+// Rotational read-outs of the finishing step, before its commit (one lane):
+// * rates (solver.py:722-767): il-dot, m-dot from the differentiated Liouville equations with the
+//   ramp-consistent states r_k + (1 - phi_k) d(.), Lambda-dot by the chain rule, and the degree-2
+//   sea-level / radial rate terms (rate_sl, rate_rr);
+// * horizontal (solver.py:708-716): the degree-2 rotational term with the tidal l numbers (hz_rot).
+__global__ void k_rot_readout(Pts p, Consts c, int rates, int hor, double* rot_rates) {
+  DevState* s = p.st;
+  if (!finish_active(s) || threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int n2 = c.nmod2, nl = c.nl, K = c.kmax;
+  const bool rot = c.tpw && c.L >= 2;
+  if (rates) {
+    double ild[9] = {}, md[3] = {}, lad[18] = {};
+    for (int q = 0; q < 6; ++q) s->rate_sl[q] = s->rate_rr[q] = 0.0;
+    if (rot) {
+      const double h = c.dt;
+      const double* il = s->st.il;
+      const double* mm = s->st.mm;
+      const double* tot = s->st.lam + 12;
+      double dil[9], dm[3], dla[6], sild[9] = {}, smd[3] = {}, tsld[6] = {}, trrd[6] = {};
+      for (int q = 0; q < 9; ++q) dil[q] = il[q] - s->p_il[q];
+      for (int q = 0; q < 3; ++q) dm[q] = mm[q] - s->p_m[q];
+      for (int q = 0; q < 6; ++q) dla[q] = tot[q] - s->p_lam[q];
+      for (int q = 0; q < 9; ++q) ild[q] = dil[q] / h;
+      for (int k = 0; k < n2; ++k) {
+        const double o = p.omphi[k * nl + 2], sk = p.s[k * nl + 2] / 1000.0;
+        const double* r = p.r2 + 18 * k;
+        const double wil = p.a2[k] * sk, wm = p.a2[K + k] * sk, wt = p.a2[2 * K + k] * sk,
+                     wr = p.a2[3 * K + k] * sk;
+        for (int q = 0; q < 9; ++q) sild[q] += wil * (il[q] - (r[q] + o * dil[q]));
+        for (int q = 0; q < 3; ++q) smd[q] += wm * (mm[q] - (r[9 + q] + o * dm[q]));
+        for (int q = 0; q < 6; ++q) {
+          const double rl = r[12 + q] + o * dla[q];
+          tsld[q] += wt * (tot[q] - rl);
+          trrd[q] += wr * (tot[q] - rl);
+        }
+      }
+      for (int i = 0; i < 2; ++i)
+        md[i] = c.alpha * (c.inv_CA * (c.one_kE2 * ild[3 * i + 2] + sild[3 * i + 2]) + smd[i] * c.inv_kf);
+      md[2] = -c.inv_C * (c.one_kE2 * ild[8] + sild[8]);
+      const double m1 = mm[0], m2 = mm[1], m3 = mm[2], d1 = md[0], d2 = md[1], d3 = md[2];
+      lad[0] = c.K0 * (2 * m1 * d1 + 2 * m2 * d2 + 2 * m3 * d3 + 2.0 * d3);
+      lad[12] = c.K20 * (2 * m1 * d1 + 2 * m2 * d2 - 4 * m3 * d3 - 4 * d3);
+      // order 1 in the U basis: -(r^2 w^2 / sqrt 30) ((d1 - i d2)(1 + m3) + (m1 - i m2) d3)
+      lad[14] = -c.K21 * (d1 * (1.0 + m3) + m1 * d3);
+      lad[15] = -c.K21 * (-d2 * (1.0 + m3) - m2 * d3);
+      lad[16] = c.K22 * (2 * m2 * d2 - 2 * m1 * d1);
+      lad[17] = c.K22 * (2.0 * (d1 * m2 + m1 * d2));
+      const double ig = 1.0 / c.gacc;
+      for (int q = 0; q < 6; ++q) {
+        s->rate_sl[q] = (c.ekhTE * lad[12 + q] + tsld[q]) * ig;
+        s->rate_rr[q] = (c.hTE2 * lad[12 + q] + trrd[q]) * ig;
+      }
+    }
+    for (int q = 0; q < 9; ++q) rot_rates[q] = ild[q];
+    for (int q = 0; q < 3; ++q) rot_rates[9 + q] = md[q];
+    for (int q = 0; q < 18; ++q) rot_rates[12 + q] = lad[q];
+  }
+  if (hor) {
+    for (int q = 0; q < 6; ++q) s->hz_rot[q] = 0.0;
+    if (rot) {
+      for (int q = 0; q < 6; ++q) {
+        double thl = 0.0;
+        for (int k = 0; k < n2; ++k) thl += p.a_thl[k] * p.r2[18 * k + 12 + q];
+        const double tot = s->p_lam[q] + s->dlam[q];
+        if (c.ramp) thl = thl + c.c_thl * (tot - s->p_lam[q]);
+        s->hz_rot[q] = (c.lTE2 * tot + thl) / c.gacc;
+      }
+    }
+  }
+}
+
+// Spectral read-outs of the finishing step, before its commit, one packed (l, m) per lane:
+// * rates (solver.py:454-474,769-790): cur = rho_i dice + rho_w load_inc (row l = 0 zero),
+//   rs = ws P - sum_k w_sl R_k + wsphi cur, dSL/dt = T E cur / dt + T rs (row l = 0 zero) plus
+//   the rotational term; dR/dt likewise with h and w_rr;
+// * horizontal (solver.py:687-720): Psi = T l^E A + T sum_k a_hl R_k
+//   (+ T c_hl (rho_i dice + rho_w load_inc) with the ramp), row l = 0 zero, plus hz_rot.
+// The modal sums run over k in order, each product rounded before its sum.
+struct ReadoutSpec {
+  double2 *dsl, *drr, *dsl_user, *drr_user, *hz;
+};
+__global__ void __launch_bounds__(256) k_readout_spec(Pts p, Consts c, ReadoutSpec a) {
+#pragma clang fp contract(off)
+  if (!finish_active(p.st)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.ncoef) return;
+  const DevState* s = p.st;
+  const int l = p.l_of[i], m = p.m_of[i], nl = c.nl;
+  const int nk = p.nmod[l] < c.kmax ? p.nmod[l] : c.kmax;
+  const double2 d = p.dice[i], q = p.load_inc[i];
+  const double2 cur = l == 0 ? make_double2(0.0, 0.0)
+                             : make_double2(d.x * c.rhoi + q.x * c.rhow, d.y * c.rhoi + q.y * c.rhow);
+  const bool rates = a.dsl != nullptr, hor = a.hz != nullptr;
+  double sr = 0.0, si = 0.0, tr = 0.0, ti = 0.0, hr = 0.0, hi = 0.0;
+  for (int k = 0; k < nk; ++k) {
+    const double2 r = p.R[(long)k * c.ncoef + i];
+    if (rates) {
+      const double u = p.w_sl[k * nl + l], v = p.w_rr[k * nl + l];
+      sr += u * r.x;
+      si += u * r.y;
+      tr += v * r.x;
+      ti += v * r.y;
+    }
+    if (hor) {
+      const double w = p.a_hl[k * nl + l];
+      hr += w * r.x;
+      hi += w * r.y;
+    }
+  }
+  const bool rot = c.tpw && l == 2 && m <= 2;
+  if (rates) {
+    const double2 P = p.p_load[i];
+    const double* g = p.deg;
+    const double ws = g[2 * nl + l], wr = g[3 * nl + l], wps = g[4 * nl + l], wpr = g[5 * nl + l];
+    const double rsx = (ws * P.x - sr) + wps * cur.x, rsy = (ws * P.y - si) + wps * cur.y;
+    const double rrx = (wr * P.x - tr) + wpr * cur.x, rry = (wr * P.y - ti) + wpr * cur.y;
+    const double t = p.ttl[l], e = p.ttlekhl[l], hh = p.ttlhe[l];
+    double2 ds = make_double2(e * cur.x / c.dt + t * rsx, e * cur.y / c.dt + t * rsy);
+    double2 dr = make_double2(hh * cur.x / c.dt + t * rrx, hh * cur.y / c.dt + t * rry);
+    if (l == 0) ds = dr = make_double2(0.0, 0.0);
+    if (rot) {
+      ds.x += s->rate_sl[2 * m];
+      ds.y += s->rate_sl[2 * m + 1];
+      dr.x += s->rate_rr[2 * m];
+      dr.y += s->rate_rr[2 * m + 1];
+    }
+    a.dsl[i] = ds;
+    a.drr[i] = dr;
+    a.dsl_user[i] = ds;
+    a.drr_user[i] = dr;
+  }
+  if (hor) {
+    const double2 I = p.dI[i], Sp = p.p_ocean[i];
+    const double Ax = (c.rhoi * I.x + c.rhow * Sp.x) + c.rhow * q.x;
+    const double Ay = (c.rhoi * I.y + c.rhow * Sp.y) + c.rhow * q.y;
+    const double t = p.ttl[l], e = p.ttlle[l];
+    double2 h = make_double2(e * Ax + t * hr, e * Ay + t * hi);
+    if (c.ramp) {
+      const double w = p.deg[6 * nl + l];
+      h.x += w * (c.rhoi * d.x + c.rhow * q.x);
+      h.y += w * (c.rhoi * d.y + c.rhow * q.y);
+    }
+    if (l == 0) h = make_double2(0.0, 0.0);
+    if (rot) {
+      h.x += s->hz_rot[2 * m];
+      h.y += s->hz_rot[2 * m + 1];
+    }
+    a.hz[i] = h;
+  }
+}
+
+// dSL/dt grid: add the degree-0 term rate00 (synthesised as the constant rate00 K00)
+__global__ void k_rate_grid(Pts p, Consts c, double* g) {
+  if (!finish_active(p.st)) return;
+  const double add = p.st->rate00 * c.K00;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < c.npts; i += (long)gridDim.x * blockDim.x)
+    g[i] += add;
+}
+
+__global__ void k_close(Pts p) {
+  if (!finish_active(p.st) || threadIdx.x != 0 || blockIdx.x != 0) return;
+  p.st->st.finished = 1;
 }
 
 __global__ void k_init_areas(Pts p, Consts c, const double2* c0, const double2* cs0, const double2* i0) {
@@ -708,6 +952,17 @@ static Pts make_pts(slx_solver* s, const double* ice_cur, const double2* oldice)
   p.a2 = s->a2;
   p.r2 = s->r2;
   p.l_of = s->h->l_of;
+  p.m_of = s->h->m_of;
+  p.nmod = s->nmod;
+  p.R = s->R;
+  p.s = s->s;
+  p.omphi = s->omphi;
+  p.w_sl = s->w_sl;
+  p.w_rr = s->w_rr;
+  p.a_hl = s->a_hl;
+  p.a_thl = s->a_thl;
+  p.ttlle = s->ttlle;
+  p.deg = s->deg;
   return p;
 }
 
@@ -754,6 +1009,10 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   c.K22 = r2w2 / std::sqrt(5.0 * 24.0);
   c.ekhTE = 1 + p->kTE2 - p->hTE2;
   c.hTE2 = p->hTE2;
+  c.lTE2 = p->lTE2;
+  c.ramp = p->ramp ? 1 : 0;
+  c.dt = -1.0;
+  c.c_il = c.c_m = c.c_tsl = c.c_trr = c.c_thl = 0.0;
   c.L = L;
   c.nl = nl;
   c.ncoef = h->ncoef;
@@ -774,7 +1033,7 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   SLX_HIP(hipMalloc(&s->st, sizeof(DevState)));
   SLX_HIP(hipMemset(s->st, 0, sizeof(DevState)));
   SLX_HIP(hipMalloc(&s->partial, nblk_max * sizeof(double)));
-  SLX_HIP(hipMalloc(&s->rows, 4 * (size_t)c.nlat * sizeof(double)));
+  SLX_HIP(hipMalloc(&s->rows, 5 * (size_t)c.nlat * sizeof(double)));
   SLX_HIP(hipMalloc(&s->w00, (size_t)c.nlat * sizeof(double)));
   std::vector<double> w00(c.nlat);
   for (int j = 0; j < c.nlat; ++j) w00[j] = h->h_w[j] / c.nlon * std::sqrt(0.5) / SLX_SQRT2_LEGACY;
@@ -804,6 +1063,42 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   SLX_HIP(hipMemcpy(s->a2, a2.data(), a2.size() * sizeof(double), hipMemcpyHostToDevice));
   SLX_HIP(hipMalloc(&s->r2, 18 * (size_t)std::max(K, 1) * sizeof(double)));
   SLX_HIP(hipMemset(s->r2, 0, 18 * (size_t)std::max(K, 1) * sizeof(double)));
+  // second-order placement and read-outs
+  s->h_nmod = nmod;
+  s->h_a_sl.assign(p->a_sl, p->a_sl + (size_t)K * nl);
+  s->h_a_rr.assign(p->a_rr, p->a_rr + (size_t)K * nl);
+  s->h_a2 = a2;
+  s->h_ttl.assign(p->ttl, p->ttl + nl);
+  s->has_horiz = p->a_hl != nullptr;
+  SLX_CHECK(!s->has_horiz || (p->a_thl && p->ttlle), "slx_solver_create: a_hl needs a_thl and ttlle");
+  s->h_a_hl.assign((size_t)K * nl, 0.0);
+  s->h_a_thl.assign((size_t)std::max(K, 1), 0.0);
+  std::vector<double> ttlle(nl, 0.0);
+  if (s->has_horiz) {
+    std::copy(p->a_hl, p->a_hl + (size_t)K * nl, s->h_a_hl.begin());
+    std::copy(p->a_thl, p->a_thl + K, s->h_a_thl.begin());
+    std::copy(p->ttlle, p->ttlle + nl, ttlle.begin());
+  }
+  std::vector<double> wsl((size_t)K * nl), wrr((size_t)K * nl);
+  for (size_t i = 0; i < wsl.size(); ++i) {  // rate weights a_k s_k per year (solver.py:295-296)
+    wsl[i] = s->h_a_sl[i] * s->h_s[i] / 1000.0;
+    wrr[i] = s->h_a_rr[i] * s->h_s[i] / 1000.0;
+  }
+  for (double** q : {&s->omphi, &s->w_sl, &s->w_rr, &s->a_hl}) {
+    SLX_HIP(hipMalloc(q, KL));
+    SLX_HIP(hipMemset(*q, 0, KL));
+  }
+  SLX_HIP(hipMalloc(&s->a_thl, (size_t)std::max(K, 1) * sizeof(double)));
+  SLX_HIP(hipMalloc(&s->ttlle, nl * sizeof(double)));
+  SLX_HIP(hipMalloc(&s->deg, 7 * (size_t)nl * sizeof(double)));
+  SLX_HIP(hipMemset(s->deg, 0, 7 * (size_t)nl * sizeof(double)));
+  if (K > 0) {
+    SLX_HIP(hipMemcpy(s->w_sl, wsl.data(), wsl.size() * sizeof(double), hipMemcpyHostToDevice));
+    SLX_HIP(hipMemcpy(s->w_rr, wrr.data(), wrr.size() * sizeof(double), hipMemcpyHostToDevice));
+    SLX_HIP(hipMemcpy(s->a_hl, s->h_a_hl.data(), s->h_a_hl.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  SLX_HIP(hipMemcpy(s->a_thl, s->h_a_thl.data(), s->h_a_thl.size() * sizeof(double), hipMemcpyHostToDevice));
+  SLX_HIP(hipMemcpy(s->ttlle, ttlle.data(), nl * sizeof(double), hipMemcpyHostToDevice));
   for (auto pr : {std::make_pair(&s->ttl, p->ttl), std::make_pair(&s->ttlekhl, p->ttlekhl),
                   std::make_pair(&s->ttlhe, p->ttlhe)}) {
     SLX_HIP(hipMalloc(pr.first, nl * sizeof(double)));
@@ -835,7 +1130,8 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
 extern "C" int slx_solver_destroy(slx_solver* s) {
   if (!s) return 0;
   void* ptrs[] = {s->st, s->partial, s->rows, s->w00, s->nmod, s->s, s->decay, s->omd, s->a_sl, s->a_rr, s->a2,
-                  s->r2, s->ttl, s->ttlekhl, s->ttlhe, s->tinit, s->cxy0, s->beta0, s->cstar0,
+                  s->r2, s->ttl, s->ttlekhl, s->ttlhe, s->omphi, s->w_sl, s->w_rr, s->a_hl, s->a_thl, s->ttlle,
+                  s->deg, s->tinit, s->cxy0, s->beta0, s->cstar0,
                   s->cstar, s->tO, s->icestar, s->dsl, s->topo_prev, s->ice_prev, s->tmp,
                   s->p_load, s->p_ice, s->p_ocean, s->prev_ice, s->icestar_lm, s->oldice_lm, s->dice, s->dI,
                   s->cstar_lm, s->rt_lm, s->load_inc, s->dS_cur, s->visc, s->rad, s->h1, s->h2, s->scratch_lm,
@@ -847,20 +1143,76 @@ extern "C" int slx_solver_destroy(slx_solver* s) {
   return 0;
 }
 
+// phi(x) = (1 - e^-x)/x and 1 - phi(x), third-order series where |x| < 1e-4 (solver.py:229-241)
+static void ramp_factor(double x, double& phi, double& omp) {
+  if (std::fabs(x) < 1.0e-4) {
+    phi = 1.0 - x / 2.0 + x * x / 6.0 - std::pow(x, 3.0) / 24.0;
+    omp = x / 2.0 - x * x / 6.0 + std::pow(x, 3.0) / 24.0;
+  } else {
+    phi = -std::expm1(-x) / x;
+    omp = (x + std::expm1(-x)) / x;
+  }
+}
+
 extern "C" int slx_solver_set_dt(slx_solver* s, double dt) {
   SLX_CHECK(s != nullptr, "null solver");
   if (dt == s->dt) return 0;
   const int K = s->c.kmax, nl = s->c.nl;
-  std::vector<int> nmod(nl);
-  SLX_HIP(hipMemcpy(nmod.data(), s->nmod, nl * sizeof(int), hipMemcpyDeviceToHost));
+  const std::vector<int>& nmod = s->h_nmod;
   std::vector<double> d((size_t)K * nl, 0.0), o((size_t)K * nl, 0.0);
   for (int l = 1; l < nl; ++l)
     for (int k = 0; k < nmod[l] && k < K; ++k) d[(size_t)k * nl + l] = std::exp(-s->h_s[(size_t)k * nl + l] * dt / 1000.0);
   for (size_t i = 0; i < d.size(); ++i) o[i] = 1.0 - d[i];
+  // ramp factors and their sums over the modes, k in order (solver.py:348-367)
+  std::vector<double> phi((size_t)K * nl), omphi((size_t)K * nl), deg(7 * (size_t)nl, 0.0), chl(nl, 0.0);
+  for (size_t i = 0; i < phi.size(); ++i) {
+    const double sv = s->h_s[i];
+    double ph, om;
+    ramp_factor(sv * (dt / 1000.0), ph, om);
+    phi[i] = sv > 0 ? ph : 1.0;
+    omphi[i] = sv > 0 ? om : 0.0;
+  }
+  std::vector<double> csl(nl, 0.0), crr(nl, 0.0), wssl(nl, 0.0), wsrr(nl, 0.0), wpsl(nl, 0.0), wprr(nl, 0.0);
+  for (int k = 0; k < K; ++k)
+    for (int l = 0; l < nl; ++l) {
+      const size_t i = (size_t)k * nl + l;
+      const double wsl = s->h_a_sl[i] * s->h_s[i] / 1000.0, wrr = s->h_a_rr[i] * s->h_s[i] / 1000.0;
+      csl[l] += s->h_a_sl[i] * omphi[i];
+      crr[l] += s->h_a_rr[i] * omphi[i];
+      wssl[l] += wsl;
+      wsrr[l] += wrr;
+      wpsl[l] += wsl * phi[i];
+      wprr[l] += wrr * phi[i];
+      chl[l] += s->h_a_hl[i] * omphi[i];
+    }
+  for (int l = 0; l < nl; ++l) {
+    const double t = s->h_ttl[l];
+    deg[l] = t * csl[l];
+    deg[nl + l] = t * crr[l];
+    deg[2 * nl + l] = wssl[l];
+    deg[3 * nl + l] = wsrr[l];
+    deg[4 * nl + l] = wpsl[l];
+    deg[5 * nl + l] = wprr[l];
+    deg[6 * nl + l] = t * chl[l];
+  }
+  Consts& c = s->c;
+  c.c_il = c.c_m = c.c_tsl = c.c_trr = c.c_thl = 0.0;
+  if (c.L >= 2)
+    for (int k = 0; k < c.nmod2; ++k) {
+      const double w = omphi[(size_t)k * nl + 2];
+      c.c_il += w * s->h_a2[k];
+      c.c_m += w * s->h_a2[K + k];
+      c.c_tsl += w * s->h_a2[2 * K + k];
+      c.c_trr += w * s->h_a2[3 * K + k];
+      c.c_thl += w * s->h_a_thl[k];
+    }
+  c.dt = dt;
   if (K > 0) {
     SLX_HIP(hipMemcpy(s->decay, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
     SLX_HIP(hipMemcpy(s->omd, o.data(), o.size() * sizeof(double), hipMemcpyHostToDevice));
+    SLX_HIP(hipMemcpy(s->omphi, omphi.data(), omphi.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  SLX_HIP(hipMemcpy(s->deg, deg.data(), deg.size() * sizeof(double), hipMemcpyHostToDevice));
   s->dt = dt;
   return 0;
 }
@@ -942,7 +1294,7 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     const bool first = (s->iters_enqueued == 0);
     const int it_step = s->iters_enqueued;
     s->iters_enqueued += 1;
-    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->st, c, 1};
+    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->dice, s->deg, s->st, c, 1};
     int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
     if (rc) return rc;
     {
@@ -980,26 +1332,73 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
   return 0;
 }
 
-extern "C" int slx_step_finish(slx_solver* s, void* stream) {
+namespace {
+struct StorePair {  // synthesis output of two fields into two separate grids
+  double *g0, *g1;
+  __device__ void operator()(int f, long i, double v) const { (f == 0 ? g0 : g1)[i] = v; }
+};
+}  // namespace
+
+extern "C" int slx_step_finish(slx_solver* s, void* stream) { return slx_step_finish_readouts(s, nullptr, stream); }
+
+extern "C" int slx_step_finish_readouts(slx_solver* s, const slx_readouts* ro, void* stream) {
   SLX_CHECK(s != nullptr, "null solver");
   hipStream_t st = (hipStream_t)stream;
   const Consts& c = s->c;
   const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
   Pts p = make_pts(s, s->ice_cur, oldice);
+  const Pred fin{&s->st->st.done, 1, &s->st->st.finished, 0};
+  bool rates = false, hor = false;
+  if (ro) {
+    const int given = (ro->dSLdt != nullptr) + (ro->dRdt != nullptr) + (ro->dSLdt_lm != nullptr) +
+                      (ro->dRdt_lm != nullptr) + (ro->rot_rates != nullptr);
+    SLX_CHECK(given == 0 || given == 5,
+              "slx_step_finish_readouts: give all of dSLdt, dRdt, dSLdt_lm, dRdt_lm, rot_rates, or none");
+    rates = given == 5;
+    hor = ro->horiz_lm != nullptr;
+    SLX_CHECK(!hor || s->has_horiz, "slx_step_finish_readouts: horizontal read-out needs a_hl at solver creation");
+  }
   if (c.calcRG) {
-    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->rad, s->dS_cur, s->ttlhe, s->ttl, s->st, c, 0};
-    const Pred fin{&s->st->st.done, 1, &s->st->st.finished, 0};
+    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->rad, s->dS_cur, s->ttlhe, s->ttl, s->dice, s->deg + c.nl, s->st, c, 0};
     int rc = launch_leg_syn<1>(s->h, cb, fin, st);
     if (rc) return rc;
     rc = launch_fft_syn<256>(s->h, StorePlain{s->rr, 0}, 1, fin, st);
     if (rc) return rc;
   }
+  double2* dsl_user = rates ? static_cast<double2*>(ro->dSLdt_lm) : nullptr;
+  if (rates || hor) {
+    // read-outs on the states before the commit (solver.py:1074-1093)
+    k_rot_readout<<<1, 64, 0, st>>>(p, c, rates ? 1 : 0, hor ? 1 : 0, rates ? ro->rot_rates : nullptr);
+    SLX_LAUNCH_CHECK();
+    ReadoutSpec a{};
+    if (rates) {
+      a.dsl = s->scratch_lm;
+      a.drr = s->scratch_lm + c.ncoef;
+      a.dsl_user = dsl_user;
+      a.drr_user = static_cast<double2*>(ro->dRdt_lm);
+    }
+    if (hor) a.hz = static_cast<double2*>(ro->horiz_lm);
+    k_readout_spec<<<(c.ncoef + 255) / 256, 256, 0, st>>>(p, c, a);
+    SLX_LAUNCH_CHECK();
+    if (rates) {  // rate grids without the degree-0 term (solver.py:797-798)
+      int rc = launch_leg_syn<2>(s->h, CoefPlain{s->scratch_lm, c.ncoef}, fin, st);
+      if (rc) return rc;
+      rc = launch_fft_syn<256>(s->h, StorePair{ro->dSLdt, ro->dRdt}, 2, fin, st);
+      if (rc) return rc;
+    }
+  }
   k_commit<<<(c.ncoef + 255) / 256, 256, 0, st>>>(p, c, s->extrap);
   SLX_LAUNCH_CHECK();
-  k_post_grid<<<c.nlat, 256, 0, st>>>(p, c);
+  k_post_grid<<<c.nlat, 256, 0, st>>>(p, c, rates ? ro->dSLdt : nullptr);
   SLX_LAUNCH_CHECK();
-  k_finish<<<1, 256, 0, st>>>(p, c);
+  k_finish<<<1, 256, 0, st>>>(p, c, rates ? 1 : 0, dsl_user, rates ? 0 : 1);
   SLX_LAUNCH_CHECK();
+  if (rates) {
+    k_rate_grid<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, ro->dSLdt);
+    SLX_LAUNCH_CHECK();
+    k_close<<<1, 64, 0, st>>>(p);
+    SLX_LAUNCH_CHECK();
+  }
   return 0;
 }
 

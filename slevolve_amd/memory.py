@@ -40,6 +40,21 @@ def love_weights(love, lmax):
     return nmod, s, a_sl, a_rr, a2, nmod2
 
 
+def horizontal_weights(love, lmax):
+    """Horizontal read-out weights of solver.py:424-434: a_hl = l_k/s_k per (k, l) and the tidal
+    a_thl = l_k/s_k at degree 2."""
+    nl = lmax + 1
+    K = int(love.max_modes)
+    a_hl = np.zeros((K, nl))
+    a_thl = np.zeros(K)
+    for l in range(1, nl):
+        n = int(love.nmod[l])
+        a_hl[:n, l] = love.rl[:n, l] / love.s[:n, l]
+        if l == 2:
+            a_thl[:n] = love.rlT[:n, l] / love.s[:n, l]
+    return a_hl, a_thl
+
+
 def decay_factors(s, nmod, dt_years):
     """decay[k, l] = exp(-s dt / 1000) for existing modes, 0 elsewhere; omd = 1 - decay."""
     d = np.zeros_like(s)

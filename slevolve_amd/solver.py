@@ -8,10 +8,11 @@ reads one small status block back, and returns the reference's result dictionary
 it are device tensors that the solver overwrites at the next step (the reference also returns
 its reused buffers).
 
-Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap", and
-the "warm" cross-pass reuse of OptSeaLevelSolver (per-step converged states kept on the device).
-The second-order ramp, analytic rates, horizontal read-out, axial-rotation variants, Anderson
-mixing and the other reuse modes of the reference are not provided and raise.
+Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap", the
+second-order load placement (ramp), the analytic rates (step(rates=True)), the horizontal read-out
+(solver.horizontal = True, result key "horiz") and the "warm" cross-pass reuse of OptSeaLevelSolver
+(per-step converged states kept on the device).  Axial-rotation variants, Anderson mixing and the
+other reuse modes of the reference are not provided and raise.
 """
 from __future__ import annotations
 
@@ -24,7 +25,7 @@ import torch
 
 from . import _device as D
 from . import _lib
-from .memory import decay_factors, love_weights
+from .memory import decay_factors, horizontal_weights, love_weights
 from .sht import SHT
 
 PI = 3.1415926535898
@@ -113,8 +114,6 @@ class SeaLevelSolver:
     def __init__(self, sht, love, topo0, ice0, *, norder=512, dt=1, tpw=True, checkmarine=True, calcRG=True,
                  epsilon1=1.0e-5, planet=None, mem_threads=1, ramp=False, axial_rotation="off",
                  axial_tau_c=None, inner="off"):
-        if ramp:
-            raise NotImplementedError("the second-order ramp placement is not provided")
         if axial_rotation != "off":
             raise NotImplementedError("axial_rotation other than 'off' is not provided")
         if inner not in ("off", "extrap"):
@@ -131,8 +130,8 @@ class SeaLevelSolver:
         self.tpw, self.checkmarine, self.calcRG = bool(tpw), bool(checkmarine), bool(calcRG)
         self.epsilon1 = float(epsilon1)
         self.inner = inner
-        self.horizontal = False
-        self.ramp = False
+        self.horizontal = False  # set True for the horizontal read-out (result key "horiz")
+        self.ramp = bool(ramp)
         self.p = planet or Planet()
         self.nlat, self.nlon = sht.nlat, sht.nlon
         self.dev = dev = sht.dev
@@ -148,9 +147,12 @@ class SeaLevelSolver:
         ekhl = np.zeros(nl)
         ekhl[1:] = 1 + love.ke[1:nl] - love.he[1:nl]
         he = np.concatenate(([0.0], love.he[1:nl]))
+        le = np.concatenate(([0.0], love.le[1:nl]))
+        a_hl, a_thl = horizontal_weights(love, norder)
         self._host = dict(nmod=np.ascontiguousarray(nmod, dtype=np.int32), s=np.ascontiguousarray(s),
                           a_sl=np.ascontiguousarray(a_sl), a_rr=np.ascontiguousarray(a_rr),
-                          a2=np.ascontiguousarray(a2), ttl=ttl, ttlekhl=ttl * ekhl, ttlhe=ttl * he)
+                          a2=np.ascontiguousarray(a2), ttl=ttl, ttlekhl=ttl * ekhl, ttlhe=ttl * he,
+                          a_hl=np.ascontiguousarray(a_hl), a_thl=np.ascontiguousarray(a_thl), ttlle=ttl * le)
         h = self._host
         dp = lambda a: a.ctypes.data_as(_lib.P_double)  # noqa: E731
         prm = _lib.SlxSolverParams(
@@ -161,7 +163,9 @@ class SeaLevelSolver:
             hTE2=float(love.hTE[2]) if norder >= 2 else 0.0, kE2=float(love.ke[2]) if norder >= 2 else 0.0,
             nmod=h["nmod"].ctypes.data_as(_lib.P_int), s=dp(h["s"]), a_sl=dp(h["a_sl"]), a_rr=dp(h["a_rr"]),
             a_il=dp(h["a2"][0]), a_m=dp(h["a2"][1]), a_tsl=dp(h["a2"][2]), a_trr=dp(h["a2"][3]),
-            ttl=dp(h["ttl"]), ttlekhl=dp(h["ttlekhl"]), ttlhe=dp(h["ttlhe"]))
+            ttl=dp(h["ttl"]), ttlekhl=dp(h["ttlekhl"]), ttlhe=dp(h["ttlhe"]), ramp=int(self.ramp),
+            a_hl=dp(h["a_hl"]), a_thl=dp(h["a_thl"]), ttlle=dp(h["ttlle"]),
+            lTE2=float(love.lTE[2]) if norder >= 2 else 0.0)
         g = lambda: torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=dev)  # noqa: E731
         self.topoxy, self.cxy, self.beta, self.rrxy, self.Gxy = g(), g(), g(), g(), g()
         outs = _lib.SlxOutputs(tgrid=self.topoxy.data_ptr(), ocean=self.cxy.data_ptr(), beta=self.beta.data_ptr(),
@@ -200,8 +204,26 @@ class SeaLevelSolver:
         self.il = np.zeros((3, 3))
         self.mm = np.zeros(3)
         self.lam = np.zeros((3, 3), dtype=np.complex128)
+        self._ro_bufs = None
 
     # ---------------------------------------------------------------------------- helpers
+    def _readouts(self, rates, horizontal):
+        """Device buffers of the read-outs (allocated once) and the slx_readouts block."""
+        if self._ro_bufs is None:
+            dev, n = self.dev, self.sht.ncoef
+            g = lambda: torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=dev)  # noqa: E731
+            c = lambda: torch.zeros(n, dtype=torch.complex128, device=dev)  # noqa: E731
+            self._ro_bufs = {"dSLdt": g(), "dRdt": g(), "dSLdt_lm": c(), "dRdt_lm": c(),
+                             "rot": torch.zeros(30, dtype=torch.float64, device=dev), "horiz": c()}
+        b = self._ro_bufs
+        ro = _lib.SlxReadouts()
+        if rates:
+            ro.dSLdt, ro.dRdt = b["dSLdt"].data_ptr(), b["dRdt"].data_ptr()
+            ro.dSLdt_lm, ro.dRdt_lm = b["dSLdt_lm"].data_ptr(), b["dRdt_lm"].data_ptr()
+            ro.rot_rates = b["rot"].data_ptr()
+        if horizontal:
+            ro.horiz_lm = b["horiz"].data_ptr()
+        return ro
     def _status(self):
         st = _lib.SlxStatus()
         _lib.call("slx_solver_status", self._s_handle, C.byref(st), D.stream_ptr())
@@ -224,10 +246,12 @@ class SeaLevelSolver:
 
     # ------------------------------------------------------------------------------- step
     def step(self, icexy_cur, profile=False, dt=None, rates=False):
-        """Advance one time step (solver.py:814-1144); returns the reference's result dict."""
-        if rates:
-            raise NotImplementedError("analytic rates are not provided")
+        """Advance one time step (solver.py:814-1144); returns the reference's result dict.
+        rates: also the analytic rates dSL/dt, dR/dt at the end of the step (grids in m/yr, dense
+        spectra, and with rotation il-dot, m-dot, Lambda-dot; result key "rates")."""
         t_all = time.perf_counter()
+        horizontal = bool(self.horizontal)
+        ro = self._readouts(rates, horizontal) if (rates or horizontal) else None
         self.set_step_dt(dt)
         self.iter += 1
         if isinstance(icexy_cur, torch.Tensor) and icexy_cur.device == self.dev and \
@@ -250,7 +274,10 @@ class SeaLevelSolver:
         while True:
             _lib.call("slx_step_iterate", h, n, sp)
             total += n
-            _lib.call("slx_step_finish", h, sp)
+            if ro is None:
+                _lib.call("slx_step_finish", h, sp)
+            else:
+                _lib.call("slx_step_finish_readouts", h, C.byref(ro), sp)
             st = self._status()
             if st.failed:
                 raise RuntimeError(f"inner loop failed to converge, xi={st.xi}")
@@ -274,6 +301,17 @@ class SeaLevelSolver:
                     "cxy": torch.empty_like(self.cxy)}
             _copy_dev(ent["dS"], self._fields.load_inc, self.sht.ncoef * 16)
             ent["cxy"].copy_(self.cxy)
+        rate_out = hz = None
+        if rates:
+            b = self._ro_bufs
+            rate_out = {"dSLdt": b["dSLdt"], "dRdt": b["dRdt"], "dSLdt_lm": self.sht.to_dense(b["dSLdt_lm"]),
+                        "dRdt_lm": self.sht.to_dense(b["dRdt_lm"])}
+            if self.tpw and self.norder >= 2:
+                r = b["rot"].cpu().numpy()
+                rate_out.update({"ildot": r[:9].reshape(3, 3).copy(), "mdot": r[9:12].copy(),
+                                 "lambdadot": r[12:30:2].reshape(3, 3) + 1j * r[13:30:2].reshape(3, 3)})
+        if horizontal:
+            hz = self.sht.to_dense(self._ro_bufs["horiz"])
         wall = time.perf_counter() - t_all
         if profile:
             self.prof = {"total": wall}
@@ -285,7 +323,7 @@ class SeaLevelSolver:
             "beta": self.beta,
             "R": self.rrxy if self.calcRG else None,
             "G": self.Gxy if self.calcRG else None,
-            "horiz": None,
+            "horiz": hz,
             "dS_converged": self._dS,
             "ocean_area": st.ocean_area,
             "oceanBeta_area": st.oceanBeta_area,
@@ -295,7 +333,7 @@ class SeaLevelSolver:
             "il": self.il.copy(),
             "mm": self.mm.copy(),
             "lambda": self.lam.copy(),
-            "rates": None,
+            "rates": rate_out,
             "wall": wall,
             "prof": self.prof if profile else None,
         }

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_string():
     lib = _lib.load()
-    assert lib.slx_abi_version() == 1
+    assert lib.slx_abi_version() == 2
     assert isinstance(lib.slx_last_error(), bytes)
 
 
