@@ -12,7 +12,10 @@ Restates the reference's exact-memory solver:
   read-out ``horizontal`` (solver.py:687-720) and the analytic rates ``step(rates=True)``
   (solver.py:722-806);
 * ``inner="extrap"``: the extrapolated inner start of ``OptSeaLevelSolver``
-  (``solver_opt.py:148-165,504-527,817-822``).
+  (``solver_opt.py:148-165,504-527,817-822``);
+* ``inner="anderson"``: the same start, then Anderson mixing on the load increment with the ocean
+  mask frozen after ``anderson_freeze`` plain iterations, and at convergence the mask re-checked
+  (at most ``max_mask_refresh`` refreshes) (``solver_opt.py:532-535,705-775``).
 
 Constants are those of ``solver.py:31-39,98-110``.
 """
@@ -22,6 +25,7 @@ import math
 
 import numpy as np
 
+from .anderson import Anderson
 from .memory import ExactMemory
 
 PI = 3.1415926535898
@@ -70,9 +74,15 @@ class SeaLevelSolver:
 
     def __init__(self, sht, love, topo0, ice0, *, norder, dt=1.0, tpw=True, checkmarine=True,
                  calcRG=True, epsilon1=1.0e-5, inner="off", planet=None, reuse="off", reuse_store=None,
-                 ramp=False):
-        if inner not in ("off", "extrap"):
-            raise ValueError("inner must be 'off' or 'extrap'")
+                 ramp=False, anderson_depth=3, max_mask_refresh=3, anderson_freeze=2, anderson_keep=False):
+        if inner not in ("off", "extrap", "anderson"):
+            raise ValueError("inner must be 'off', 'extrap' or 'anderson'")
+        self.aa = Anderson(anderson_depth) if inner == "anderson" else None
+        self.max_mask_refresh = int(max_mask_refresh)
+        self.anderson_freeze = int(anderson_freeze)
+        self.anderson_keep = bool(anderson_keep)
+        self.inner_stats = {"mixed": 0, "plain": 0, "fallback": 0, "singular": 0, "mask_refresh": 0,
+                            "frozen_iters": 0}
         if reuse not in ("off", "warm"):
             raise ValueError("reuse must be 'off' or 'warm'")
         self.reuse = reuse
@@ -264,7 +274,7 @@ class SeaLevelSolver:
         # ---- first guess
         load_inc = (cstar_lm / cstar_lm[0, 0]) * ((-p.rhoi / p.rhow) * dice[0, 0])
         dS_cur = np.zeros_like(load_inc)
-        if self.inner == "extrap" and len(self.hist) >= 2:
+        if self.inner != "off" and len(self.hist) >= 2:
             (h1, dt1), (h2, dt2) = self.hist[0], self.hist[1]
             load_inc = extrap_guess(h1, h2, self.dt, dt1, dt2)
             dS_cur = dS_prev + load_inc
@@ -281,6 +291,10 @@ class SeaLevelSolver:
         rr_rot = np.zeros(3, dtype=np.complex128)
         n_acc = 0
         numiter = 1
+        frozen = refreshing = False
+        n_refresh = 0
+        if self.aa is not None:
+            self.aa.reset()
         while True:
             if self.tpw:
                 # products of inertia from the degree-2 load (solver.py:912-921)
@@ -343,7 +357,37 @@ class SeaLevelSolver:
             deltasl = sht.synthesis(sl_tot)
 
             xi = convergence_ratio(np.abs(load_inc).sum(), np.abs(old).sum())
-            if xi > self.epsilon1:
+            if self.aa is not None:
+                # Anderson mixing with the ocean mask frozen (solver_opt.py:705-775)
+                if xi <= self.epsilon1:
+                    # converged on the frozen mask: accept if this iterate's own mask equals it
+                    same = np.array_equal(self.tinit - deltasl < 0.0, self.cxy != 0.0)
+                    if same or n_refresh >= self.max_mask_refresh:
+                        break
+                    n_refresh += 1
+                    self.inner_stats["mask_refresh"] += 1
+                    refreshing = True
+                elif frozen:
+                    self.aa.push(old, load_inc)
+                    load_inc, kind = self.aa.mix()
+                    self.inner_stats[kind] += 1
+                    self.inner_stats["frozen_iters"] += 1
+                    dS_cur = dS_prev + load_inc
+                    if numiter == MAX_INNER:
+                        raise RuntimeError(f"inner loop failed to converge, xi={xi}")
+                    numiter += 1
+                    continue
+                # plain step (load_inc = G(x)) with the mask update, then freeze
+                if refreshing:
+                    if not self.anderson_keep:
+                        self.aa.reset()
+                    refreshing = False
+                    frozen = True
+                elif numiter >= self.anderson_freeze:
+                    self.aa.reset()
+                    self.aa.push(old, load_inc)
+                    frozen = True
+            if xi > self.epsilon1 or self.aa is not None:
                 topo = self.tinit - deltasl
                 self.cxy = (topo < 0.0).astype(np.float64)
                 if self.checkmarine and n_acc < GR_MAXACC:
@@ -369,7 +413,7 @@ class SeaLevelSolver:
                 raise RuntimeError(f"inner loop failed to converge, xi={xi}")
             numiter += 1
 
-        if self.inner == "extrap":
+        if self.inner != "off":
             self.hist = [(load_inc.copy(), self.dt)] + self.hist[:1]
 
         rrxy = None
