@@ -68,9 +68,9 @@ int slx_modal_advance_read(const slx_sht* h, int kmax, const int* nmod, const do
                            const void* P, void* visc, void* rad, void* stream);
 
 /* ------------------------------------------------------------------- the step ------------
- * Replaces SeaLevelSolver (solver.py:521-1144) and OptSeaLevelSolver with inner = "off" or
- * "extrap" (solver_opt.py:369-979), step or second-order ("ramp") load placement, axial
- * rotation "off".  All state is device-resident; the inner fixed point runs as predicated
+ * Replaces SeaLevelSolver (solver.py:521-1144) and OptSeaLevelSolver with inner = "off",
+ * "extrap" or "anderson" (solver_opt.py:369-979), step or second-order ("ramp") load
+ * placement, axial rotation "off".  All state is device-resident; the inner fixed point runs as predicated
  * launches that stop on a device convergence flag. */
 typedef struct slx_solver_params {
   int lmax, kmax, nmod2;
@@ -90,6 +90,11 @@ typedef struct slx_solver_params {
   const double* a_thl;                              /* host [kmax] tidal l_k/s_k at degree 2 */
   const double* ttlle;                              /* host [lmax+1] T_l l^E_l */
   double lTE2;                                      /* tidal elastic l number at degree 2 */
+  /* inner = "anderson" (solver_opt.py:268-293,705-775; needs extrap = 1): Anderson mixing of
+   * depth 1..5 on the ocean-load increment with the ocean mask frozen after anderson_freeze plain
+   * iterations, at most max_mask_refresh mask refreshes per step, history kept across a refresh
+   * if anderson_keep */
+  int anderson, anderson_depth, anderson_freeze, max_mask_refresh, anderson_keep;
 } slx_solver_params;
 
 typedef struct slx_status {
@@ -99,6 +104,8 @@ typedef struct slx_status {
   double il[9], mm[3], lam[18]; /* lam: 3x3 complex, row-major, (re, im) */
   double ocean_area, oceanBeta_area, gmslc_ocn, gmslc_ocnBeta, ice_volume;
   double ocean_area0, oceanBeta_area0, ice_volume0;
+  /* inner = "anderson" counters over the run (OptSeaLevelSolver.inner_stats) */
+  int aa_mixed, aa_plain, aa_fallback, aa_singular, aa_mask_refresh, aa_frozen_iters;
 } slx_status;
 
 typedef struct slx_fields { /* device pointers owned by the solver */

@@ -22,7 +22,9 @@ class SlxSolverParams(C.Structure):
                 ("nmod", P_int), ("s", P_double), ("a_sl", P_double), ("a_rr", P_double),
                 ("a_il", P_double), ("a_m", P_double), ("a_tsl", P_double), ("a_trr", P_double),
                 ("ttl", P_double), ("ttlekhl", P_double), ("ttlhe", P_double),
-                ("ramp", c_int), ("a_hl", P_double), ("a_thl", P_double), ("ttlle", P_double), ("lTE2", c_double)]
+                ("ramp", c_int), ("a_hl", P_double), ("a_thl", P_double), ("ttlle", P_double), ("lTE2", c_double),
+                ("anderson", c_int), ("anderson_depth", c_int), ("anderson_freeze", c_int),
+                ("max_mask_refresh", c_int), ("anderson_keep", c_int)]
 
 
 class SlxStatus(C.Structure):
@@ -32,7 +34,9 @@ class SlxStatus(C.Structure):
                 ("il", c_double * 9), ("mm", c_double * 3), ("lam", c_double * 18),
                 ("ocean_area", c_double), ("oceanBeta_area", c_double), ("gmslc_ocn", c_double),
                 ("gmslc_ocnBeta", c_double), ("ice_volume", c_double),
-                ("ocean_area0", c_double), ("oceanBeta_area0", c_double), ("ice_volume0", c_double)]
+                ("ocean_area0", c_double), ("oceanBeta_area0", c_double), ("ice_volume0", c_double),
+                ("aa_mixed", c_int), ("aa_plain", c_int), ("aa_fallback", c_int), ("aa_singular", c_int),
+                ("aa_mask_refresh", c_int), ("aa_frozen_iters", c_int)]
 
 
 class SlxFields(C.Structure):

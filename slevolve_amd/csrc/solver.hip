@@ -1,7 +1,7 @@
 // Device-resident sea-level time step (FP64).  Restates python/slevolve/solver.py:521-1144 (step
 // or second-order load placement; the optional rate and horizontal read-outs of solver.py:687-806)
-// and the extrapolated start of solver_opt.py:148-165,504-527 with the grid and spectral algebra
-// fused into the transform passes:
+// and the extrapolated start and Anderson mixing of solver_opt.py:67-165,504-535,705-775 with the
+// grid and spectral algebra fused into the transform passes:
 //   * rO - tO is analysed as one field (its analysis is all the update uses, solver.py:1002-1015);
 //   * the ocean function C* is analysed in the same sweep as rO - tO of the next iteration;
 //   * deltasl = dslxy + conserv K00 (one synthesis less per iteration, solver_opt.py:28-32);
@@ -38,6 +38,15 @@ struct DevState {
   double sum_il[9], sum_m[3], sum_tsl[6], sum_trr[6];
   double dsl_rot[6], rr_rot[6];
   double rate_sl[6], rate_rr[6], hz_rot[6], rate00;  // read-outs of the finishing step
+  // inner = "anderson", per step: mask frozen / being refreshed, refreshes done, pairs stored,
+  // newest slot, accepted grounded re-guesses, C* analysed by the next iteration, this
+  // iteration's re-guess accepted; the iteration index in which each branch runs (tokens)
+  int aa_frozen, aa_refreshing, aa_nrefresh, aa_n, aa_head, aa_nacc, aa_cst, aa_changed;
+  int aa_tok_mask, aa_tok_accept, aa_tok_push, aa_tok_mix;
+  int aa_mdiff[2];   // ocean-mask cells that would change, by iteration parity
+  int aa_kind;       // last mix: 0 mixed, 1 plain, 2 fallback, 3 singular
+  double aa_fn[6];   // residual norm of each history slot
+  double aa_gam[5];
 };
 
 // Constants evaluated on the host once, in the reference's operation order.
@@ -50,6 +59,7 @@ struct Consts {
   int L, nl, ncoef, npts, nlat, nlon, kmax, nmod2, tpw, checkmarine, calcRG;
   int idx20, idx21, idx22;
   int ramp;                                  // second-order load placement
+  int aa, aa_depth, aa_freeze, aa_maxref, aa_keep;  // inner = "anderson"
   double dt;                                 // length of the current step (years)
   double c_il, c_m, c_tsl, c_trr, c_thl;     // sum_k (1 - phi_k) a_k at degree 2 (solver.py:362-367)
 };
@@ -77,6 +87,10 @@ struct slx_solver {
   double *omphi = nullptr, *w_sl = nullptr, *w_rr = nullptr, *a_hl = nullptr, *a_thl = nullptr, *ttlle = nullptr,
          *deg = nullptr;
   int has_horiz = 0;
+  // inner = "anderson": history X, G, F [depth+1][ncoef], this iteration's output G(x) [ncoef],
+  // block partials of the Gram sums [32][PSLOT]
+  double2 *aaX = nullptr, *aaG = nullptr, *aaF = nullptr, *gnew = nullptr;
+  double* aa_part = nullptr;
   std::vector<double> h_a_sl, h_a_rr, h_a2, h_a_hl, h_a_thl, h_ttl;  // host copies for the ramp sums
   std::vector<int> h_nmod;
   // grids
@@ -111,6 +125,8 @@ struct Pts {  // all device pointers a kernel may need, passed by value
   const int* nmod;
   double2* R;
   const double *s, *omphi, *w_sl, *w_rr, *a_hl, *a_thl, *ttlle, *deg;
+  double2 *aaX, *aaG, *aaF, *gnew;
+  double* aa_part;
 };
 
 __device__ __forceinline__ double keep_factor(double ice, double topo, double rhow, double rhoi) {
@@ -282,6 +298,10 @@ __global__ void __launch_bounds__(64) k_begin(Pts p, Consts c) {
     s->nflip[0] = s->nflip[1] = 0;
     s->nacc2[0] = s->nacc2[1] = 0;
     s->counter = 0;
+    s->aa_frozen = s->aa_refreshing = s->aa_nrefresh = s->aa_n = s->aa_nacc = s->aa_cst = s->aa_changed = 0;
+    s->aa_head = 0;
+    s->aa_tok_mask = s->aa_tok_accept = s->aa_tok_push = s->aa_tok_mix = -1;
+    s->aa_mdiff[0] = s->aa_mdiff[1] = 0;
   }
 }
 
@@ -420,15 +440,16 @@ struct LoadPtrs {
 // Ring-pair pass of an inner iteration: the synthesised Fourier coefficients of eq. B18 are
 // inverse-transformed (one complex FFT for rows j and nlat-1-j), written as dslxy, multiplied by
 // C* and reduced by tO in LDS (rO - tO, solver.py:993-996), forward-transformed and folded into
-// analysis field 0; with nf == 2 the ocean function C* of the same rows is transformed into
-// analysis field 1 as well.  Replaces separate synthesis and analysis ring passes.
+// analysis field 0; with nf == 2 (or a set cst_flag) the ocean function C* of the same rows is
+// transformed into analysis field 1 as well.  Replaces separate synthesis and analysis ring passes.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __restrict__ FN,
                                                    const double2* __restrict__ FS, double* __restrict__ dsl,
                                                    const double* __restrict__ cstar, const double* __restrict__ tO,
                                                    double2* __restrict__ WS, double2* __restrict__ WD, int nf,
-                                                   Pred pr) {
+                                                   const int* cst_flag, Pred pr) {
   if (!pr.active()) return;
+  if (cst_flag) nf = *(volatile const int*)cst_flag ? 2 : 1;  // C* analysed only after a mask update
   extern __shared__ double2 smem[];
   const int N = d.nlon, L = d.lmax;
   double2* a = smem;
@@ -489,7 +510,9 @@ __device__ __forceinline__ double conserv_of(const Pts& p, const Consts& c) {
   return (1.0 / p.cstar_lm[0].x) * (-(c.rhoi / c.rhow) * p.dI[0].x - p.rt_lm[0].x);
 }
 
-__global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it) {
+// out = load_inc, or with Anderson mixing the separate output buffer G(x) (the input x is kept for
+// the mixing history)
+__global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it, double2* out) {
   __shared__ double sh[256];
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
@@ -499,7 +522,7 @@ __global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double2 sp = p.p_ocean[i], rt = p.rt_lm[i], cs = p.cstar_lm[i];
     const double2 nw = make_double2((-sp.x + rt.x) + cs.x * conserv, (-sp.y + rt.y) + cs.y * conserv);
-    p.load_inc[i] = nw;
+    out[i] = nw;
     p.dS_cur[i] = make_double2(sp.x + nw.x, sp.y + nw.y);
     a += cabs_d(nw);
   }
@@ -512,14 +535,10 @@ __global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it) {
 // Every block first forms xi = |sum|new| - sum|old|| / sum|old| (solver.py:1023-1030) from the
 // update's block partials in a fixed order (sum|old| = the previous iteration's partials, or the
 // guess's), so that all blocks take the same decision; block 0 publishes it.
-__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nbu, int nbg) {
-  __shared__ double sh[256];
-  __shared__ int shf;
-  DevState* s = p.st;
-  if (*(volatile int*)&s->st.done) return;
+__device__ double convergence_xi(const Pts& p, int it, int nbu, int nbg, double* sh, double& sa, double& so) {
   double v = 0.0;
   for (int b = threadIdx.x; b < nbu; b += blockDim.x) v += p.partial[(it & 1) * PSLOT + b];
-  const double sa = block_sum(v, sh);
+  sa = block_sum(v, sh);
   __syncthreads();
   v = 0.0;
   if (it == 0) {
@@ -527,14 +546,19 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
   } else {
     for (int b = threadIdx.x; b < nbu; b += blockDim.x) v += p.partial[((it - 1) & 1) * PSLOT + b];
   }
-  const double so = block_sum(v, sh);
-  double xi;
-  if (fabs(sa - so) < EPS0 && fabs(so) < EPS0)
-    xi = 0.0;
-  else if (fabs(so) < EPS0)
-    xi = fabs((sa - so) / (EPS0 * 10));
-  else
-    xi = fabs((sa - so) / so);
+  so = block_sum(v, sh);
+  if (fabs(sa - so) < EPS0 && fabs(so) < EPS0) return 0.0;
+  if (fabs(so) < EPS0) return fabs((sa - so) / (EPS0 * 10));
+  return fabs((sa - so) / so);
+}
+
+__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nbu, int nbg) {
+  __shared__ double sh[256];
+  __shared__ int shf;
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done) return;
+  double sa, so;
+  const double xi = convergence_xi(p, it, nbu, nbg, sh, sa, so);
   const double conserv = conserv_of(p, c);
   const bool conv = xi <= c.eps1;
   const bool fail = !conv && it + 1 >= MAX_INNER;
@@ -599,6 +623,316 @@ __global__ void k_refresh2(Pts p, Consts c, int it) {
       s->st.nacc += 1;
     }
   }
+}
+
+// ---------------------------------------------------------------------- Anderson mixing
+// Inner iteration with inner = "anderson" (solver_opt.py:705-775): xi as above in every block;
+// when the iteration has converged, or the mask is not frozen, count the ocean-mask cells the
+// topography iterate would change and the grounded flips on it (no grid is written here).
+__global__ void __launch_bounds__(256) k_refresh_aa(Pts p, Consts c, int it, int nbu, int nbg) {
+  __shared__ double sh[256];
+  __shared__ int shm, shf;
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done) return;
+  double sa, so;
+  const double xi = convergence_xi(p, it, nbu, nbg, sh, sa, so);
+  const double conserv = conserv_of(p, c);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s->st.numiter = it + 1;
+    s->st.xi = xi;
+    s->st.conserv = conserv;
+    s->sa = sa;
+    s->so = so;
+  }
+  const bool conv = xi <= c.eps1;
+  if (!conv && s->aa_frozen) return;
+  if (threadIdx.x == 0) shm = shf = 0;
+  __syncthreads();
+  const double cadd = conserv * c.K00;
+  const bool regs = c.checkmarine && s->aa_nacc < GR_MAXACC;
+  int md = 0, flips = 0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < c.npts; i += (long)gridDim.x * blockDim.x) {
+    const double t = p.tinit[i] - (p.dsl[i] + cadd);
+    if ((t < 0.0) != (p.cxy[i] != 0.0)) md += 1;
+    if (regs) {
+      const double ice = p.ice_cur[i];
+      if ((ice * keep_factor(ice, t, c.rhow, c.rhoi) < EPS0) != (p.beta[i] != 0.0)) flips += 1;
+    }
+  }
+  if (md) atomicAdd(&shm, md);
+  if (flips) atomicAdd(&shf, flips);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (shm) atomicAdd(&s->aa_mdiff[it & 1], shm);
+    if (shf) atomicAdd(&s->nflip[it & 1], shf);
+  }
+}
+
+// The branch of the iteration (one lane): converged with an unchanged mask (or refreshes spent)
+// -> done; converged otherwise -> refresh the mask; frozen -> mix; else a plain step with the mask
+// update, freezing the mask from iteration anderson_freeze on (the history restarts with this
+// step's pair).  Sets the tokens of the kernels that follow and the grounded re-guess bookkeeping.
+__global__ void k_decide_aa(Pts p, Consts c, int it) {
+  DevState* s = p.st;
+  if (threadIdx.x != 0 || blockIdx.x != 0 || *(volatile int*)&s->st.done) return;
+  const int slot = it & 1;
+  const bool conv = s->st.xi <= c.eps1;
+  bool mask = false, mix = false, push = false, accept = false;
+  if (conv) {
+    if (s->aa_mdiff[slot] == 0 || s->aa_nrefresh >= c.aa_maxref) {
+      s->st.done = 1;
+      accept = true;
+    } else {
+      s->aa_nrefresh += 1;
+      s->st.aa_mask_refresh += 1;
+      s->aa_refreshing = 1;
+    }
+  } else if (s->aa_frozen) {
+    mix = push = true;
+    s->st.aa_frozen_iters += 1;
+  }
+  if (!s->st.done && !mix) {
+    if (s->aa_refreshing) {
+      if (!c.aa_keep) s->aa_n = 0;
+      s->aa_refreshing = 0;
+      s->aa_frozen = 1;
+    } else if (it + 1 >= c.aa_freeze) {
+      s->aa_n = 0;
+      push = true;
+      s->aa_frozen = 1;
+    }
+    mask = accept = true;
+  }
+  if (!s->st.done && it + 1 >= MAX_INNER) {
+    s->st.failed = 1;
+    s->st.done = 1;
+    mask = mix = push = accept = false;
+  }
+  if (push) {
+    s->aa_head = (s->aa_head + 1) % (c.aa_depth + 1);
+    s->aa_n = min(s->aa_n + 1, c.aa_depth + 1);
+  }
+  bool changed = false;
+  if (mask) {
+    const bool regs = c.checkmarine && s->aa_nacc < GR_MAXACC;
+    changed = regs && s->nflip[slot] > 0;
+    if (regs) s->st.reguess += 1;
+    if (changed) {
+      s->aa_nacc += 1;
+      s->st.ice_analyses += 1;
+      s->st.nacc += 1;
+    }
+  }
+  s->aa_changed = changed ? 1 : 0;
+  s->ice_changed = changed ? 1 : 0;
+  s->aa_cst = mask ? 1 : 0;
+  s->aa_tok_mask = mask ? it : -1;
+  s->aa_tok_accept = accept ? it : -1;
+  s->aa_tok_push = push ? it : -1;
+  s->aa_tok_mix = mix ? it : -1;
+  s->aa_mdiff[slot ^ 1] = 0;
+  s->nflip[slot ^ 1] = 0;
+}
+
+// mask update of a plain / refresh step: topography, ocean function, grounded re-guess, C*, tO
+__global__ void k_refresh2_aa(Pts p, Consts c, int it) {
+  DevState* s = p.st;
+  if (*(volatile int*)&s->aa_tok_mask != it) return;
+  const bool changed = s->aa_changed;
+  const double cadd = s->st.conserv * c.K00;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < c.npts; i += (long)gridDim.x * blockDim.x) {
+    const double t = p.tinit[i] - (p.dsl[i] + cadd);
+    const double cx = t < 0.0 ? 1.0 : 0.0;
+    p.topo[i] = t;
+    p.cxy[i] = cx;
+    if (changed) {
+      const double ice = p.ice_cur[i];
+      const double cand = ice * keep_factor(ice, t, c.rhow, c.rhoi);
+      p.icestar[i] = cand;
+      p.beta[i] = cand < EPS0 ? 1.0 : 0.0;
+    }
+    const double cs = cx * p.beta[i];
+    p.cstar[i] = cs;
+    p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
+  }
+}
+
+// Store the pair (x = load_inc, g = G(x)) in the newest history slot with f = g - x, and form the
+// block partials of f.f, dF_j . f and dF_j . dF_k (dF_j = f_j - f_{j+1}, newest first; real inner
+// product over real and imaginary parts).  Partial q of block b at aa_part[q * PSLOT + b]:
+// q = 0 f.f, 1..m dF.f, then the upper triangle of dF.dF row by row.
+__global__ void __launch_bounds__(256) k_aa_push(Pts p, Consts c, int it) {
+  __shared__ double sh[256];
+  DevState* s = p.st;
+  if (*(volatile int*)&s->aa_tok_push != it) return;
+  const int D1 = c.aa_depth + 1, h = s->aa_head, n = s->aa_n, m = n - 1;
+  const long nc = c.ncoef;
+  int sl[6];
+  for (int j = 0; j < 6; ++j) sl[j] = ((h - j) % D1 + D1) % D1;
+  double acc[21];
+  for (int q = 0; q < 21; ++q) acc[q] = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nc; i += (long)gridDim.x * blockDim.x) {
+    const double2 x = p.load_inc[i], g = p.gnew[i];
+    const double2 f = make_double2(g.x - x.x, g.y - x.y);
+    p.aaX[h * nc + i] = x;
+    p.aaG[h * nc + i] = g;
+    p.aaF[h * nc + i] = f;
+    acc[0] += f.x * f.x + f.y * f.y;
+    double2 df[5];
+    double2 fa = f;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      if (j < m) {
+        const double2 fb = p.aaF[sl[j + 1] * nc + i];
+        df[j] = make_double2(fa.x - fb.x, fa.y - fb.y);
+        fa = fb;
+        acc[1 + j] += df[j].x * f.x + df[j].y * f.y;
+      }
+    }
+    int q = 6;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int k = j; k < 5; ++k, ++q)
+        if (k < m) acc[q] += df[j].x * df[k].x + df[j].y * df[k].y;
+  }
+  for (int q = 0; q < 21; ++q) {
+    const double v = block_sum(acc[q], sh);
+    if (threadIdx.x == 0) p.aa_part[q * PSLOT + blockIdx.x] = v;
+    __syncthreads();
+  }
+}
+
+// Least squares of the mixing step (one block): the partials reduced in a fixed order; the
+// residual norm of the new pair; with a mix, the fallback / plain rules and
+// gamma = pinv(A) rhs of the normal equations A = dF.dF, rhs = dF.f (cyclic Jacobi
+// eigendecomposition of the symmetric A, eigenvalues of magnitude <= 1e-12 of the largest
+// dropped, as a least-squares solve with rcond 1e-12; a non-finite gamma is "singular").
+__global__ void __launch_bounds__(256) k_aa_solve(Pts p, Consts c, int it, int nb) {
+  __shared__ double sh[256];
+  __shared__ double tot[21];
+  DevState* s = p.st;
+  if (*(volatile int*)&s->aa_tok_push != it) return;
+  for (int q = 0; q < 21; ++q) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += p.aa_part[q * PSLOT + b];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) tot[q] = v;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const int D1 = c.aa_depth + 1, h = s->aa_head, n = s->aa_n, m = n - 1;
+  const double fn = sqrt(tot[0]);
+  const double fprev = s->aa_fn[(h - 1 + D1) % D1];
+  s->aa_fn[h] = fn;
+  if (s->aa_tok_mix != it) return;
+  int kind;
+  if (n >= 2 && fn > fprev) {
+    kind = 2;
+    s->aa_n = 1;  // the history restarts from the newest pair
+  } else if (n < 2) {
+    kind = 1;
+  } else {
+    double a[5][5], v[5][5], rhs[5];
+    int q = 6;
+    for (int j = 0; j < 5; ++j)
+      for (int k = j; k < 5; ++k, ++q)
+        if (k < m) a[j][k] = a[k][j] = tot[q];
+    for (int j = 0; j < m; ++j) {
+      rhs[j] = tot[1 + j];
+      for (int k = 0; k < m; ++k) v[j][k] = j == k ? 1.0 : 0.0;
+    }
+    for (int sweep = 0; sweep < 64; ++sweep) {
+      double off = 0.0;
+      for (int j = 0; j < m; ++j)
+        for (int k = j + 1; k < m; ++k) off += a[j][k] * a[j][k];
+      if (!(off > 0.0)) break;
+      for (int j = 0; j < m; ++j)
+        for (int k = j + 1; k < m; ++k) {
+          if (a[j][k] == 0.0) continue;
+          const double th = (a[k][k] - a[j][j]) / (2.0 * a[j][k]);
+          const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+          const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+          for (int r = 0; r < m; ++r) {  // columns j, k
+            const double arj = a[r][j], ark = a[r][k];
+            a[r][j] = cs * arj - sn * ark;
+            a[r][k] = sn * arj + cs * ark;
+          }
+          for (int r = 0; r < m; ++r) {  // rows j, k
+            const double ajr = a[j][r], akr = a[k][r];
+            a[j][r] = cs * ajr - sn * akr;
+            a[k][r] = sn * ajr + cs * akr;
+          }
+          for (int r = 0; r < m; ++r) {
+            const double vrj = v[r][j], vrk = v[r][k];
+            v[r][j] = cs * vrj - sn * vrk;
+            v[r][k] = sn * vrj + cs * vrk;
+          }
+        }
+    }
+    double lmax = 0.0;
+    for (int j = 0; j < m; ++j) lmax = fmax(lmax, fabs(a[j][j]));
+    double gam[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < m; ++j) {
+      const double lam = a[j][j];
+      if (!(fabs(lam) > 1e-12 * lmax)) continue;
+      double proj = 0.0;
+      for (int r = 0; r < m; ++r) proj += v[r][j] * rhs[r];
+      for (int r = 0; r < m; ++r) gam[r] += v[r][j] * (proj / lam);
+    }
+    bool finite = true;
+    for (int j = 0; j < m; ++j) finite = finite && isfinite(gam[j]);
+    for (int j = 0; j < m; ++j) finite = finite && isfinite(a[j][j]);
+    kind = finite ? 0 : 3;
+    for (int j = 0; j < 5; ++j) s->aa_gam[j] = finite && j < m ? gam[j] : 0.0;
+  }
+  s->aa_kind = kind;
+  if (kind == 0) s->st.aa_mixed += 1;
+  if (kind == 1) s->st.aa_plain += 1;
+  if (kind == 2) s->st.aa_fallback += 1;
+  if (kind == 3) s->st.aa_singular += 1;
+}
+
+// next input x = g_0 - sum_j gamma_j (g_j - g_{j+1}) (g_0 alone unless "mixed"); deltaS_cur; the
+// block partials of |x| for the next convergence ratio
+__global__ void __launch_bounds__(256) k_aa_mix(Pts p, Consts c, int it) {
+#pragma clang fp contract(off)
+  __shared__ double sh[256];
+  DevState* s = p.st;
+  if (*(volatile int*)&s->aa_tok_mix != it) return;
+  const int D1 = c.aa_depth + 1, h = s->aa_head, m = s->aa_n - 1, kind = s->aa_kind;
+  const long nc = c.ncoef;
+  int sl[6];
+  for (int j = 0; j < 6; ++j) sl[j] = ((h - j) % D1 + D1) % D1;
+  double gam[5];
+  for (int j = 0; j < 5; ++j) gam[j] = s->aa_gam[j];
+  double a = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nc; i += (long)gridDim.x * blockDim.x) {
+    double2 g = p.aaG[h * nc + i];
+    double2 out = g;
+    if (kind == 0) {
+      for (int j = 0; j < m; ++j) {
+        const double2 gb = p.aaG[sl[j + 1] * nc + i];
+        const double2 dg = make_double2(g.x - gb.x, g.y - gb.y);
+        out.x -= gam[j] * dg.x;
+        out.y -= gam[j] * dg.y;
+        g = gb;
+      }
+    }
+    p.load_inc[i] = out;
+    const double2 sp = p.p_ocean[i];
+    p.dS_cur[i] = make_double2(sp.x + out.x, sp.y + out.y);
+    a += cabs_d(out);
+  }
+  const double bs = block_sum(a, sh);
+  if (threadIdx.x == 0) p.partial[(it & 1) * PSLOT + blockIdx.x] = bs;
+}
+
+// a plain or final step takes the output as the next input (deltaS_cur and |x| are already those)
+__global__ void k_aa_plain(Pts p, Consts c, int it) {
+  if (*(volatile int*)&p.st->aa_tok_accept != it) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < c.ncoef; i += (long)gridDim.x * blockDim.x)
+    p.load_inc[i] = p.gnew[i];
 }
 
 __global__ void k_dice(Pts p, Consts c) {
@@ -963,6 +1297,11 @@ static Pts make_pts(slx_solver* s, const double* ice_cur, const double2* oldice)
   p.a_thl = s->a_thl;
   p.ttlle = s->ttlle;
   p.deg = s->deg;
+  p.aaX = s->aaX;
+  p.aaG = s->aaG;
+  p.aaF = s->aaF;
+  p.gnew = s->gnew;
+  p.aa_part = s->aa_part;
   return p;
 }
 
@@ -1013,6 +1352,14 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   c.ramp = p->ramp ? 1 : 0;
   c.dt = -1.0;
   c.c_il = c.c_m = c.c_tsl = c.c_trr = c.c_thl = 0.0;
+  c.aa = p->anderson ? 1 : 0;
+  c.aa_depth = p->anderson_depth;
+  c.aa_freeze = p->anderson_freeze;
+  c.aa_maxref = p->max_mask_refresh;
+  c.aa_keep = p->anderson_keep ? 1 : 0;
+  SLX_CHECK(!c.aa || (p->extrap && p->anderson_depth >= 1 && p->anderson_depth <= 5 && p->anderson_freeze >= 1 &&
+                      p->max_mask_refresh >= 0),
+            "anderson needs extrap = 1, depth 1..5, freeze >= 1, max_mask_refresh >= 0");
   c.L = L;
   c.nl = nl;
   c.ncoef = h->ncoef;
@@ -1120,6 +1467,16 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
     SLX_HIP(hipMemset(*q, 0, S));
   }
   SLX_HIP(hipMalloc(&s->scratch_lm, 3 * S));
+  if (c.aa) {
+    const size_t H = (size_t)(c.aa_depth + 1) * S;
+    for (double2** q : {&s->aaX, &s->aaG, &s->aaF}) {
+      SLX_HIP(hipMalloc(q, H));
+      SLX_HIP(hipMemset(*q, 0, H));
+    }
+    SLX_HIP(hipMalloc(&s->gnew, S));
+    SLX_HIP(hipMemset(s->gnew, 0, S));
+    SLX_HIP(hipMalloc(&s->aa_part, 32 * (size_t)PSLOT * sizeof(double)));
+  }
   SLX_HIP(hipMalloc(&s->R, (size_t)std::max(K, 1) * S));
   SLX_HIP(hipMemset(s->R, 0, (size_t)std::max(K, 1) * S));
   SLX_HIP(hipHostMalloc(&s->pinned, sizeof(slx_status), hipHostMallocDefault));
@@ -1131,7 +1488,7 @@ extern "C" int slx_solver_destroy(slx_solver* s) {
   if (!s) return 0;
   void* ptrs[] = {s->st, s->partial, s->rows, s->w00, s->nmod, s->s, s->decay, s->omd, s->a_sl, s->a_rr, s->a2,
                   s->r2, s->ttl, s->ttlekhl, s->ttlhe, s->omphi, s->w_sl, s->w_rr, s->a_hl, s->a_thl, s->ttlle,
-                  s->deg, s->tinit, s->cxy0, s->beta0, s->cstar0,
+                  s->deg, s->aaX, s->aaG, s->aaF, s->gnew, s->aa_part, s->tinit, s->cxy0, s->beta0, s->cstar0,
                   s->cstar, s->tO, s->icestar, s->dsl, s->topo_prev, s->ice_prev, s->tmp,
                   s->p_load, s->p_ice, s->p_ocean, s->prev_ice, s->icestar_lm, s->oldice_lm, s->dice, s->dI,
                   s->cstar_lm, s->rt_lm, s->load_inc, s->dS_cur, s->visc, s->rad, s->h1, s->h2, s->scratch_lm,
@@ -1281,6 +1638,63 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
   return 0;
 }
 
+// n inner iterations with Anderson mixing (solver_opt.py:532-535,705-775): the same ring and
+// Legendre passes (C* analysed only after a mask update), then the decision lane and the
+// kernels of the chosen branch, each gated by its iteration token.
+static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
+  const Consts& c = s->c;
+  const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
+  Pts p = make_pts(s, s->ice_cur, oldice);
+  const int* done = &s->st->st.done;
+  const int* changed = &s->st->ice_changed;
+  const int* cst = &s->st->aa_cst;
+  const int nsb = (c.ncoef + 255) / 256, nrb = red_blocks(c.ncoef);
+  for (int it = 0; it < n; ++it) {
+    const int it_step = s->iters_enqueued;
+    s->iters_enqueued += 1;
+    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->dice, s->deg, s->st, c, 1};
+    int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
+    if (rc) return rc;
+    const size_t lds = (2 * (size_t)c.nlon + c.nlon / 4) * sizeof(double2);
+    static size_t lds_allowed = 65536;
+    if (lds > lds_allowed) {
+      SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_synana<256>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      lds_allowed = lds;
+    }
+    k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl, s->cstar, s->tO,
+                                                   s->h->WS, s->h->WD, 1, cst, Pred{done, 0});
+    SLX_LAUNCH_CHECK();
+    rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0, cst, 1}, st);
+    if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0, cst, 0}, st);
+    if (rc) return rc;
+    k_update<<<nrb, 256, 0, st>>>(p, c, it_step, s->gnew);
+    SLX_LAUNCH_CHECK();
+    k_refresh_aa<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, nrb, nrb);
+    SLX_LAUNCH_CHECK();
+    k_decide_aa<<<1, 64, 0, st>>>(p, c, it_step);
+    SLX_LAUNCH_CHECK();
+    k_refresh2_aa<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step);
+    SLX_LAUNCH_CHECK();
+    if (c.checkmarine) {
+      rc = launch_fft_ana<256>(s->h, LoadPtrs<1>{{s->icestar}}, 1, Pred{done, 0, changed, 1}, st);
+      if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
+      if (rc) return rc;
+      k_dice<<<nsb, 256, 0, st>>>(p, c);
+      SLX_LAUNCH_CHECK();
+    }
+    k_aa_push<<<nrb, 256, 0, st>>>(p, c, it_step);
+    SLX_LAUNCH_CHECK();
+    k_aa_solve<<<1, 256, 0, st>>>(p, c, it_step, nrb);
+    SLX_LAUNCH_CHECK();
+    k_aa_mix<<<nrb, 256, 0, st>>>(p, c, it_step);
+    SLX_LAUNCH_CHECK();
+    k_aa_plain<<<grid_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step);
+    SLX_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
 extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
   SLX_CHECK(s != nullptr, "null solver");
   hipStream_t st = (hipStream_t)stream;
@@ -1290,6 +1704,7 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
   const int* done = &s->st->st.done;
   const int* changed = &s->st->ice_changed;
   const int nsb = (c.ncoef + 255) / 256;
+  if (c.aa) return iterate_anderson(s, n, st);
   for (int it = 0; it < n; ++it) {
     const bool first = (s->iters_enqueued == 0);
     const int it_step = s->iters_enqueued;
@@ -1307,7 +1722,7 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
         lds_allowed = lds;
       }
       k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl, s->cstar, s->tO,
-                                                     s->h->WS, s->h->WD, nf, Pred{done, 0});
+                                                     s->h->WS, s->h->WD, nf, nullptr, Pred{done, 0});
       SLX_LAUNCH_CHECK();
     }
     if (first)
@@ -1315,7 +1730,7 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     else
       rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
     if (rc) return rc;
-    k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step);
+    k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step, s->load_inc);
     SLX_LAUNCH_CHECK();
     k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
     SLX_LAUNCH_CHECK();

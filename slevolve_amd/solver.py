@@ -8,11 +8,12 @@ reads one small status block back, and returns the reference's result dictionary
 it are device tensors that the solver overwrites at the next step (the reference also returns
 its reused buffers).
 
-Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap", the
+Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap" |
+"anderson" (Anderson mixing with the frozen ocean mask, OptSeaLevelSolver), the
 second-order load placement (ramp), the analytic rates (step(rates=True)), the horizontal read-out
 (solver.horizontal = True, result key "horiz") and the "warm" cross-pass reuse of OptSeaLevelSolver
-(per-step converged states kept on the device).  Axial-rotation variants, Anderson mixing and the
-other reuse modes of the reference are not provided and raise.
+(per-step converged states kept on the device).  Axial-rotation variants and the other reuse
+modes of the reference are not provided and raise.
 """
 from __future__ import annotations
 
@@ -113,11 +114,11 @@ class SeaLevelSolver:
 
     def __init__(self, sht, love, topo0, ice0, *, norder=512, dt=1, tpw=True, checkmarine=True, calcRG=True,
                  epsilon1=1.0e-5, planet=None, mem_threads=1, ramp=False, axial_rotation="off",
-                 axial_tau_c=None, inner="off"):
+                 axial_tau_c=None, inner="off", _anderson=(3, 2, 3, False)):
         if axial_rotation != "off":
             raise NotImplementedError("axial_rotation other than 'off' is not provided")
-        if inner not in ("off", "extrap"):
-            raise NotImplementedError(f"inner={inner!r} is not provided (off | extrap)")
+        if inner not in INNER_MODES:
+            raise ValueError(f"inner must be one of {INNER_MODES}, got {inner!r}")
         if not isinstance(sht, SHT):
             raise TypeError("sht must be a slevolve_amd.SHT (the GPU transform)")
         if sht.lmax != norder:
@@ -157,7 +158,7 @@ class SeaLevelSolver:
         dp = lambda a: a.ctypes.data_as(_lib.P_double)  # noqa: E731
         prm = _lib.SlxSolverParams(
             lmax=self.norder, kmax=K, nmod2=nmod2, tpw=int(self.tpw), checkmarine=int(self.checkmarine),
-            calcRG=int(self.calcRG), extrap=int(inner == "extrap"), epsilon1=self.epsilon1,
+            calcRG=int(self.calcRG), extrap=int(inner != "off"), epsilon1=self.epsilon1,
             radius=p.radius, mass=p.mass, rhoi=p.rhoi, rhow=p.rhow, gacc=p.gacc, omega=p.omega, moiA=p.moiA,
             moiC=p.moiC, kf=p.kf, kTE2=float(love.kTE[2]) if norder >= 2 else 0.0,
             hTE2=float(love.hTE[2]) if norder >= 2 else 0.0, kE2=float(love.ke[2]) if norder >= 2 else 0.0,
@@ -165,7 +166,9 @@ class SeaLevelSolver:
             a_il=dp(h["a2"][0]), a_m=dp(h["a2"][1]), a_tsl=dp(h["a2"][2]), a_trr=dp(h["a2"][3]),
             ttl=dp(h["ttl"]), ttlekhl=dp(h["ttlekhl"]), ttlhe=dp(h["ttlhe"]), ramp=int(self.ramp),
             a_hl=dp(h["a_hl"]), a_thl=dp(h["a_thl"]), ttlle=dp(h["ttlle"]),
-            lTE2=float(love.lTE[2]) if norder >= 2 else 0.0)
+            lTE2=float(love.lTE[2]) if norder >= 2 else 0.0, anderson=int(inner == "anderson"),
+            anderson_depth=int(_anderson[0]), anderson_freeze=int(_anderson[1]), max_mask_refresh=int(_anderson[2]),
+            anderson_keep=int(bool(_anderson[3])))
         g = lambda: torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=dev)  # noqa: E731
         self.topoxy, self.cxy, self.beta, self.rrxy, self.Gxy = g(), g(), g(), g(), g()
         outs = _lib.SlxOutputs(tgrid=self.topoxy.data_ptr(), ocean=self.cxy.data_ptr(), beta=self.beta.data_ptr(),
@@ -285,6 +288,7 @@ class SeaLevelSolver:
                 break
             n = 1 if total < 16 else max(1, total // 2)
         self._guess_iters = max(1, st.numiter)
+        self._last_status = st
         self._dt_hist = [self.dt, self._dt_hist[0]]
         self.gr_counts = {"reguess": st.reguess, "ice_analyses": st.ice_analyses}
         self.il = np.array(st.il).reshape(3, 3)
@@ -341,23 +345,39 @@ class SeaLevelSolver:
 
 class OptSeaLevelSolver(SeaLevelSolver):
     """``OptSeaLevelSolver`` signature (solver_opt.py:268-296).  The optimisation switches ``opt``
-    are accepted (the device path always uses the fused forms); ``inner`` may be "off" or
-    "extrap"."""
+    are accepted (the device path always uses the fused forms); ``inner`` may be "off", "extrap"
+    or "anderson" (with anderson_depth 1..5, anderson_freeze, max_mask_refresh, anderson_keep;
+    ``inner_stats`` counts the mixing outcomes over the run)."""
 
     def __init__(self, *args, opt=(), inner="off", anderson_depth=3, max_mask_refresh=3, anderson_freeze=2,
                  anderson_keep=False, reuse="off", reuse_store=None, **kw):
-        if inner == "anderson":
-            raise NotImplementedError("Anderson mixing is not provided")
+        if inner not in INNER_MODES:
+            raise ValueError(f"inner must be one of {INNER_MODES}, got {inner!r}")
         if reuse not in ("off", "warm"):
             raise NotImplementedError(f"reuse={reuse!r} is not provided (off | warm)")
         self.opt = set(opt)
-        super().__init__(*args, inner=inner, **kw)
+        self.anderson_depth, self.anderson_freeze = int(anderson_depth), int(anderson_freeze)
+        self.max_mask_refresh, self.anderson_keep = int(max_mask_refresh), bool(anderson_keep)
+        super().__init__(*args, inner=inner, _anderson=(anderson_depth, anderson_freeze, max_mask_refresh,
+                                                        anderson_keep), **kw)
         # cross-pass reuse (solver_opt.py:272-286): the caller-owned store keeps, per step, the
         # converged load increment (packed, device) and ocean function (device grid)
         self._reuse = reuse
         self._reuse_store = reuse_store if reuse_store is not None else {}
         self.reuse_store = self._reuse_store
-        self.inner_stats = {}
+
+    def step(self, icexy_cur, profile=False, dt=None, rates=False):
+        r = super().step(icexy_cur, profile=profile, dt=dt, rates=rates)
+        r["inner_stats"] = dict(self.inner_stats)
+        return r
+
+    @property
+    def inner_stats(self):
+        st = getattr(self, "_last_status", None)
+        if st is None:
+            return {"mixed": 0, "plain": 0, "fallback": 0, "singular": 0, "mask_refresh": 0, "frozen_iters": 0}
+        return {"mixed": st.aa_mixed, "plain": st.aa_plain, "fallback": st.aa_fallback, "singular": st.aa_singular,
+                "mask_refresh": st.aa_mask_refresh, "frozen_iters": st.aa_frozen_iters}
 
 
 __all__ = ["SeaLevelSolver", "OptSeaLevelSolver", "Planet", "PI", "EPS0", "GR_MAXACC", "parse_opts"]
