@@ -5,8 +5,9 @@ Workload (default "lmax512"): Gauss-Legendre 512 x 1024 grid, lmax 512, 30 visco
 degree, 550 steps of 1 yr (seeded synthetic Love numbers, topography to degree 256, a polar cap and
 a retreating marine sector), FP64, extrapolated inner start, rotation and marine check on,
 epsilon1 = 1e-5.  A "step" is one solver step (modal advance, inner fixed point to convergence,
-radial displacement, commit).  All ice grids are uploaded before the timed region; W warm-up steps,
-then K timed steps bracketed by device synchronisation.  Prints one JSON line.
+radial displacement, commit).  All ice grids are uploaded before the timed region; W warm-up steps
+(SeaLevelSolver.step), then K timed steps through the native step loop (SeaLevelSolver.run)
+bracketed by device synchronisation.  Prints one JSON line.
 
 Also reported: the roofline of the dominant kernel (the per-order Legendre analysis contraction),
 timed live with HIP events on the stream it is launched on, with the HBM traffic per launch from
@@ -113,12 +114,11 @@ def run_workload(w, warmup, steps, dev, inner="extrap"):
     for k in range(1, warmup + 1):
         slv.step(ice_dev[k])
     torch.cuda.synchronize()
-    numiter = []
     t0 = time.perf_counter()
-    for k in range(warmup + 1, nsteps + 1):
-        numiter.append(slv.step(ice_dev[k])["numiter"])
+    rec = slv.run(ice_dev, warmup + 1, nsteps + 1)  # the native step loop
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    numiter = [r["numiter"] for r in rec]
     print(f"[bench] {w.name}: {steps} steps in {el:.3f} s, numiter mean {np.mean(numiter):.2f}", file=sys.stderr,
           flush=True)
     del slv, ice_dev

@@ -167,6 +167,14 @@ typedef struct slx_readouts {
 int slx_step_finish_readouts(slx_solver* s, const slx_readouts* r, void* stream);
 /* Synchronous: copies the status block to *out (host) after the stream's pending work. */
 int slx_solver_status(slx_solver* s, slx_status* out, void* stream);
+/* The step loop of run_ais.py:389-425 (SeaLevelSolver.step called per ice file) in native code:
+ * steps k = 0..nsteps-1 with ice grid ice + k * ice_stride (device) and length dts[k] (host),
+ * dt_prev = the lengths of the two steps before the first (host), *guess = inner iterations
+ * enqueued in the first batch (updated to the last step's count); out[k] (host) = the status of
+ * step k after its finish.  Outputs of intermediate steps are overwritten; the solver's grids
+ * hold the last step's. */
+int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, int nsteps, const double* dts,
+                  const double* dt_prev, int* guess, slx_status* out, void* stream);
 int slx_solver_fields(slx_solver* s, slx_fields* out);
 
 #ifdef __cplusplus

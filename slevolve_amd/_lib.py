@@ -76,6 +76,8 @@ SIGNATURES = {
     "slx_step_finish": (c_int, [c_void_p, c_void_p]),
     "slx_step_finish_readouts": (c_int, [c_void_p, C.POINTER(SlxReadouts), c_void_p]),
     "slx_solver_status": (c_int, [c_void_p, C.POINTER(SlxStatus), c_void_p]),
+    "slx_run_steps": (c_int, [c_void_p, c_void_p, C.c_long, c_int, P_double, P_double, P_int,
+                              C.POINTER(SlxStatus), c_void_p]),
     "slx_solver_fields": (c_int, [c_void_p, C.POINTER(SlxFields)]),
 }
 

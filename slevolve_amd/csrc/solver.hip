@@ -9,6 +9,7 @@
 // The inner iterations are launched ahead and predicated on a device convergence flag.
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "sht_launch.h"
 #include "slx.h"
@@ -1834,6 +1835,46 @@ extern "C" int slx_solver_status(slx_solver* s, slx_status* out, void* stream) {
   SLX_HIP(hipStreamSynchronize(st));
   std::memcpy(out, s->pinned, sizeof(slx_status));
   if (out->finished && !s->closed && s->step > 0) step_close(s);
+  return 0;
+}
+
+// Native step loop (the loop of run_ais.py:389-425 without per-step host outputs): per step, the
+// step length, begin, batches of inner iterations each followed by the predicated finish and one
+// status read, until the step has finished.  The first batch has *guess iterations (the previous
+// step's count); a short batch is followed by single iterations while fewer than 16 are enqueued,
+// then by batches of half the count so far.
+extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, int nsteps, const double* dts,
+                             const double* dt_prev, int* guess, slx_status* out, void* stream) {
+  SLX_CHECK(s && ice && dts && dt_prev && guess && out, "slx_run_steps: null argument");
+  SLX_CHECK(nsteps >= 0 && ice_stride >= s->c.npts, "slx_run_steps: bad nsteps or ice_stride");
+  double d1 = dt_prev[0], d2 = dt_prev[1];
+  int n = *guess > 0 ? *guess : 1;
+  for (int k = 0; k < nsteps; ++k) {
+    int rc = slx_solver_set_dt(s, dts[k]);
+    if (!rc) rc = slx_step_begin(s, ice + (long)k * ice_stride, dts[k], d1, d2, stream);
+    if (rc) return rc;
+    int total = 0, batch = n;
+    slx_status st;
+    for (;;) {
+      rc = slx_step_iterate(s, batch, stream);
+      if (!rc) rc = slx_step_finish(s, stream);
+      if (!rc) rc = slx_solver_status(s, &st, stream);
+      if (rc) return rc;
+      total += batch;
+      if (st.failed) {
+        out[k] = st;
+        slx_set_error("slx_run_steps: inner loop failed to converge at step " + std::to_string(st.step));
+        return -34;
+      }
+      if (st.finished) break;
+      batch = total < 16 ? 1 : std::max(1, total / 2);
+    }
+    out[k] = st;
+    n = st.numiter > 0 ? st.numiter : 1;
+    d2 = d1;
+    d1 = dts[k];
+  }
+  *guess = n;
   return 0;
 }
 

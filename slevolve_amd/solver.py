@@ -109,6 +109,14 @@ class _HIPRT:
         return cls._lib
 
 
+def _status_record(st):
+    lam = np.ctypeslib.as_array(st.lam).reshape(3, 3, 2)
+    return {"iter": st.step, "numiter": st.numiter, "ocean_area": st.ocean_area, "oceanBeta_area": st.oceanBeta_area,
+            "gmslc_ocn": st.gmslc_ocn, "gmslc_ocnBeta": st.gmslc_ocnBeta, "ice_volume": st.ice_volume,
+            "il": np.ctypeslib.as_array(st.il).reshape(3, 3).copy(), "mm": np.ctypeslib.as_array(st.mm).copy(),
+            "lambda": lam[..., 0] + 1j * lam[..., 1], "reguess": st.reguess, "ice_analyses": st.ice_analyses}
+
+
 class SeaLevelSolver:
     """Exact-memory sea-level solver; constructor and ``step`` as in solver.py:521-1144."""
 
@@ -341,6 +349,44 @@ class SeaLevelSolver:
             "wall": wall,
             "prof": self.prof if profile else None,
         }
+
+    def run(self, ice, k0=1, k1=None, dts=None):
+        """Steps k0..k1-1 with the ice grids ice[k] (a device float64 tensor [n][nlat][nlon], e.g.
+        ``IceStack.grids``) through the native step loop (slx_run_steps: the per-step loop of
+        run_ais.py:389-425 without per-step host outputs).  dts: the step lengths (default: the current
+        dt for every step).  Returns one record of the step's scalar outputs per step; the solver's
+        grids (tgrid, ocean, beta, R, G) hold the last step's."""
+        if getattr(self, "_reuse", "off") != "off":
+            raise NotImplementedError("run() does not keep the warm-start store; use step()")
+        if not (isinstance(ice, torch.Tensor) and ice.is_cuda and ice.dtype == torch.float64 and ice.is_contiguous()
+                and ice.dim() == 3 and tuple(ice.shape[1:]) == (self.nlat, self.nlon)):
+            raise ValueError("ice must be a contiguous float64 device tensor [n][nlat][nlon]")
+        k1 = ice.shape[0] if k1 is None else int(k1)
+        n = k1 - k0
+        if n <= 0:
+            return []
+        if k0 < 0 or k1 > ice.shape[0]:
+            raise IndexError("steps outside the ice stack")
+        d = np.full(n, self.dt) if dts is None else np.ascontiguousarray(dts, dtype=np.float64)
+        if d.shape != (n,):
+            raise ValueError(f"dts must have {n} entries")
+        prev = np.array(self._dt_hist, dtype=np.float64)
+        guess = C.c_int(self._guess_iters)
+        out = (_lib.SlxStatus * n)()
+        stride = self.nlat * self.nlon
+        _lib.call("slx_run_steps", self._s_handle, C.c_void_p(ice.data_ptr() + k0 * stride * 8), stride, n,
+                  d.ctypes.data_as(_lib.P_double), prev.ctypes.data_as(_lib.P_double), C.byref(guess), out,
+                  D.stream_ptr())
+        self.iter += n
+        self._guess_iters = max(1, guess.value)
+        self.dt = float(d[-1])
+        self._dt_hist = [float(d[-1]), float(d[-2]) if n >= 2 else self._dt_hist[0]]
+        st = out[n - 1]
+        self._last_status = st
+        self.gr_counts = {"reguess": st.reguess, "ice_analyses": st.ice_analyses}
+        rec = [_status_record(x) for x in out]
+        self.il, self.mm, self.lam = rec[-1]["il"], rec[-1]["mm"], rec[-1]["lambda"]
+        return rec
 
 
 class OptSeaLevelSolver(SeaLevelSolver):
