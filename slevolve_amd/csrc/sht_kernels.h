@@ -9,6 +9,8 @@
 // / 1.41421356; g = irfft(nlon sum_l 1.41421356 U P~).  P~ orthonormal, no Condon-Shortley phase,
 // regenerated on the fly by the three-term recurrence of sht.py:105-144 in FP64.
 #pragma once
+#include <type_traits>
+#include <utility>
 #include "slx_internal.h"
 
 namespace slx {
@@ -139,10 +141,24 @@ struct ShtDev {
 // written order-major: WS[f][m][j], WD[f][m][j].
 // Loader: double operator()(int f, long idx) const -> grid value of field f at flat index idx.
 
+// A loader may carry a device-side gate() (evaluated by every thread; the pass returns when it
+// is false); loaders without one always run.
+template <class L, class = void>
+struct has_gate : std::false_type {};
+template <class L>
+struct has_gate<L, std::void_t<decltype(std::declval<const L&>().gate())>> : std::true_type {};
+template <class L>
+__device__ __forceinline__ bool loader_gate(const L& l) {
+  if constexpr (has_gate<L>::value)
+    return l.gate();
+  else
+    return true;
+}
+
 template <int NT, class Loader>
 __global__ void __launch_bounds__(NT) k_fft_ana(ShtDev d, Loader ld, double2* __restrict__ WS,
                                                 double2* __restrict__ WD, Pred pr) {
-  if (!pr.active()) return;
+  if (!pr.active() || !loader_gate(ld)) return;
   extern __shared__ double2 smem[];
   const int N = d.nlon;
   double2* a = smem;

@@ -39,6 +39,7 @@ struct DevState {
   double sum_il[9], sum_m[3], sum_tsl[6], sum_trr[6];
   double dsl_rot[6], rr_rot[6];
   double rate_sl[6], rate_rr[6], hz_rot[6], rate00;  // read-outs of the finishing step
+  double dice00;                                     // degree-0 ice increment of the finishing step
   // inner = "anderson", per step: mask frozen / being refreshed, refreshes done, pairs stored,
   // newest slot, accepted grounded re-guesses, C* analysed by the next iteration, this
   // iteration's re-guess accepted; the iteration index in which each branch runs (tokens)
@@ -95,10 +96,10 @@ struct slx_solver {
   std::vector<double> h_a_sl, h_a_rr, h_a2, h_a_hl, h_a_thl, h_ttl;  // host copies for the ramp sums
   std::vector<int> h_nmod;
   // grids
-  double *tinit, *cxy0, *beta0, *cstar0, *cxy, *beta, *cstar, *tO, *icestar, *dsl, *rr, *G, *topo, *topo_prev,
+  double *tinit, *cxy0, *beta0, *cstar0, *cxy, *beta, *cstar, *icestar, *dsl, *rr, *G, *topo, *topo_prev,
       *ice_prev, *tmp;
   // spectra (packed)
-  double2 *R, *p_load, *p_ice, *p_ocean, *prev_ice, *icestar_lm, *oldice_lm, *dice, *dI, *cstar_lm, *rt_lm,
+  double2 *R, *p_load, *p_ice, *p_ocean, *prev_ice, *icestar_lm, *oldice_lm, *cstar_lm, *rt_lm,
       *load_inc, *dS_cur, *visc, *rad, *h1, *h2, *scratch_lm;
   std::vector<double> h_s;  // host copy of s [kmax][nl]
   slx_status* pinned = nullptr;
@@ -114,10 +115,10 @@ struct Pts {  // all device pointers a kernel may need, passed by value
   DevState* st;
   double* partial;
   const double *tinit, *cxy0, *beta0, *cstar0;
-  double *cxy, *beta, *cstar, *tO, *icestar, *dsl, *rr, *G, *topo, *topo_prev, *ice_prev, *tmp, *rows;
+  double *cxy, *beta, *cstar, *icestar, *dsl, *rr, *G, *topo, *topo_prev, *ice_prev, *tmp, *rows;
   const double* w00;
   const double* ice_cur;
-  double2 *p_load, *p_ice, *p_ocean, *prev_ice, *icestar_lm, *oldice, *dice, *dI, *cstar_lm, *rt_lm, *load_inc,
+  double2 *p_load, *p_ice, *p_ocean, *prev_ice, *icestar_lm, *oldice, *cstar_lm, *rt_lm, *load_inc,
       *dS_cur, *visc, *rad, *h1, *h2;
   const double *ttl, *ttlekhl, *ttlhe, *decay, *a2;
   double* r2;
@@ -136,6 +137,17 @@ __device__ __forceinline__ double keep_factor(double ice, double topo, double rh
 }
 
 __device__ __forceinline__ double cabs_d(double2 z) { return hypot(z.x, z.y); }
+
+// ice increments of the step (solver.py:868-869): dicestar = icestar_lm - oldice, deltaicestar =
+// p_ice + dicestar, formed where they are used rather than stored
+__device__ __forceinline__ double2 ice_inc(const double2* ist, const double2* old, long i) {
+  const double2 a = ist[i], b = old[i];
+  return make_double2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ double2 ice_tot(const double2* pice, long i, double2 d) {
+  const double2 a = pice[i];
+  return make_double2(a.x + d.x, a.y + d.y);
+}
 
 // Block sum of v (blockDim 256) -> lane 0; fixed order.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -306,7 +318,7 @@ __global__ void __launch_bounds__(64) k_begin(Pts p, Consts c) {
   }
 }
 
-// grounded ice on the previous converged topography, beta, C*, tO (solver.py:853-887)
+// grounded ice on the previous converged topography, beta, C* (solver.py:853-887)
 __global__ void k_ice_grid(Pts p, Consts c) {
   const long n = c.npts;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -317,12 +329,11 @@ __global__ void k_ice_grid(Pts p, Consts c) {
     p.icestar[i] = ist;
     p.beta[i] = b;
     p.cstar[i] = cs;
-    p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
   }
 }
 
-// ice increments and the first guess of the ocean-load increment (solver.py:857-892;
-// solver_opt.py:504-527); so = sum |load_inc|
+// first guess of the ocean-load increment (solver.py:888-892; solver_opt.py:504-527);
+// so = sum |load_inc|
 struct GuessArgs {
   int extrap_ready, uniform;
   double c1, c2;
@@ -338,11 +349,6 @@ __global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
   const double2 e = make_double2(fr * d00.x, fr * d00.y);
   double a = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double2 old = p.oldice[i], ist = p.icestar_lm[i];
-    const double2 d = make_double2(ist.x - old.x, ist.y - old.y);
-    const double2 pi = p.p_ice[i];
-    p.dice[i] = d;
-    p.dI[i] = make_double2(pi.x + d.x, pi.y + d.y);
     double2 ld;
     if (g.extrap_ready) {
       const double2 a1 = p.h1[i], a2 = p.h2[i];
@@ -379,9 +385,9 @@ __global__ void __launch_bounds__(256) k_guess(Pts p, Consts c, GuessArgs g) {
 // block in its prologue from dS_cur and dI (identical arithmetic everywhere) and block 0
 // publishes it; otherwise (radial synthesis after the loop) the stored rr terms are used.
 struct CoefB18 {
-  const double2 *dI, *dSp, *ld, *resp, *dS;
+  const double2 *ist, *old, *pice;  // dicestar = ist - old, deltaicestar = pice + dicestar
+  const double2 *dSp, *ld, *resp, *dS;
   const double *w1, *w2;
-  const double2* dice;
   const double* w3;
   DevState* st;
   Consts c;
@@ -397,7 +403,8 @@ struct CoefB18 {
     if (eval) {
       RotOut o;
       if (c.tpw) {
-        rotation_eval(st, c, dS[c.idx20], dS[c.idx21], dI[c.idx20], dI[c.idx21], o);
+        rotation_eval(st, c, dS[c.idx20], dS[c.idx21], ice_tot(pice, c.idx20, ice_inc(ist, old, c.idx20)),
+                      ice_tot(pice, c.idx21, ice_inc(ist, old, c.idx21)), o);
         if (blockIdx.x == 0 && blockIdx.y == 0) rotation_store(st, o);
       } else {
         for (int q = 0; q < 6; ++q) o.dsl[q] = 0.0;
@@ -408,13 +415,13 @@ struct CoefB18 {
     }
   }
   __device__ double2 operator()(int, int idx, int l, int m, const double* pro) const {
-    const double2 a = dI[idx], b = dSp[idx], q = ld[idx], v = resp[idx];
+    const double2 d = ice_inc(ist, old, idx);
+    const double2 a = ice_tot(pice, idx, d), b = dSp[idx], q = ld[idx], v = resp[idx];
     const double ar = (a.x * c.rhoi + b.x * c.rhow) + q.x * c.rhow;
     const double ai = (a.y * c.rhoi + b.y * c.rhow) + q.y * c.rhow;
     const double u = w1[l], t = w2[l];
     double2 r = make_double2(ar * u + v.x * t, ai * u + v.y * t);
     if (c.ramp) {
-      const double2 d = dice[idx];
       const double w = w3[l];
       r.x += w * (c.rhoi * d.x + c.rhow * q.x);
       r.y += w * (c.rhoi * d.y + c.rhow * q.y);
@@ -427,11 +434,6 @@ struct CoefB18 {
   }
 };
 
-// analysis input of the inner sweep: field 0 = dslxy C* - tO (rO - tO), field 1 = C*
-struct LoadRT {
-  const double *dsl, *cstar, *tO;
-  __device__ double operator()(int f, long i) const { return f == 0 ? dsl[i] * cstar[i] - tO[i] : cstar[i]; }
-};
 template <int NF>
 struct LoadPtrs {
   const double* g[NF];
@@ -440,13 +442,19 @@ struct LoadPtrs {
 
 // Ring-pair pass of an inner iteration: the synthesised Fourier coefficients of eq. B18 are
 // inverse-transformed (one complex FFT for rows j and nlat-1-j), written as dslxy, multiplied by
-// C* and reduced by tO in LDS (rO - tO, solver.py:993-996), forward-transformed and folded into
+// C* and reduced by tO (both formed from C, beta, tinit, C*0 as they are read) in LDS (rO - tO,
+// solver.py:993-996), forward-transformed and folded into
 // analysis field 0; with nf == 2 (or a set cst_flag) the ocean function C* of the same rows is
 // transformed into analysis field 1 as well.  Replaces separate synthesis and analysis ring passes.
+struct OceanFn {  // C* = C beta and tO = tinit (C* - C*0) of a grid point (solver.py:881-884)
+  const double *cxy, *beta, *tinit, *cstar0;
+  __device__ __forceinline__ double cstar(long i) const { return cxy[i] * beta[i]; }
+  __device__ __forceinline__ double tO(long i, double cs) const { return tinit[i] * (cs - cstar0[i]); }
+};
+
 template <int NT>
 __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __restrict__ FN,
-                                                   const double2* __restrict__ FS, double* __restrict__ dsl,
-                                                   const double* __restrict__ cstar, const double* __restrict__ tO,
+                                                   const double2* __restrict__ FS, double* __restrict__ dsl, OceanFn oc,
                                                    double2* __restrict__ WS, double2* __restrict__ WD, int nf,
                                                    const int* cst_flag, Pred pr) {
   if (!pr.active()) return;
@@ -477,14 +485,15 @@ __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __re
     const double2 v = z[i];
     dsl[rn + i] = v.x;
     dsl[rs + i] = v.y;
-    z[i] = make_double2(v.x * cstar[rn + i] - tO[rn + i], v.y * cstar[rs + i] - tO[rs + i]);
+    const double cn = oc.cstar(rn + i), cs = oc.cstar(rs + i);
+    z[i] = make_double2(v.x * cn - oc.tO(rn + i, cn), v.y * cs - oc.tO(rs + i, cs));
   }
   __syncthreads();
   const double wn = d.wn[j], ws = d.ws[j];
   const long fstride = (long)d.nm * d.nh;
   for (int f = 0; f < nf; ++f) {
     if (f == 1) {
-      for (int i = threadIdx.x; i < N; i += NT) z[i] = make_double2(cstar[rn + i], cstar[rs + i]);
+      for (int i = threadIdx.x; i < N; i += NT) z[i] = make_double2(oc.cstar(rn + i), oc.cstar(rs + i));
       __syncthreads();
     }
     double2* Z = fft_lds<NT>(z, w, tws, N, d.log2n, false);
@@ -508,7 +517,8 @@ __global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __re
 // conservation constant and ocean-load update (solver.py:1002-1015); per-block partial sums of
 // |load_inc| for the convergence ratio, which the refresh kernel forms
 __device__ __forceinline__ double conserv_of(const Pts& p, const Consts& c) {
-  return (1.0 / p.cstar_lm[0].x) * (-(c.rhoi / c.rhow) * p.dI[0].x - p.rt_lm[0].x);
+  const double dI00 = ice_tot(p.p_ice, 0, ice_inc(p.icestar_lm, p.oldice, 0)).x;
+  return (1.0 / p.cstar_lm[0].x) * (-(c.rhoi / c.rhow) * dI00 - p.rt_lm[0].x);
 }
 
 // out = load_inc, or with Anderson mixing the separate output buffer G(x) (the input x is kept for
@@ -595,36 +605,41 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
   if (threadIdx.x == 0 && shf) atomicAdd(&s->nflip[it & 1], shf);
 }
 
-// apply the re-guess when a grounded flag flipped (at most GR_MAXACC times per step); C*, tO
-__global__ void k_refresh2(Pts p, Consts c, int it) {
-  DevState* s = p.st;
-  if (*(volatile int*)&s->st.done) return;
-  const int slot = it & 1;
-  const bool regs = c.checkmarine && s->nacc2[slot] < GR_MAXACC;
-  const bool changed = regs && s->nflip[slot] > 0;
-  const long n = c.npts;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    if (changed) {
-      const double ice = p.ice_cur[i];
-      const double cand = ice * keep_factor(ice, p.topo[i], c.rhow, c.rhoi);
-      p.icestar[i] = cand;
-      p.beta[i] = cand < EPS0 ? 1.0 : 0.0;
+// The grounded-ice re-guess of an inner iteration, applied by the ice ring pass
+// (solver.py:661-684): gate() accepts it when a grounded flag flipped on the topography iterate
+// (at most GR_MAXACC times per step) and block 0 records the bookkeeping; the loader stores
+// I* = ice where (topo > 0) or ice > |topo| rho_w / rho_i, else 0, and beta = (I* < eps) as the
+// pass reads them.
+struct LoadReguess {
+  const double *ice, *topo;
+  double *icestar, *beta;
+  double rhow, rhoi;
+  DevState* st;
+  int it, checkmarine;
+  __device__ bool gate() const {
+    const int slot = it & 1;
+    const bool regs = checkmarine && st->nacc2[slot] < GR_MAXACC;
+    const bool changed = regs && st->nflip[slot] > 0;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+      st->ice_changed = changed ? 1 : 0;
+      st->nacc2[slot ^ 1] = st->nacc2[slot] + (changed ? 1 : 0);
+      st->nflip[slot ^ 1] = 0;
+      if (regs) st->st.reguess += 1;
+      if (changed) {
+        st->st.ice_analyses += 1;
+        st->st.nacc += 1;
+      }
     }
-    const double cs = p.cxy[i] * p.beta[i];
-    p.cstar[i] = cs;
-    p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
+    return changed;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    s->ice_changed = changed ? 1 : 0;
-    s->nacc2[slot ^ 1] = s->nacc2[slot] + (changed ? 1 : 0);
-    s->nflip[slot ^ 1] = 0;
-    if (regs) s->st.reguess += 1;
-    if (changed) {
-      s->st.ice_analyses += 1;
-      s->st.nacc += 1;
-    }
+  __device__ double operator()(int, long i) const {
+    const double v = ice[i];
+    const double cand = v * keep_factor(v, topo[i], rhow, rhoi);
+    icestar[i] = cand;
+    beta[i] = cand < EPS0 ? 1.0 : 0.0;
+    return cand;
   }
-}
+};
 
 // ---------------------------------------------------------------------- Anderson mixing
 // Inner iteration with inner = "anderson" (solver_opt.py:705-775): xi as above in every block;
@@ -735,7 +750,8 @@ __global__ void k_decide_aa(Pts p, Consts c, int it) {
   s->nflip[slot ^ 1] = 0;
 }
 
-// mask update of a plain / refresh step: topography, ocean function, grounded re-guess, C*, tO
+// mask update of a plain / refresh step: topography, ocean function, grounded re-guess (C* and tO
+// are formed from them by the next ring pass)
 __global__ void k_refresh2_aa(Pts p, Consts c, int it) {
   DevState* s = p.st;
   if (*(volatile int*)&s->aa_tok_mask != it) return;
@@ -752,9 +768,6 @@ __global__ void k_refresh2_aa(Pts p, Consts c, int it) {
       p.icestar[i] = cand;
       p.beta[i] = cand < EPS0 ? 1.0 : 0.0;
     }
-    const double cs = cx * p.beta[i];
-    p.cstar[i] = cs;
-    p.tO[i] = p.tinit[i] * (cs - p.cstar0[i]);
   }
 }
 
@@ -936,17 +949,6 @@ __global__ void k_aa_plain(Pts p, Consts c, int it) {
     p.load_inc[i] = p.gnew[i];
 }
 
-__global__ void k_dice(Pts p, Consts c) {
-  DevState* s = p.st;
-  if (*(volatile int*)&s->st.done || !*(volatile int*)&s->ice_changed) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c.ncoef) return;
-  const double2 ist = p.icestar_lm[i], old = p.oldice[i], pi = p.p_ice[i];
-  const double2 d = make_double2(ist.x - old.x, ist.y - old.y);
-  p.dice[i] = d;
-  p.dI[i] = make_double2(pi.x + d.x, pi.y + d.y);
-}
-
 // ---------------------------------------------------------------------- step finish
 // commit (solver.py:1080-1095): load increment without degree 0 into P; ice, ocean, previous ice
 __device__ __forceinline__ bool finish_active(const DevState* s) {
@@ -958,7 +960,8 @@ __global__ void k_commit(Pts p, Consts c, int extrap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.ncoef) return;
   const int l = p.l_of[i];
-  const double2 d = p.dice[i], q = p.load_inc[i];
+  const double2 d = ice_inc(p.icestar_lm, p.oldice, i), q = p.load_inc[i];
+  const double2 dI = ice_tot(p.p_ice, i, d);
   double2 cur = make_double2(d.x * c.rhoi + q.x * c.rhow, d.y * c.rhoi + q.y * c.rhow);
   if (l == 0) cur = make_double2(0.0, 0.0);
   if (c.ramp) {
@@ -973,7 +976,7 @@ __global__ void k_commit(Pts p, Consts c, int extrap) {
   }
   const double2 pl = p.p_load[i];
   p.p_load[i] = make_double2(pl.x + cur.x, pl.y + cur.y);
-  p.p_ice[i] = p.dI[i];
+  p.p_ice[i] = dI;
   p.p_ocean[i] = p.dS_cur[i];
   p.prev_ice[i] = p.icestar_lm[i];
   if (extrap) p.h2[i] = q;  // becomes the newest history entry (the host swaps h1/h2)
@@ -1048,7 +1051,7 @@ __global__ void __launch_bounds__(256) k_finish(Pts p, Consts c, int rates, doub
   }
   if (threadIdx.x != 0) return;
   if (rates) {
-    const double d00 = ((-(c.rhoi / c.rhow) * p.dice[0].x) / c.dt - q[4]) / q[3];
+    const double d00 = ((-(c.rhoi / c.rhow) * s->dice00) / c.dt - q[4]) / q[3];
     s->rate00 = d00;
     dsl_lm[0] = make_double2(d00, 0.0);
   }
@@ -1077,6 +1080,7 @@ __global__ void k_rot_readout(Pts p, Consts c, int rates, int hor, double* rot_r
   const int n2 = c.nmod2, nl = c.nl, K = c.kmax;
   const bool rot = c.tpw && c.L >= 2;
   if (rates) {
+    s->dice00 = ice_inc(p.icestar_lm, p.oldice, 0).x;  // before the commit replaces the previous ice
     double ild[9] = {}, md[3] = {}, lad[18] = {};
     for (int q = 0; q < 6; ++q) s->rate_sl[q] = s->rate_rr[q] = 0.0;
     if (rot) {
@@ -1155,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_readout_spec(Pts p, Consts c, ReadoutSp
   const DevState* s = p.st;
   const int l = p.l_of[i], m = p.m_of[i], nl = c.nl;
   const int nk = p.nmod[l] < c.kmax ? p.nmod[l] : c.kmax;
-  const double2 d = p.dice[i], q = p.load_inc[i];
+  const double2 d = ice_inc(p.icestar_lm, p.oldice, i), q = p.load_inc[i];
   const double2 cur = l == 0 ? make_double2(0.0, 0.0)
                              : make_double2(d.x * c.rhoi + q.x * c.rhow, d.y * c.rhoi + q.y * c.rhow);
   const bool rates = a.dsl != nullptr, hor = a.hz != nullptr;
@@ -1198,7 +1202,7 @@ __global__ void __launch_bounds__(256) k_readout_spec(Pts p, Consts c, ReadoutSp
     a.drr_user[i] = dr;
   }
   if (hor) {
-    const double2 I = p.dI[i], Sp = p.p_ocean[i];
+    const double2 I = ice_tot(p.p_ice, i, d), Sp = p.p_ocean[i];
     const double Ax = (c.rhoi * I.x + c.rhow * Sp.x) + c.rhow * q.x;
     const double Ay = (c.rhoi * I.y + c.rhow * Sp.y) + c.rhow * q.y;
     const double t = p.ttl[l], e = p.ttlle[l];
@@ -1252,7 +1256,6 @@ static Pts make_pts(slx_solver* s, const double* ice_cur, const double2* oldice)
   p.cxy = s->cxy;
   p.beta = s->beta;
   p.cstar = s->cstar;
-  p.tO = s->tO;
   p.icestar = s->icestar;
   p.dsl = s->dsl;
   p.rr = s->rr;
@@ -1270,8 +1273,6 @@ static Pts make_pts(slx_solver* s, const double* ice_cur, const double2* oldice)
   p.prev_ice = s->prev_ice;
   p.icestar_lm = s->icestar_lm;
   p.oldice = const_cast<double2*>(oldice);
-  p.dice = s->dice;
-  p.dI = s->dI;
   p.cstar_lm = s->cstar_lm;
   p.rt_lm = s->rt_lm;
   p.load_inc = s->load_inc;
@@ -1457,13 +1458,12 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   s->beta = o->beta;
   s->rr = o->R;
   s->G = o->G;
-  for (double** q : {&s->tinit, &s->cxy0, &s->beta0, &s->cstar0, &s->cstar, &s->tO, &s->icestar, &s->dsl,
+  for (double** q : {&s->tinit, &s->cxy0, &s->beta0, &s->cstar0, &s->cstar, &s->icestar, &s->dsl,
                      &s->topo_prev, &s->ice_prev, &s->tmp}) {
     SLX_HIP(hipMalloc(q, G));
     SLX_HIP(hipMemset(*q, 0, G));
   }
-  for (double2** q : {&s->p_load, &s->p_ice, &s->p_ocean, &s->prev_ice, &s->icestar_lm, &s->oldice_lm, &s->dice,
-                      &s->dI, &s->cstar_lm, &s->rt_lm, &s->load_inc, &s->dS_cur, &s->visc, &s->rad, &s->h1, &s->h2}) {
+  for (double2** q : {&s->p_load, &s->p_ice, &s->p_ocean, &s->prev_ice, &s->icestar_lm, &s->oldice_lm, &s->cstar_lm, &s->rt_lm, &s->load_inc, &s->dS_cur, &s->visc, &s->rad, &s->h1, &s->h2}) {
     SLX_HIP(hipMalloc(q, S));
     SLX_HIP(hipMemset(*q, 0, S));
   }
@@ -1490,8 +1490,8 @@ extern "C" int slx_solver_destroy(slx_solver* s) {
   void* ptrs[] = {s->st, s->partial, s->rows, s->w00, s->nmod, s->s, s->decay, s->omd, s->a_sl, s->a_rr, s->a2,
                   s->r2, s->ttl, s->ttlekhl, s->ttlhe, s->omphi, s->w_sl, s->w_rr, s->a_hl, s->a_thl, s->ttlle,
                   s->deg, s->aaX, s->aaG, s->aaF, s->gnew, s->aa_part, s->tinit, s->cxy0, s->beta0, s->cstar0,
-                  s->cstar, s->tO, s->icestar, s->dsl, s->topo_prev, s->ice_prev, s->tmp,
-                  s->p_load, s->p_ice, s->p_ocean, s->prev_ice, s->icestar_lm, s->oldice_lm, s->dice, s->dI,
+                  s->cstar, s->icestar, s->dsl, s->topo_prev, s->ice_prev, s->tmp,
+                  s->p_load, s->p_ice, s->p_ocean, s->prev_ice, s->icestar_lm, s->oldice_lm,
                   s->cstar_lm, s->rt_lm, s->load_inc, s->dS_cur, s->visc, s->rad, s->h1, s->h2, s->scratch_lm,
                   s->R};
   for (void* q : ptrs)
@@ -1649,11 +1649,12 @@ static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
   const int* done = &s->st->st.done;
   const int* changed = &s->st->ice_changed;
   const int* cst = &s->st->aa_cst;
-  const int nsb = (c.ncoef + 255) / 256, nrb = red_blocks(c.ncoef);
+  const int nrb = red_blocks(c.ncoef);
   for (int it = 0; it < n; ++it) {
     const int it_step = s->iters_enqueued;
     s->iters_enqueued += 1;
-    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->dice, s->deg, s->st, c, 1};
+    CoefB18 cb{s->icestar_lm, oldice, s->p_ice, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->deg,
+               s->st, c, 1};
     int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
     if (rc) return rc;
     const size_t lds = (2 * (size_t)c.nlon + c.nlon / 4) * sizeof(double2);
@@ -1663,8 +1664,9 @@ static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       lds_allowed = lds;
     }
-    k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl, s->cstar, s->tO,
-                                                   s->h->WS, s->h->WD, 1, cst, Pred{done, 0});
+    k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl,
+                                                   OceanFn{s->cxy, s->beta, s->tinit, s->cstar0}, s->h->WS, s->h->WD,
+                                                   1, cst, Pred{done, 0});
     SLX_LAUNCH_CHECK();
     rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0, cst, 1}, st);
     if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0, cst, 0}, st);
@@ -1681,8 +1683,6 @@ static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
       rc = launch_fft_ana<256>(s->h, LoadPtrs<1>{{s->icestar}}, 1, Pred{done, 0, changed, 1}, st);
       if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
       if (rc) return rc;
-      k_dice<<<nsb, 256, 0, st>>>(p, c);
-      SLX_LAUNCH_CHECK();
     }
     k_aa_push<<<nrb, 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
@@ -1704,13 +1704,13 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
   Pts p = make_pts(s, s->ice_cur, oldice);
   const int* done = &s->st->st.done;
   const int* changed = &s->st->ice_changed;
-  const int nsb = (c.ncoef + 255) / 256;
   if (c.aa) return iterate_anderson(s, n, st);
   for (int it = 0; it < n; ++it) {
     const bool first = (s->iters_enqueued == 0);
     const int it_step = s->iters_enqueued;
     s->iters_enqueued += 1;
-    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->dice, s->deg, s->st, c, 1};
+    CoefB18 cb{s->icestar_lm, oldice, s->p_ice, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->deg,
+               s->st, c, 1};
     int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
     if (rc) return rc;
     {
@@ -1722,8 +1722,9 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         lds_allowed = lds;
       }
-      k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl, s->cstar, s->tO,
-                                                     s->h->WS, s->h->WD, nf, nullptr, Pred{done, 0});
+      k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl,
+                                                     OceanFn{s->cxy, s->beta, s->tinit, s->cstar0}, s->h->WS,
+                                                     s->h->WD, nf, nullptr, Pred{done, 0});
       SLX_LAUNCH_CHECK();
     }
     if (first)
@@ -1735,14 +1736,11 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     SLX_LAUNCH_CHECK();
     k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
     SLX_LAUNCH_CHECK();
-    k_refresh2<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step);
-    SLX_LAUNCH_CHECK();
     if (c.checkmarine) {
-      rc = launch_fft_ana<256>(s->h, LoadPtrs<1>{{s->icestar}}, 1, Pred{done, 0, changed, 1}, st);
+      const LoadReguess ld{s->ice_cur, s->topo, s->icestar, s->beta, c.rhow, c.rhoi, s->st, it_step, c.checkmarine};
+      rc = launch_fft_ana<256>(s->h, ld, 1, Pred{done, 0}, st);
       if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
       if (rc) return rc;
-      k_dice<<<nsb, 256, 0, st>>>(p, c);
-      SLX_LAUNCH_CHECK();
     }
   }
   return 0;
@@ -1775,7 +1773,8 @@ extern "C" int slx_step_finish_readouts(slx_solver* s, const slx_readouts* ro, v
     SLX_CHECK(!hor || s->has_horiz, "slx_step_finish_readouts: horizontal read-out needs a_hl at solver creation");
   }
   if (c.calcRG) {
-    CoefB18 cb{s->dI, s->p_ocean, s->load_inc, s->rad, s->dS_cur, s->ttlhe, s->ttl, s->dice, s->deg + c.nl, s->st, c, 0};
+    CoefB18 cb{s->icestar_lm, oldice, s->p_ice, s->p_ocean, s->load_inc, s->rad, s->dS_cur, s->ttlhe, s->ttl,
+               s->deg + c.nl, s->st, c, 0};
     int rc = launch_leg_syn<1>(s->h, cb, fin, st);
     if (rc) return rc;
     rc = launch_fft_syn<256>(s->h, StorePlain{s->rr, 0}, 1, fin, st);
