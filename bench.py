@@ -181,7 +181,8 @@ def main():
     traffic, traffic_src = measured_traffic(w.name, "leg_ana_2field")
     roof = {"bound": "mfma", "achieved": flops / t_kernel / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": flops / t_kernel / PEAK_FP64, "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": "k_leg_ana_mf<2,R> (per-order FP64 Legendre analysis contraction on the matrix cores, 2 fields)",
+            "kernel": f"{sht.kernels['analysis']}<2,{sht.kernels['analysis_rings_per_lane']}> (per-order FP64 "
+                      f"Legendre analysis contraction on the matrix cores, 2 fields; autotuned variant)",
             "kernel_us": t_kernel * 1e6, "flops_per_launch": flops}
     out = {
         "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,

@@ -36,8 +36,10 @@ int slx_abi_version(void);
  * max_fields (1..3) fields per call.  nlat even, nlon a power of two, lmax <= nlon/2. */
 int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_sht** out);
 int slx_sht_destroy(slx_sht* h);
-/* dims (host, 5 ints or NULL) = {nlat, nlon, lmax, ncoef, max_fields}; gaulats/weights (host,
- * nlat doubles each, or NULL) = the nodes mu_j north -> south and weights (SHT.gaulats). */
+/* dims (host, 9 ints or NULL) = {nlat, nlon, lmax, ncoef, max_fields, analysis kernel (-1 VALU,
+ * 0 matrix cores one group per block, 1 persistent chunked), its rings per lane, synthesis kernel
+ * (0 scalar-fed VALU, 1 matrix cores), its rings per lane}; gaulats/weights (host, nlat doubles
+ * each, or NULL) = the nodes mu_j north -> south and weights (SHT.gaulats). */
 int slx_sht_info(const slx_sht* h, int* dims, double* gaulats, double* weights);
 /* Replaces SHT.analysis (sht.py:274-302) / DuccSHT.analysis (sht_ducc.py:134-154) for nf fields
  * at once: grid (device) nf x [nlat][nlon] -> coef (device) nf x [ncoef] packed. */

@@ -224,6 +224,9 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     h->ngrp = (int)gm.size();
     ALLOC_COPY(h->grp_m, gm);
     ALLOC_COPY(h->grp_s0, gs);
+    std::vector<int4> gd(gm.size());
+    for (size_t i = 0; i < gm.size(); ++i) gd[i] = make_int4(gm[i], gs[i], first[gm[i]] + gs[i], h->h_moff[gm[i]]);
+    ALLOC_COPY(h->grp_desc, gd);
   }
 #undef ALLOC_COPY
   SLX_HIP(hipMalloc(&h->WS, wsz));
@@ -244,11 +247,19 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     SLX_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int best_v = -1;
-    for (int v = 1; v <= 2; ++v) {
+    // matrix-core candidates v = 1..4: rings per lane 2 or 4, one group per block (v = 1, 2) or
+    // the chunked-accumulator persistent kernel (v = 3, 4); SLX_ANA_VARIANT = mf2 | mf4 | ck2 | ck4
+    // fixes the choice
+    const char* av = std::getenv("SLX_ANA_VARIANT");
+    const std::string awant = av ? av : "";
+    const int vmax = h->ana_mfma ? 4 : 2;
+    for (int v = 1; v <= vmax; ++v) {
       if (h->ana_mfma) {
-        const int r = 2 * v;
+        const int r = (v & 1) ? 2 : 4, ck = v > 2 ? 1 : 0;
         if ((nh + 16 * r - 1) / (16 * r) > 8) continue;
+        if (!awant.empty() && awant != std::string(ck ? "ck" : "mf") + std::to_string(r)) continue;
         h->ana_r = r;
+        h->ana_ck = ck;
       } else {
         if ((nh + 64 * v - 1) / (64 * v) > 4) continue;
         h->ana_nw = v;
@@ -275,10 +286,15 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       }
     }
     if (best_v > 0) {
-      if (h->ana_mfma)
-        h->ana_r = 2 * best_v;
-      else
+      if (h->ana_mfma) {
+        h->ana_r = (best_v & 1) ? 2 : 4;
+        h->ana_ck = best_v > 2 ? 1 : 0;
+      } else {
         h->ana_nw = best_v;
+      }
+    } else if (h->ana_mfma) {  // the requested variant does not fit this handle: 4 rings per lane
+      h->ana_r = 4;
+      h->ana_ck = 0;
     }
     // synthesis kernel: scalar-fed VALU, or matrix cores with 2 or 4 rings per lane (the latter
     // only with the matrix-core segments).  SLX_SYN_VARIANT = scalar | mfma2 | mfma4 | mfma fixes
@@ -330,7 +346,7 @@ extern "C" int slx_sht_destroy(slx_sht* h) {
   if (!h) return 0;
   void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->tw,
                   h->WS, h->WD, h->FN, h->FSo, h->seg_m, h->seg_l0, h->seg_first, h->seed_a, h->seed_b,
-                  h->seed_s, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg};
+                  h->seed_s, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg, h->grp_desc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -345,6 +361,12 @@ extern "C" int slx_sht_info(const slx_sht* h, int* dims, double* gaulats, double
     dims[2] = h->lmax;
     dims[3] = h->ncoef;
     dims[4] = h->max_fields;
+    // kernels chosen at creation: analysis (0 one group per block, 1 persistent chunked; rings per
+    // lane), synthesis (0 scalar-fed VALU, 1 matrix cores; rings per lane)
+    dims[5] = h->ana_mfma ? h->ana_ck : -1;
+    dims[6] = h->ana_mfma ? h->ana_r : 0;
+    dims[7] = h->syn_mfma;
+    dims[8] = h->syn_r;
   }
   if (gaulats) std::memcpy(gaulats, h->h_x.data(), h->nlat * sizeof(double));
   if (weights) std::memcpy(weights, h->h_w.data(), h->nlat * sizeof(double));

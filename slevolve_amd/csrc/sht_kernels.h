@@ -463,6 +463,7 @@ struct GrpDev {
   int ngrp;
   const int* m;   // order of the group
   const int* s0;  // first segment (index within the order) of the group
+  const int4* desc = nullptr;  // (m, s0, seed row of segment s0, packed offset of order m)
 };
 
 constexpr int MF_S = 32;  // degrees per segment for the matrix-core analysis
@@ -650,6 +651,165 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
     double sum = red[v * 64 + ln];
     for (int w = 1; w < NW; ++w) sum += red[(w * 8 + v) * 64 + ln];
     emit(v, ln, sum);
+  }
+}
+
+// Chunked-accumulator form of ana_mf_steps: 8 degree steps per chunk, reduced over the 4 MFMA
+// blocks after each chunk and written to this wave's LDS partials (8 x 64 doubles); afterwards the
+// lane of partial v holds t = 8 (v >> 1) + 4 ((lane >> 3) & 1) + 2 ((lane >> 2) & 1) + (v & 1).
+// The MFMA accumulation of every t and the pairing of the reduction (lanes xor 8, then xor 4) are
+// those of ana_mf_steps + mf_block_reduce, so the results are identical; 16 accumulator registers
+// instead of 64.
+template <int R, bool HEADG>
+__device__ __forceinline__ void ana_ck_steps(LegState (&st)[R], const double (&xr)[R], const double (&wv)[2][R],
+                                             const double* qcs, int slot, bool head, double s23, double* redw,
+                                             int lane) {
+  constexpr int S = MF_S;
+#pragma unroll
+  for (int c = 0; c < S / 8; ++c) {
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      double pv[2][R];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int tt = 8 * c + u + h;
+        const double cq = qcs[slot * S + tt];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          double p;
+          if (HEADG && tt == 0) {  // head: P_m itself, the state does not move
+            const double pa0 = st[r].pa, pb0 = st[r].pb;
+            const double pn = q_next(xr[r], cq, st[r]);
+            p = head ? pb0 : pn;
+            st[r].pa = head ? pa0 : st[r].pa;
+            st[r].pb = head ? pb0 : st[r].pb;
+          } else if (HEADG && tt == 1) {  // head: P_{m+1} = sqrt(2m+3) x P_m
+            const double ps = s23 * xr[r] * st[r].pb;
+            const double pn = q_next(xr[r], cq, st[r]);
+            p = head ? ps : pn;
+            st[r].pb = p;
+          } else {
+            p = q_next(xr[r], cq, st[r]);
+          }
+          pv[h][r] = p;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        acc[u] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv[0][r], wv[0][r], acc[u], 0, 0, 0);
+        acc[u + 1] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv[1][r], wv[1][r], acc[u + 1], 0, 0, 0);
+      }
+    }
+    {
+      const bool up = (lane & 8) != 0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double send = up ? acc[v] : acc[v + 4];
+        const double keep = up ? acc[v + 4] : acc[v];
+        acc[v] = keep + __shfl_xor(send, 8, 64);
+      }
+    }
+    {
+      const bool up = (lane & 4) != 0;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const double send = up ? acc[v] : acc[v + 2];
+        const double keep = up ? acc[v + 2] : acc[v];
+        acc[v] = keep + __shfl_xor(send, 4, 64);
+      }
+    }
+    redw[(2 * c) * 64 + lane] = acc[0];
+    redw[(2 * c + 1) * 64 + lane] = acc[1];
+  }
+}
+
+// Persistent form of k_leg_ana_mf: the grid holds as many blocks as can be resident (5 waves per
+// SIMD with the chunked accumulators), each looping over the groups of its XCD's share in order,
+// so no late round of a few leftover groups follows the resident ones.  A group's order, first
+// segment, seed row and packed offset come from one descriptor load, so its seeds, inputs and
+// tables are requested after a single dependent load.  Same arithmetic as k_leg_ana_mf.
+template <int NF, int R>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_leg_ana_ck(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, const double2* __restrict__ WD,
+             OutPtrs<NF> out, Pred pr) {
+  static_assert(NF == 1 || NF == 2, "one or two fields per launch");
+  if (!pr.active()) return;
+  constexpr int S = MF_S;
+  extern __shared__ double red[];  // [NW][8][64] wave partials
+  __shared__ double qcs[4 * S];    // scaled recurrence coefficients of the 4 slots
+  __shared__ double qgs[4 * S];    // and the factors P~ = qg Q
+  const int per = (gr.ngrp + 7) >> 3;
+  const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
+  const int gend = min(gr.ngrp, (x + 1) * per);
+  const int L = d.lmax, nh = d.nh;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const int blk = (lane >> 2) & 3, slot = lane & 3, kk = lane >> 4;
+  const long fstride = (long)d.nm * nh;
+  const int col = slot;  // B column of this lane (same lane bits as its A row)
+  const bool colok = (col >> 1) < NF;
+  const int fb = colok ? (col >> 1) : 0;
+  for (int gi = x * per + (blockIdx.x >> 3); gi < gend; gi += nbx) {
+    const int4 ds = gr.desc[gi];
+    const int m = ds.x, s0 = ds.y, mo = ds.w;
+    const int nsm = (L - m + S) / S;  // segments of order m
+    for (int i = threadIdx.x; i < 4 * S; i += blockDim.x) {
+      const int off = (s0 + i / S) * S + i % S;  // l - m
+      const bool use = off <= L - m;
+      qcs[i] = use ? d.qc[mo + off] : 0.0;
+      qgs[i] = use ? d.qg[mo + off] : 0.0;
+    }
+    const int segi = s0 + slot;
+    const bool sval = segi < nsm;
+    const bool head = sval && segi == 0;
+    const long sx = (long)(ds.z + (sval ? slot : 0)) * nh;
+    double xr[R], wv[2][R];
+    LegState st[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = (wave * R + r) * 16 + 4 * blk + kk;  // ring of this lane (A and B roles alike)
+      const bool ok = j < nh;
+      const int jj = ok ? j : 0;
+      xr[r] = d.x[jj];
+      const bool on = ok && sval;
+      double a0 = 0.0, b0 = 0.0;
+      int c0 = 0;
+      if (on && head) {
+        b0 = d.pmm[(long)m * nh + jj];
+        c0 = d.pms[(long)m * nh + jj];
+      } else if (on) {
+        a0 = g.sa[sx + jj];
+        b0 = g.sb[sx + jj];
+        c0 = g.ss[sx + jj];
+      }
+      mf_unscale(a0, b0, c0, segi + 1 < nsm ? g.ss[sx + nh + jj] : 0);
+      st[r] = LegState{a0, b0, 0};
+      const double2 vs = WS[fb * fstride + (long)m * nh + jj], vd = WD[fb * fstride + (long)m * nh + jj];
+      const bool use = ok && colok;
+      wv[0][r] = use ? ((col & 1) ? vs.y : vs.x) : 0.0;
+      wv[1][r] = use ? ((col & 1) ? vd.y : vd.x) : 0.0;
+    }
+    const double s23 = d.s2m3[m];
+    __syncthreads();
+    if (s0 == 0)
+      ana_ck_steps<R, true>(st, xr, wv, qcs, slot, head, s23, red + wave * 8 * 64, lane);
+    else
+      ana_ck_steps<R, false>(st, xr, wv, qcs, slot, head, s23, red + wave * 8 * 64, lane);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 512; e += blockDim.x) {
+      const int v = e >> 6, ln = e & 63;
+      double sum = red[v * 64 + ln];
+      for (int w = 1; w < NW; ++w) sum += red[(w * 8 + v) * 64 + ln];
+      const int t = 8 * (v >> 1) + 4 * ((ln >> 3) & 1) + 2 * ((ln >> 2) & 1) + (v & 1);
+      const int row = ln >> 4, cj = ln & 3;
+      const int off = (s0 + row) * S + t;
+      if ((s0 + row) < nsm && off <= L - m && (cj >> 1) < NF)
+        reinterpret_cast<double*>(out.p[cj >> 1])[2 * (long)(mo + off) + (cj & 1)] =
+            (sum * qgs[row * S + t]) / SLX_SQRT2_LEGACY;
+    }
+    __syncthreads();  // the tables and partials are refilled by the next group
   }
 }
 

@@ -82,6 +82,31 @@ int launch_leg_ana_mf(const slx_sht* h, const double2* WS, const double2* WD, Ou
   return 0;
 }
 
+// Persistent chunked-accumulator analysis: one block per resident slot (as many as the device
+// holds at this block shape, a multiple of 8 so that every XCD gets the same number), at most one
+// per group.
+template <int NF, int R>
+int launch_leg_ana_ck(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
+                      hipStream_t s) {
+  const int nw = (h->nh + 16 * R - 1) / (16 * R);
+  const size_t lds = (size_t)nw * 8 * 64 * sizeof(double);
+  static int cached_nw = -1, cached_grid = 0;
+  if (nw != cached_nw) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    SLX_HIP(hipGetDevice(&dev));
+    SLX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    SLX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_leg_ana_ck<NF, R>),
+                                                         64 * nw, lds));
+    cached_grid = 8 * std::max(1, (ncu * std::max(per_cu, 1)) / 8);
+    cached_nw = nw;
+  }
+  const int grid = std::min(cached_grid, 8 * ((h->ngrp + 7) / 8));
+  GrpDev gr{h->ngrp, h->grp_m, h->grp_s0, h->grp_desc};
+  k_leg_ana_ck<NF, R><<<grid, 64 * nw, lds, s>>>(dev_of(h), seg_of(h), gr, WS, WD, out, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
 template <int NF>
 int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
                    hipStream_t s) {
@@ -91,6 +116,9 @@ int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPt
       int rc = launch_leg_ana<2>(h, WS, WD, OutPtrs<2>{{out.p[0], out.p[1]}}, pr, s);
       if (rc) return rc;
       return launch_leg_ana<1>(h, WS + 2 * fs, WD + 2 * fs, OutPtrs<1>{{out.p[2]}}, pr, s);
+    } else if (h->ana_ck) {
+      return h->ana_r == 4 ? launch_leg_ana_ck<NF, 4>(h, WS, WD, out, pr, s)
+                           : launch_leg_ana_ck<NF, 2>(h, WS, WD, out, pr, s);
     } else {
       return h->ana_r == 4 ? launch_leg_ana_mf<NF, 4>(h, WS, WD, out, pr, s)
                            : launch_leg_ana_mf<NF, 2>(h, WS, WD, out, pr, s);

@@ -71,6 +71,8 @@ struct slx_sht {
   int* seed_s = nullptr;
   // analysis on the matrix cores: groups of 4 consecutive segments of one order
   int ana_mfma = 0, ana_r = 2, ngrp = 0;
+  int ana_ck = 0;         // chunked-accumulator persistent analysis kernel (else one group per block)
+  int4* grp_desc = nullptr;  // [ngrp] (m, first segment s0, seed row first[m] + s0, moff[m])
   int syn_mfma = 0;       // synthesis on the matrix cores (else wave-uniform scalar-fed VALU)
   int syn_r = 4;          // rings per lane of the matrix-core synthesis (2 or 4)
   int *grp_m = nullptr, *grp_s0 = nullptr;

@@ -44,12 +44,16 @@ class SHT:
             _lib.check(lib.slx_sht_create(self.nlat, self.nlon, self.lmax, self.max_fields, C.byref(h)),
                        "slx_sht_create")
         self._h = h
-        dims = (C.c_int * 5)()
+        dims = (C.c_int * 9)()
         x = np.zeros(self.nlat)
         w = np.zeros(self.nlat)
         _lib.check(lib.slx_sht_info(h, dims, x.ctypes.data_as(_lib.P_double), w.ctypes.data_as(_lib.P_double)))
         self.ncoef = int(dims[3])
         self.gaulats, self.weights = x, w
+        # kernels the handle chose at creation (autotuned on this device)
+        self.kernels = {"analysis": {-1: "k_leg_ana", 0: "k_leg_ana_mf", 1: "k_leg_ana_ck"}[int(dims[5])],
+                        "analysis_rings_per_lane": int(dims[6]),
+                        "synthesis": "k_leg_syn_mf" if dims[7] else "k_leg_syn_s", "synthesis_rings_per_lane": int(dims[8])}
         self._tri = torch.zeros((self.max_fields, self.ncoef), dtype=torch.complex128, device=self.dev)
         self._coef_out = torch.zeros((self.nm, self.nm), dtype=torch.complex128, device=self.dev)
         self._grid_out = torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=self.dev)
