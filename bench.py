@@ -46,16 +46,17 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a 256 MiB device copy),
-    or None when no summary covers this kernel and workload."""
-    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.txt")
+def measured_traffic(workload, path_kernel, kernel):
+    """HBM bytes per launch of `kernel` (e.g. "k_leg_ana_ck<2,2>") on `workload` from the committed
+    rocprofv3 PMC summary profiles/pmc_<path_kernel>_<workload>.txt (FETCH_SIZE and WRITE_SIZE,
+    each scaled by what the counter reads for a 256 MiB device copy), or None when no summary
+    covers this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{path_kernel}_{workload}.txt")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
                 return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
     return None, None
 
@@ -178,11 +179,12 @@ def main():
     torch.cuda.synchronize()
     t_kernel = ev0.elapsed_time(ev1) / 1e3 / reps
     flops = 2.0 * 2 * sht.ncoef * w.nlat  # 4 flops per (l, m, ring pair) per field, 2 fields
-    traffic, traffic_src = measured_traffic(w.name, "leg_ana_2field")
+    kname = f"{sht.kernels['analysis']}<2,{sht.kernels['analysis_rings_per_lane']}>"
+    traffic, traffic_src = measured_traffic(w.name, "leg_ana_2field", kname)
     roof = {"bound": "mfma", "achieved": flops / t_kernel / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": flops / t_kernel / PEAK_FP64, "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": f"{sht.kernels['analysis']}<2,{sht.kernels['analysis_rings_per_lane']}> (per-order FP64 "
-                      f"Legendre analysis contraction on the matrix cores, 2 fields; autotuned variant)",
+            "kernel": f"{kname} (per-order FP64 Legendre analysis contraction on the matrix cores, 2 fields; "
+                      f"autotuned variant)",
             "kernel_us": t_kernel * 1e6, "flops_per_launch": flops}
     out = {
         "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,

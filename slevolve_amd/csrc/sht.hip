@@ -258,6 +258,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
         const int r = (v & 1) ? 2 : 4, ck = v > 2 ? 1 : 0;
         if ((nh + 16 * r - 1) / (16 * r) > 8) continue;
         if (!awant.empty() && awant != std::string(ck ? "ck" : "mf") + std::to_string(r)) continue;
+        if (awant.empty() && ck && r == 4) continue;  // spills at its occupancy: only when asked for
         h->ana_r = r;
         h->ana_ck = ck;
       } else {
@@ -272,7 +273,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       int rc = tune();
       if (rc) return rc;
       SLX_HIP(hipEventRecord(e0, nullptr));
-      for (int rep = 0; rep < 3; ++rep) {
+      for (int rep = 0; rep < 5; ++rep) {
         rc = tune();
         if (rc) return rc;
       }
