@@ -184,7 +184,7 @@ def main():
     roof = {"bound": "mfma", "achieved": flops / t_kernel / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": flops / t_kernel / PEAK_FP64, "traffic": traffic, "traffic_source": traffic_src,
             "kernel": f"{kname} (per-order FP64 Legendre analysis contraction on the matrix cores, 2 fields; "
-                      f"autotuned variant)",
+                      f"variant chosen by shape)",
             "kernel_us": t_kernel * 1e6, "flops_per_launch": flops}
     out = {
         "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,

@@ -54,6 +54,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   SLX_CHECK(out != nullptr, "slx_sht_create: null handle pointer");
   SLX_CHECK(nlat >= 2 && nlat % 2 == 0, "nlat must be even");
   SLX_CHECK(nlon >= 8 && (nlon & (nlon - 1)) == 0, "nlon must be a power of two >= 8");
+  SLX_CHECK(nlon <= 4096, "nlon must be <= 4096 (a ring pass holds two rows and a work buffer in LDS)");
   SLX_CHECK(lmax >= 0 && lmax <= nlon / 2, "lmax must be <= nlon/2");
   SLX_CHECK(lmax <= nlat * 2, "lmax too large for the grid");
   SLX_CHECK(max_fields >= 1 && max_fields <= 3, "max_fields must be 1..3");
@@ -236,9 +237,13 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   SLX_HIP(hipMalloc(&h->CS, (size_t)max_fields * h->ncoef * sizeof(double2)));
   SLX_HIP(hipMemset(h->WS, 0, wsz));
   SLX_HIP(hipMemset(h->WD, 0, wsz));
-  // autotune of the analysis contraction (2 fields, or 1 for a 1-field handle), timed on the
-  // device with the handle's own shapes: rings per lane (2 or 4) on the matrix-core path, waves
-  // per segment (1 or 2) otherwise
+  // choice of the analysis contraction.  By default a fixed rule per shape, so that every handle
+  // of a shape sums in the same order and results repeat bit for bit across handles and runs:
+  // the chunked-accumulator persistent kernel with 2 rings per lane where it fits (nh <= 256),
+  // else one group per block with 4 rings per lane (matrix-core path); 1 wave per segment where
+  // it fits, else 2 (VALU path).  SLX_ANA_VARIANT = mf2 | mf4 | ck2 | ck4 fixes another variant;
+  // SLX_ANA_VARIANT = auto times the candidates (2 fields, or 1 for a 1-field handle) on the
+  // device with the handle's own shapes and keeps the fastest.
   {
     double2* tmp = nullptr;
     SLX_HIP(hipMalloc(&tmp, 2 * (size_t)h->ncoef * sizeof(double2)));
@@ -248,21 +253,23 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     float best = 1e30f;
     int best_v = -1;
     // matrix-core candidates v = 1..4: rings per lane 2 or 4, one group per block (v = 1, 2) or
-    // the chunked-accumulator persistent kernel (v = 3, 4); SLX_ANA_VARIANT = mf2 | mf4 | ck2 | ck4
-    // fixes the choice
+    // the chunked-accumulator persistent kernel (v = 3, 4)
     const char* av = std::getenv("SLX_ANA_VARIANT");
-    const std::string awant = av ? av : "";
+    std::string awant = av ? av : "";
+    if (awant.empty()) awant = h->ana_mfma ? ((nh + 31) / 32 <= 8 ? "ck2" : "mf4") : "";
+    const bool timed = awant == "auto";
     const int vmax = h->ana_mfma ? 4 : 2;
     for (int v = 1; v <= vmax; ++v) {
       if (h->ana_mfma) {
         const int r = (v & 1) ? 2 : 4, ck = v > 2 ? 1 : 0;
         if ((nh + 16 * r - 1) / (16 * r) > 8) continue;
-        if (!awant.empty() && awant != std::string(ck ? "ck" : "mf") + std::to_string(r)) continue;
-        if (awant.empty() && ck && r == 4) continue;  // spills at its occupancy: only when asked for
+        if (!timed && awant != std::string(ck ? "ck" : "mf") + std::to_string(r)) continue;
+        if (timed && ck && r == 4) continue;  // spills at its occupancy: only when asked for
         h->ana_r = r;
         h->ana_ck = ck;
       } else {
         if ((nh + 64 * v - 1) / (64 * v) > 4) continue;
+        if (!timed && best_v > 0) continue;  // the first that fits
         h->ana_nw = v;
       }
       // as many fields as the work buffers hold (a 1-field handle has no second field to read)
@@ -298,8 +305,10 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       h->ana_ck = 0;
     }
     // synthesis kernel: scalar-fed VALU, or matrix cores with 2 or 4 rings per lane (the latter
-    // only with the matrix-core segments).  SLX_SYN_VARIANT = scalar | mfma2 | mfma4 | mfma fixes
-    // the choice (mfma: rings per lane still timed); otherwise the fastest is kept.
+    // only with the matrix-core segments).  By default a fixed rule per shape (matrix cores, 2
+    // rings per lane up to nh = 256, else 4; scalar without the matrix-core segments);
+    // SLX_SYN_VARIANT = scalar | mfma2 | mfma4 fixes the choice, mfma times the two ring counts and
+    // auto all three.
     {
       const char* v = std::getenv("SLX_SYN_VARIANT");
       const std::string want = v ? v : "";
@@ -309,7 +318,8 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       else if (want == "mfma2") cand = {{1, 2}};
       else if (want == "mfma4") cand = {{1, 4}};
       else if (want == "mfma") cand = {{1, 2}, {1, 4}};
-      else cand = {{0, 4}, {1, 2}, {1, 4}};
+      else if (want == "auto") cand = {{0, 4}, {1, 2}, {1, 4}};
+      else cand = {{1, nh <= 256 ? 2 : 4}};
       float bs = 1e30f;
       Cand bc = cand[0];
       for (const Cand& c : cand) {
@@ -393,7 +403,7 @@ extern "C" int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* co
   SLX_CHECK(h && grid && coef, "slx_sht_analysis: null argument");
   SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_analysis: nf out of range");
   const long gstride = (long)h->nlat * h->nlon;
-  int rc = launch_fft_ana<256>(h, LoadPlain{grid, gstride}, nf, Pred{}, (hipStream_t)stream);
+  int rc = launch_fft_ana(h, LoadPlain{grid, gstride}, nf, Pred{}, (hipStream_t)stream);
   if (rc) return rc;
   return slx_sht_analysis_contraction(h, nf, coef, stream);
 }
@@ -407,7 +417,7 @@ extern "C" int slx_sht_synthesis(slx_sht* h, const void* coef_v, int nf, double*
     const int rc = launch_leg_syn<1>(h, CoefPlain{coef + (long)f * h->ncoef, 0}, Pred{}, s, f);
     if (rc) return rc;
   }
-  return launch_fft_syn<256>(h, StorePlain{grid, (long)h->nlat * h->nlon}, nf, Pred{}, s);
+  return launch_fft_syn(h, StorePlain{grid, (long)h->nlat * h->nlon}, nf, Pred{}, s);
 }
 
 // packed <-> dense [l][m] (L+1)x(L+1) complex, zeros above the diagonal

@@ -25,31 +25,41 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
 }
 
 // -------------------------------------------------------------------------------------------
+// Ring passes run one block of nt = N / P threads per ring pair (N = nlon): thread t owns points
+// t + k nt (k < P) of both rows.  The passes are latency-bound (a few hundred blocks, a short chain
+// of dependent phases each), so every global load of a pass is issued into registers before the
+// first LDS phase and the FFT loops have exact trip counts.  P = 4 up to N = 2048 (nt <= 512), 8
+// at N = 4096 (two rows and their ping-pong buffer fill the LDS at that size).
+constexpr int RING_MAXNT = 512;
+
 // Complex FFT of length N = 2^log2N held in LDS (a), Stockham autosort with ping-pong buffer b:
 // one radix-2 stage first when log2N is odd, then radix-4 stages.  Forward uses
 // exp(-2 pi i k n / N), inverse exp(+...), both unnormalised.  tw4 = exp(-2 pi i t / N) for
 // t < N/4 (the callers stage it in LDS); the other quadrants follow by exact rotations:
 // exp(-2 pi i (t + q N/4) / N) = tw4[t] (-i)^q.  Returns the buffer holding the result.  Every
-// thread of the block must call it.
+// thread of the block (nt = N / P of them) must call it.
 __device__ __forceinline__ double2 tw_quarter(const double2* tw4, int u, int log2N) {
   const int q = u >> (log2N - 2), r = u & ((1 << (log2N - 2)) - 1);
   const double2 w = tw4[r];
   return q == 0 ? w : q == 1 ? make_double2(w.y, -w.x) : q == 2 ? make_double2(-w.x, -w.y) : make_double2(-w.y, w.x);
 }
 
-// stage the quarter-period twiddle table (N/4 entries) in LDS
-template <int NT>
-__device__ __forceinline__ void stage_tw4(double2* tw4, const double2* __restrict__ tw, int N) {
-  for (int i = threadIdx.x; i < N / 4; i += NT) tw4[i] = tw[i];
+// stage the quarter-period twiddle table (N/4 = (P/4) nt entries) in LDS
+template <int P>
+__device__ __forceinline__ void stage_tw4(double2* tw4, const double2* __restrict__ tw, int nt) {
+#pragma unroll
+  for (int r = 0; r < P / 4; ++r) tw4[threadIdx.x + r * nt] = tw[threadIdx.x + r * nt];
 }
 
-template <int NT>
+template <int P>
 __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ tw, int N, int log2N,
                             bool inverse) {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, nt = N / P;
   int Ns = 1;
   if (log2N & 1) {
-    for (int j = tid; j < N / 2; j += NT) {
+#pragma unroll
+    for (int r = 0; r < P / 2; ++r) {
+      const int j = tid + r * nt;
       double2 v0 = a[j], v1 = a[j + N / 2];
       b[2 * j] = cadd(v0, v1);
       b[2 * j + 1] = csub(v0, v1);
@@ -61,7 +71,9 @@ __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ 
   const int q = N / 4;
   for (; Ns < N; Ns *= 4) {
     const int tstride = N / (4 * Ns);
-    for (int j = tid; j < q; j += NT) {
+#pragma unroll
+    for (int r = 0; r < P / 4; ++r) {
+      const int j = tid + r * nt;
       const int k = j & (Ns - 1);
       double2 v0 = a[j], v1 = a[j + q], v2 = a[j + 2 * q], v3 = a[j + 3 * q];
       if (Ns > 1) {
@@ -97,6 +109,53 @@ __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ 
     double2* t = a; a = b; b = t;
   }
   return a;
+}
+
+// Fourier coefficients of a ring pair, read into registers: m = tid + k nt for k < MP covers
+// 0..L (L <= N/2 < MP nt); entries past L are zero.
+template <int P>
+struct RingCoef {
+  static constexpr int MP = P / 2 + 1;
+  double2 fn[MP], fs[MP];
+  __device__ __forceinline__ void load(const double2* __restrict__ FN, const double2* __restrict__ FS, long base,
+                                       int nh, int j, int L, int nt) {
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+      const int m = threadIdx.x + k * nt;
+      fn[k] = fs[k] = make_double2(0.0, 0.0);
+      if (m <= L) {
+        fn[k] = FN[base + (long)m * nh + j];
+        fs[k] = FS[base + (long)m * nh + j];
+      }
+    }
+  }
+  // Hermitian spectra of rows j (Fn) and nlat-1-j (Fs) packed as Z = Xn + i Xs, Im of m = 0 and
+  // of the Nyquist bin dropped as irfft does: Z[m] = Fn + i Fs, Z[N-m] = conj(Fn) + i conj(Fs),
+  // zero for L < i < N - L.  Every index is written exactly once (no barrier between the parts).
+  __device__ __forceinline__ void scatter(double2* a, int N, int L, int nt) const {
+    for (int i = L + 1 + threadIdx.x; i < N - L; i += nt) a[i] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+      const int m = threadIdx.x + k * nt;
+      if (m > L) continue;
+      const double2 f = fn[k], g = fs[k];
+      if (m == 0 || 2 * m == N) {
+        a[m] = make_double2(f.x, g.x);
+      } else {
+        a[m] = make_double2(f.x - g.y, f.y + g.x);          // Fn + i Fs
+        a[N - m] = make_double2(f.x + g.y, -f.y + g.x);     // conj(Fn) + i conj(Fs)
+      }
+    }
+  }
+};
+
+// Ring of a ring-pass block, XCD-aware.  The hardware deals block b to XCD b % 8; block b takes
+// ring start(b % 8) + b / 8, so that every XCD handles one contiguous run of n / 8 (+1) rings.  The
+// order-major arrays [m][j] hold eight neighbouring rings in one 128-byte line, which is then
+// written (analysis) or read (synthesis) through one XCD's L2 instead of eight.
+__device__ __forceinline__ int xcd_ring(int b, int n) {
+  const int x = b & 7, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 // Predication of a launch on device flags: the kernel runs only if (*f0 != 0) == w0 and
@@ -139,10 +198,12 @@ struct ShtDev {
 // into the two real spectra; the fold with the quadrature weights gives
 //   S[m] = Fn w_n + Fs w_s   (pairs with even l - m),   D[m] = Fn w_n - Fs w_s   (odd l - m),
 // written order-major: WS[f][m][j], WD[f][m][j].
-// Loader: double operator()(int f, long idx) const -> grid value of field f at flat index idx.
+// Loader: double operator()(int f, long idx) const -> grid value of field f at flat index idx
+// (reads only, so that the pass can issue all of them at once).
 
 // A loader may carry a device-side gate() (evaluated by every thread; the pass returns when it
-// is false); loaders without one always run.
+// is false) and a commit(f, idx, v) that stores what it derives from the values it supplied
+// (called after all loads are in flight); loaders without them always run and store nothing.
 template <class L, class = void>
 struct has_gate : std::false_type {};
 template <class L>
@@ -154,33 +215,55 @@ __device__ __forceinline__ bool loader_gate(const L& l) {
   else
     return true;
 }
+template <class L, class = void>
+struct has_commit : std::false_type {};
+template <class L>
+struct has_commit<L, std::void_t<decltype(std::declval<const L&>().commit(0, 0L, 0.0))>> : std::true_type {};
 
-template <int NT, class Loader>
-__global__ void __launch_bounds__(NT) k_fft_ana(ShtDev d, Loader ld, double2* __restrict__ WS,
-                                                double2* __restrict__ WD, Pred pr) {
+// Split of the packed transform Z into the rows' spectra and the fold with the quadrature weights
+// (shared by every analysis ring pass): Fn = (Z_m + conj Z_{N-m}) / 2, Fs = (Z_m - conj Z_{N-m}) / (2i).
+__device__ __forceinline__ void ring_fold(const double2* Z, double2* __restrict__ WS, double2* __restrict__ WD,
+                                          long base, int nh, int j, int L, int N, double wn, double ws, int nt) {
+  for (int m = threadIdx.x; m <= L; m += nt) {
+    const double2 zm = Z[m], zc = Z[(N - m) & (N - 1)];
+    const double2 fn = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y - zc.y));
+    const double2 fs = make_double2(0.5 * (zm.y + zc.y), -0.5 * (zm.x - zc.x));
+    const double2 tn = make_double2(fn.x * wn, fn.y * wn), ts = make_double2(fs.x * ws, fs.y * ws);
+    WS[base + (long)m * nh + j] = cadd(tn, ts);
+    WD[base + (long)m * nh + j] = csub(tn, ts);
+  }
+}
+
+template <int P, class Loader>
+__global__ void __launch_bounds__(RING_MAXNT) k_fft_ana(ShtDev d, Loader ld, double2* __restrict__ WS,
+                                                        double2* __restrict__ WD, Pred pr) {
   if (!pr.active() || !loader_gate(ld)) return;
   extern __shared__ double2 smem[];
-  const int N = d.nlon;
+  const int N = d.nlon, nt = N / P, tid = threadIdx.x;
   double2* a = smem;
   double2* b = smem + N;
   double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
-  const int j = blockIdx.x, f = blockIdx.y, js = d.nlat - 1 - j;
+  const int j = xcd_ring(blockIdx.x, d.nh), f = blockIdx.y, js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
-  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(ld(f, rn + i), ld(f, rs + i));
-  stage_tw4<NT>(tws, d.tw, N);
-  __syncthreads();
-  double2* Z = fft_lds<NT>(a, b, tws, N, d.log2n, false);
-  const double wn = d.wn[j], ws = d.ws[j];
-  const long base = (long)f * d.nm * d.nh;
-  for (int m = threadIdx.x; m <= d.lmax; m += NT) {
-    double2 zm = Z[m], zc = Z[(N - m) & (N - 1)];
-    // Fn = (Z_m + conj Z_{N-m}) / 2 ; Fs = (Z_m - conj Z_{N-m}) / (2i)
-    double2 fn = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y - zc.y));
-    double2 fs = make_double2(0.5 * (zm.y + zc.y), -0.5 * (zm.x - zc.x));
-    double2 tn = make_double2(fn.x * wn, fn.y * wn), ts = make_double2(fs.x * ws, fs.y * ws);
-    WS[base + (long)m * d.nh + j] = cadd(tn, ts);
-    WD[base + (long)m * d.nh + j] = csub(tn, ts);
+  double vn[P], vs[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    vn[k] = ld(f, rn + tid + k * nt);
+    vs[k] = ld(f, rs + tid + k * nt);
   }
+  stage_tw4<P>(tws, d.tw, nt);
+#pragma unroll
+  for (int k = 0; k < P; ++k) a[tid + k * nt] = make_double2(vn[k], vs[k]);
+  if constexpr (has_commit<Loader>::value) {
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      ld.commit(f, rn + tid + k * nt, vn[k]);
+      ld.commit(f, rs + tid + k * nt, vs[k]);
+    }
+  }
+  __syncthreads();
+  double2* Z = fft_lds<P>(a, b, tws, N, d.log2n, false);
+  ring_fold(Z, WS, WD, (long)f * d.nm * d.nh, d.nh, j, d.lmax, N, d.wn[j], d.ws[j], nt);
 }
 
 // -------------------------------------------------------------------------------------------
@@ -406,43 +489,33 @@ __global__ void __launch_bounds__(64 * NW) k_leg_ana(ShtDev d, SegDev g, const d
 
 // -------------------------------------------------------------------------------------------
 // Synthesis back end: one block per (northern ring j, field f).  Builds the Hermitian spectra of
-// rows j and nlat-1-j (Im of m = 0 and of the Nyquist bin dropped, as irfft does), packs them as
-// Z = Xn + i Xs, inverse-transforms once and hands (row, column, value) to the Store functor:
-// void operator()(int f, long idx, double v) const.
-template <int NT, class Store>
-__global__ void __launch_bounds__(NT) k_fft_syn(ShtDev d, const double2* __restrict__ FN,
-                                                const double2* __restrict__ FS, Store st,
-                                                Pred pr) {
+// rows j and nlat-1-j (RingCoef::scatter), inverse-transforms them once and hands (row, column,
+// value) to the Store functor: void operator()(int f, long idx, double v) const.
+template <int P, class Store>
+__global__ void __launch_bounds__(RING_MAXNT) k_fft_syn(ShtDev d, const double2* __restrict__ FN,
+                                                        const double2* __restrict__ FS, Store st, Pred pr) {
   if (!pr.active()) return;
   extern __shared__ double2 smem[];
-  const int N = d.nlon, L = d.lmax;
+  const int N = d.nlon, L = d.lmax, nt = N / P, tid = threadIdx.x;
   double2* a = smem;
   double2* b = smem + N;
   double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
-  const int j = blockIdx.x, f = blockIdx.y, js = d.nlat - 1 - j;
-  const long base = (long)f * d.nm * d.nh;
-  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(0.0, 0.0);
-  stage_tw4<NT>(tws, d.tw, N);
+  const int j = xcd_ring(blockIdx.x, d.nh), f = blockIdx.y, js = d.nlat - 1 - j;
+  RingCoef<P> rc;
+  rc.load(FN, FS, (long)f * d.nm * d.nh, d.nh, j, L, nt);
+  stage_tw4<P>(tws, d.tw, nt);
+  rc.scatter(a, N, L, nt);
   __syncthreads();
-  for (int m = threadIdx.x; m <= L; m += NT) {
-    double2 fn = FN[base + (long)m * d.nh + j], fs = FS[base + (long)m * d.nh + j];
-    if (m == 0 || 2 * m == N) {
-      a[m] = make_double2(fn.x, fs.x);
-    } else {
-      a[m] = make_double2(fn.x - fs.y, fn.y + fs.x);          // Fn + i Fs
-      a[N - m] = make_double2(fn.x + fs.y, -fn.y + fs.x);     // conj(Fn) + i conj(Fs)
-    }
-  }
-  __syncthreads();
-  double2* z = fft_lds<NT>(a, b, tws, N, d.log2n, true);
+  double2* z = fft_lds<P>(a, b, tws, N, d.log2n, true);
   const long rn = (long)j * N, rs = (long)js * N;
-  for (int i = threadIdx.x; i < N; i += NT) {
-    double2 v = z[i];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int i = tid + k * nt;
+    const double2 v = z[i];
     st(f, rn + i, v.x);
     st(f, rs + i, v.y);
   }
 }
-
 
 // -------------------------------------------------------------------------------------------
 // Analysis contraction on the matrix cores: v_mfma_f64_4x4x4_4b_f64 (four independent 4x4x4

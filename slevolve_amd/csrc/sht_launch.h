@@ -147,33 +147,46 @@ int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPt
 }  // namespace slx
 
 namespace slx {
-template <int NT, class Loader>
-int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {
-  dim3 grid(h->nh, nf);
-  const size_t lds = (2 * (size_t)h->nlon + h->nlon / 4) * sizeof(double2);
-  static size_t lds_allowed = 65536;
-  if (lds > lds_allowed) {
-    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_ana<NT, Loader>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    lds_allowed = lds;
+// Dynamic LDS above the default 64 KB is opted into once per kernel.
+template <auto KERN>
+int allow_lds(size_t lds) {
+  static size_t allowed = 65536;
+  if (lds > allowed) {
+    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(KERN), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    allowed = lds;
   }
-  k_fft_ana<NT, Loader><<<grid, NT, lds, s>>>(dev_of(h), ld, h->WS, h->WD, pr);
+  return 0;
+}
+
+// Ring passes: points per thread (4 up to nlon = 2048, else 8), LDS = two rows + the quarter
+// twiddle table.
+inline int ring_points(const slx_sht* h) { return h->nlon <= 4 * RING_MAXNT ? 4 : 8; }
+inline size_t ring_lds(const slx_sht* h) { return (2 * (size_t)h->nlon + h->nlon / 4) * sizeof(double2); }
+
+template <int P, class Loader>
+int launch_fft_ana_p(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {
+  const size_t lds = ring_lds(h);
+  if (int rc = allow_lds<&k_fft_ana<P, Loader>>(lds)) return rc;
+  k_fft_ana<P, Loader><<<dim3(h->nh, nf), h->nlon / P, lds, s>>>(dev_of(h), ld, h->WS, h->WD, pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }
-template <int NT, class Store>
-int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
-  dim3 grid(h->nh, nf);
-  const size_t lds = (2 * (size_t)h->nlon + h->nlon / 4) * sizeof(double2);
-  static size_t lds_allowed = 65536;
-  if (lds > lds_allowed) {
-    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_syn<NT, Store>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    lds_allowed = lds;
-  }
-  k_fft_syn<NT, Store><<<grid, NT, lds, s>>>(dev_of(h), h->FN, h->FSo, st, pr);
+template <class Loader>
+int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {
+  return ring_points(h) == 4 ? launch_fft_ana_p<4>(h, ld, nf, pr, s) : launch_fft_ana_p<8>(h, ld, nf, pr, s);
+}
+template <int P, class Store>
+int launch_fft_syn_p(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
+  const size_t lds = ring_lds(h);
+  if (int rc = allow_lds<&k_fft_syn<P, Store>>(lds)) return rc;
+  k_fft_syn<P, Store><<<dim3(h->nh, nf), h->nlon / P, lds, s>>>(dev_of(h), h->FN, h->FSo, st, pr);
   SLX_LAUNCH_CHECK();
   return 0;
+}
+template <class Store>
+int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
+  return ring_points(h) == 4 ? launch_fft_syn_p<4>(h, st, nf, pr, s) : launch_fft_syn_p<8>(h, st, nf, pr, s);
 }
 template <int NF, int R>
 int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long off, Pred pr, hipStream_t s) {

@@ -452,59 +452,58 @@ struct OceanFn {  // C* = C beta and tO = tinit (C* - C*0) of a grid point (solv
   __device__ __forceinline__ double tO(long i, double cs) const { return tinit[i] * (cs - cstar0[i]); }
 };
 
-template <int NT>
-__global__ void __launch_bounds__(NT) k_fft_synana(ShtDev d, const double2* __restrict__ FN,
-                                                   const double2* __restrict__ FS, double* __restrict__ dsl, OceanFn oc,
-                                                   double2* __restrict__ WS, double2* __restrict__ WD, int nf,
-                                                   const int* cst_flag, Pred pr) {
+template <int P>
+__global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const double2* __restrict__ FN,
+                                                           const double2* __restrict__ FS, double* __restrict__ dsl,
+                                                           OceanFn oc, double2* __restrict__ WS,
+                                                           double2* __restrict__ WD, int nf, const int* cst_flag,
+                                                           Pred pr) {
   if (!pr.active()) return;
   if (cst_flag) nf = *(volatile const int*)cst_flag ? 2 : 1;  // C* analysed only after a mask update
   extern __shared__ double2 smem[];
-  const int N = d.nlon, L = d.lmax;
+  const int N = d.nlon, L = d.lmax, nt = N / P, tid = threadIdx.x;
   double2* a = smem;
   double2* b = smem + N;
   double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
-  const int j = blockIdx.x, js = d.nlat - 1 - j;
+  const int j = xcd_ring(blockIdx.x, d.nh), js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
-  for (int i = threadIdx.x; i < N; i += NT) a[i] = make_double2(0.0, 0.0);
-  stage_tw4<NT>(tws, d.tw, N);
-  __syncthreads();
-  for (int m = threadIdx.x; m <= L; m += NT) {
-    const double2 fn = FN[(long)m * d.nh + j], fs = FS[(long)m * d.nh + j];
-    if (m == 0 || 2 * m == N) {
-      a[m] = make_double2(fn.x, fs.x);
-    } else {
-      a[m] = make_double2(fn.x - fs.y, fn.y + fs.x);
-      a[N - m] = make_double2(fn.x + fs.y, -fn.y + fs.x);
-    }
+  // every global read of the pass first: the ring pair's Fourier coefficients, then C* and tO of
+  // the thread's points of both rows
+  RingCoef<P> rc;
+  rc.load(FN, FS, 0, d.nh, j, L, nt);
+  double cn[P], cs[P], tn[P], ts[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const long i = tid + k * nt;
+    cn[k] = oc.cstar(rn + i);
+    cs[k] = oc.cstar(rs + i);
+    tn[k] = oc.tO(rn + i, cn[k]);
+    ts[k] = oc.tO(rs + i, cs[k]);
   }
+  stage_tw4<P>(tws, d.tw, nt);
+  rc.scatter(a, N, L, nt);
   __syncthreads();
-  double2* z = fft_lds<NT>(a, b, tws, N, d.log2n, true);
+  double2* z = fft_lds<P>(a, b, tws, N, d.log2n, true);
   double2* w = (z == a) ? b : a;
-  for (int i = threadIdx.x; i < N; i += NT) {
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int i = tid + k * nt;
     const double2 v = z[i];
     dsl[rn + i] = v.x;
     dsl[rs + i] = v.y;
-    const double cn = oc.cstar(rn + i), cs = oc.cstar(rs + i);
-    z[i] = make_double2(v.x * cn - oc.tO(rn + i, cn), v.y * cs - oc.tO(rs + i, cs));
+    z[i] = make_double2(v.x * cn[k] - tn[k], v.y * cs[k] - ts[k]);
   }
   __syncthreads();
   const double wn = d.wn[j], ws = d.ws[j];
   const long fstride = (long)d.nm * d.nh;
   for (int f = 0; f < nf; ++f) {
     if (f == 1) {
-      for (int i = threadIdx.x; i < N; i += NT) z[i] = make_double2(oc.cstar(rn + i), oc.cstar(rs + i));
+#pragma unroll
+      for (int k = 0; k < P; ++k) z[tid + k * nt] = make_double2(cn[k], cs[k]);
       __syncthreads();
     }
-    double2* Z = fft_lds<NT>(z, w, tws, N, d.log2n, false);
-    for (int m = threadIdx.x; m <= L; m += NT) {
-      const double2 zm = Z[m], zc = Z[(N - m) & (N - 1)];
-      const double2 fn = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y - zc.y));
-      const double2 fs = make_double2(0.5 * (zm.y + zc.y), -0.5 * (zm.x - zc.x));
-      const double2 tn = make_double2(fn.x * wn, fn.y * wn), ts = make_double2(fs.x * ws, fs.y * ws);
-      WS[f * fstride + (long)m * d.nh + j] = cadd(tn, ts);
-      WD[f * fstride + (long)m * d.nh + j] = csub(tn, ts);
-    }
+    double2* Z = fft_lds<P>(z, w, tws, N, d.log2n, false);
+    ring_fold(Z, WS, WD, f * fstride, d.nh, j, L, N, wn, ws, nt);
     __syncthreads();
     if (Z != z) {  // keep z as the buffer the next field is loaded into
       double2* t = z;
@@ -607,9 +606,9 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
 
 // The grounded-ice re-guess of an inner iteration, applied by the ice ring pass
 // (solver.py:661-684): gate() accepts it when a grounded flag flipped on the topography iterate
-// (at most GR_MAXACC times per step) and block 0 records the bookkeeping; the loader stores
-// I* = ice where (topo > 0) or ice > |topo| rho_w / rho_i, else 0, and beta = (I* < eps) as the
-// pass reads them.
+// (at most GR_MAXACC times per step) and block 0 records the bookkeeping; the loader reads
+// I* = ice where (topo > 0) or ice > |topo| rho_w / rho_i, else 0, and its commit stores I* and
+// beta = (I* < eps).
 struct LoadReguess {
   const double *ice, *topo;
   double *icestar, *beta;
@@ -634,10 +633,11 @@ struct LoadReguess {
   }
   __device__ double operator()(int, long i) const {
     const double v = ice[i];
-    const double cand = v * keep_factor(v, topo[i], rhow, rhoi);
+    return v * keep_factor(v, topo[i], rhow, rhoi);
+  }
+  __device__ void commit(int, long i, double cand) const {
     icestar[i] = cand;
     beta[i] = cand < EPS0 ? 1.0 : 0.0;
-    return cand;
   }
 };
 
@@ -1583,7 +1583,7 @@ extern "C" int slx_solver_init(slx_solver* s, const double* topo0, const double*
   SLX_LAUNCH_CHECK();
   // degree-0 areas of C0, C*0 and I*0 (solver.py:571-580); the I*0 spectrum is the "previous ice"
   // of step 1 (solver.py:859-862)
-  int rc = launch_fft_ana<256>(s->h, LoadPtrs<3>{{s->cxy0, s->cstar0, s->tmp}}, 3, Pred{}, st);
+  int rc = launch_fft_ana(s->h, LoadPtrs<3>{{s->cxy0, s->cstar0, s->tmp}}, 3, Pred{}, st);
   if (rc) return rc;
   rc = launch_leg_ana<3>(s->h, s->h->WS, s->h->WD,
                          OutPtrs<3>{{s->scratch_lm, s->scratch_lm + s->c.ncoef, s->oldice_lm}}, Pred{}, st);
@@ -1623,7 +1623,7 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
     SLX_HIP(hipMemcpyAsync(s->cxy, warm_ocean, (size_t)c.npts * sizeof(double), hipMemcpyDeviceToDevice, st));
   k_ice_grid<<<grid_blocks(c.npts), 256, 0, st>>>(p, c);
   SLX_LAUNCH_CHECK();
-  rc = launch_fft_ana<256>(s->h, LoadPtrs<2>{{s->icestar, s->cstar}}, 2, Pred{}, st);
+  rc = launch_fft_ana(s->h, LoadPtrs<2>{{s->icestar, s->cstar}}, 2, Pred{}, st);
   if (rc) return rc;
   rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->icestar_lm, s->cstar_lm}}, Pred{}, st);
   if (rc) return rc;
@@ -1642,6 +1642,22 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
 // n inner iterations with Anderson mixing (solver_opt.py:532-535,705-775): the same ring and
 // Legendre passes (C* analysed only after a mask update), then the decision lane and the
 // kernels of the chosen branch, each gated by its iteration token.
+// ring pass of an inner iteration (k_fft_synana) on the solver's grids
+template <int P>
+static int launch_fft_synana_p(slx_solver* s, int nf, const int* cst, Pred pr, hipStream_t st) {
+  const slx_sht* h = s->h;
+  const size_t lds = ring_lds(h);
+  if (int rc = allow_lds<&k_fft_synana<P>>(lds)) return rc;
+  k_fft_synana<P><<<h->nh, h->nlon / P, lds, st>>>(dev_of(h), h->FN, h->FSo, s->dsl,
+                                                    OceanFn{s->cxy, s->beta, s->tinit, s->cstar0}, h->WS, h->WD, nf, cst,
+                                                    pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+static int launch_fft_synana(slx_solver* s, int nf, const int* cst, Pred pr, hipStream_t st) {
+  return ring_points(s->h) == 4 ? launch_fft_synana_p<4>(s, nf, cst, pr, st) : launch_fft_synana_p<8>(s, nf, cst, pr, st);
+}
+
 static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
   const Consts& c = s->c;
   const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
@@ -1657,17 +1673,8 @@ static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
                s->st, c, 1};
     int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
     if (rc) return rc;
-    const size_t lds = (2 * (size_t)c.nlon + c.nlon / 4) * sizeof(double2);
-    static size_t lds_allowed = 65536;
-    if (lds > lds_allowed) {
-      SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_synana<256>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      lds_allowed = lds;
-    }
-    k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl,
-                                                   OceanFn{s->cxy, s->beta, s->tinit, s->cstar0}, s->h->WS, s->h->WD,
-                                                   1, cst, Pred{done, 0});
-    SLX_LAUNCH_CHECK();
+    rc = launch_fft_synana(s, 1, cst, Pred{done, 0}, st);
+    if (rc) return rc;
     rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0, cst, 1}, st);
     if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0, cst, 0}, st);
     if (rc) return rc;
@@ -1680,7 +1687,7 @@ static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
     k_refresh2_aa<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
     if (c.checkmarine) {
-      rc = launch_fft_ana<256>(s->h, LoadPtrs<1>{{s->icestar}}, 1, Pred{done, 0, changed, 1}, st);
+      rc = launch_fft_ana(s->h, LoadPtrs<1>{{s->icestar}}, 1, Pred{done, 0, changed, 1}, st);
       if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
       if (rc) return rc;
     }
@@ -1713,20 +1720,8 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
                s->st, c, 1};
     int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
     if (rc) return rc;
-    {
-      const int nf = first ? 1 : 2;
-      const size_t lds = (2 * (size_t)c.nlon + c.nlon / 4) * sizeof(double2);
-      static size_t lds_allowed = 65536;
-      if (lds > lds_allowed) {
-        SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_synana<256>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        lds_allowed = lds;
-      }
-      k_fft_synana<256><<<s->h->nh, 256, lds, st>>>(dev_of(s->h), s->h->FN, s->h->FSo, s->dsl,
-                                                     OceanFn{s->cxy, s->beta, s->tinit, s->cstar0}, s->h->WS,
-                                                     s->h->WD, nf, nullptr, Pred{done, 0});
-      SLX_LAUNCH_CHECK();
-    }
+    rc = launch_fft_synana(s, first ? 1 : 2, nullptr, Pred{done, 0}, st);
+    if (rc) return rc;
     if (first)
       rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
     else
@@ -1738,7 +1733,7 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     SLX_LAUNCH_CHECK();
     if (c.checkmarine) {
       const LoadReguess ld{s->ice_cur, s->topo, s->icestar, s->beta, c.rhow, c.rhoi, s->st, it_step, c.checkmarine};
-      rc = launch_fft_ana<256>(s->h, ld, 1, Pred{done, 0}, st);
+      rc = launch_fft_ana(s->h, ld, 1, Pred{done, 0}, st);
       if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
       if (rc) return rc;
     }
@@ -1777,7 +1772,7 @@ extern "C" int slx_step_finish_readouts(slx_solver* s, const slx_readouts* ro, v
                s->deg + c.nl, s->st, c, 0};
     int rc = launch_leg_syn<1>(s->h, cb, fin, st);
     if (rc) return rc;
-    rc = launch_fft_syn<256>(s->h, StorePlain{s->rr, 0}, 1, fin, st);
+    rc = launch_fft_syn(s->h, StorePlain{s->rr, 0}, 1, fin, st);
     if (rc) return rc;
   }
   double2* dsl_user = rates ? static_cast<double2*>(ro->dSLdt_lm) : nullptr;
@@ -1798,7 +1793,7 @@ extern "C" int slx_step_finish_readouts(slx_solver* s, const slx_readouts* ro, v
     if (rates) {  // rate grids without the degree-0 term (solver.py:797-798)
       int rc = launch_leg_syn<2>(s->h, CoefPlain{s->scratch_lm, c.ncoef}, fin, st);
       if (rc) return rc;
-      rc = launch_fft_syn<256>(s->h, StorePair{ro->dSLdt, ro->dRdt}, 2, fin, st);
+      rc = launch_fft_syn(s->h, StorePair{ro->dSLdt, ro->dRdt}, 2, fin, st);
       if (rc) return rc;
     }
   }

@@ -50,7 +50,8 @@ class SHT:
         _lib.check(lib.slx_sht_info(h, dims, x.ctypes.data_as(_lib.P_double), w.ctypes.data_as(_lib.P_double)))
         self.ncoef = int(dims[3])
         self.gaulats, self.weights = x, w
-        # kernels the handle chose at creation (autotuned on this device)
+        # kernels the handle chose at creation (a fixed rule per shape unless SLX_ANA_VARIANT /
+        # SLX_SYN_VARIANT ask otherwise)
         self.kernels = {"analysis": {-1: "k_leg_ana", 0: "k_leg_ana_mf", 1: "k_leg_ana_ck"}[int(dims[5])],
                         "analysis_rings_per_lane": int(dims[6]),
                         "synthesis": "k_leg_syn_mf" if dims[7] else "k_leg_syn_s", "synthesis_rings_per_lane": int(dims[8])}

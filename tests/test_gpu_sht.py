@@ -106,3 +106,37 @@ def test_modal_state_bit_exact(prem_text):
         dil, dm, dl = rng.standard_normal((3, 3)), rng.standard_normal(3), rng.standard_normal(3) + 0j
         a.commit(load, dil, dm, dl)
         b.commit(load, dil, dm, dl)
+
+
+@pytest.mark.parametrize("nlat,nlon,L", [(64, 4096, 63), (16, 8, 3)])
+def test_ring_passes_at_the_row_length_limits(nlat, nlon, L):
+    """The ring passes run nlon / 4 threads per ring pair up to nlon = 2048 and nlon / 8 at 4096;
+    the widest and the narrowest rows against the oracle."""
+    from slevolve_amd import SHT
+
+    print(f"[sht] ring passes nlat {nlat} nlon {nlon} lmax {L}", flush=True)
+    rng = np.random.default_rng(nlon + L)
+    gpu, ref = SHT(nlat, nlon, L), OracleSHT(nlat, nlon, L)
+    c = rand_coeffs(L, rng)
+    g_ref = ref.synthesis(c)
+    g_gpu = gpu.synthesis(torch.from_numpy(c).cuda()).cpu().numpy()
+    assert np.abs(g_gpu - g_ref).max() < 1e-13 * np.abs(g_ref).max()
+    field = rng.standard_normal((nlat, nlon))
+    assert np.abs(gpu.analysis(torch.from_numpy(field).cuda()).cpu().numpy() - ref.analysis(field)).max() < 1e-14
+
+
+def test_kernel_choice_is_a_rule_per_shape(monkeypatch):
+    """Without SLX_ANA_VARIANT / SLX_SYN_VARIANT every handle of a shape takes the same kernels
+    (2 rings per lane up to nh = 256), so two handles give bit-identical transforms."""
+    from slevolve_amd import SHT
+
+    monkeypatch.delenv("SLX_ANA_VARIANT", raising=False)
+    monkeypatch.delenv("SLX_SYN_VARIANT", raising=False)
+    a, b = SHT(128, 256, 127), SHT(128, 256, 127)
+    assert a.kernels == b.kernels == {"analysis": "k_leg_ana_ck", "analysis_rings_per_lane": 2,
+                                      "synthesis": "k_leg_syn_mf", "synthesis_rings_per_lane": 2}
+    rng = np.random.default_rng(3)
+    field = torch.from_numpy(rng.standard_normal((128, 256))).cuda()
+    assert torch.equal(a.analysis(field), b.analysis(field))
+    c = torch.from_numpy(rand_coeffs(127, rng)).cuda()
+    assert torch.equal(a.synthesis(c), b.synthesis(c))
