@@ -171,10 +171,13 @@ int slx_step_finish_readouts(slx_solver* s, const slx_readouts* r, void* stream)
 int slx_solver_status(slx_solver* s, slx_status* out, void* stream);
 /* The step loop of run_ais.py:389-425 (SeaLevelSolver.step called per ice file) in native code:
  * steps k = 0..nsteps-1 with ice grid ice + k * ice_stride (device) and length dts[k] (host),
- * dt_prev = the lengths of the two steps before the first (host), *guess = inner iterations
- * enqueued in the first batch (updated to the last step's count); out[k] (host) = the status of
- * step k after its finish.  Outputs of intermediate steps are overwritten; the solver's grids
- * hold the last step's. */
+ * dt_prev = the lengths of the two steps before the first (host); *guess is set to the last
+ * step's inner-iteration count; out[k] (host) = the status of step k after its finish.  Each inner
+ * iteration is enqueued when the device has posted the decision of the previous one (to pinned host
+ * memory, as the deciding kernel starts), so no iteration runs past convergence and the host never
+ * waits for the stream to drain before the last step.  Outputs of intermediate steps are
+ * overwritten; the solver's grids hold the last step's.  Returns -34 (out[0..k] filled) when step
+ * k's inner loop fails to converge. */
 int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, int nsteps, const double* dts,
                   const double* dt_prev, int* guess, slx_status* out, void* stream);
 int slx_solver_fields(slx_solver* s, slx_fields* out);

@@ -106,6 +106,13 @@ struct slx_solver {
   double dt = -1.0;
   const double* ice_cur = nullptr;  // ice grid of the step in progress (device)
   int closed = 1;                   // host bookkeeping of the last step done
+  // native step loop: per-iteration decision flags [MAX_INNER] (coherent pinned host memory, host
+  // and device views), whether the kernels post them, and the device view of the status slot the
+  // finishing step is copied to
+  int* hflag = nullptr;
+  int* hflag_dev = nullptr;
+  int pipelined = 0;
+  slx_status* hstat_dev = nullptr;
 };
 
 // ================================================================================== kernels
@@ -129,7 +136,19 @@ struct Pts {  // all device pointers a kernel may need, passed by value
   const double *s, *omphi, *w_sl, *w_rr, *a_hl, *a_thl, *ttlle, *deg;
   double2 *aaX, *aaG, *aaF, *gnew;
   double* aa_part;
+  // native step loop: decision flag of each inner iteration (token << 2 | 1 continue, 2 converged,
+  // 3 failed) and the finishing step's status, both in coherent pinned host memory (null otherwise)
+  int* hflag;
+  int htoken;
+  slx_status* hstat;
 };
+
+// Posts an inner iteration's decision to the host (system-scope store into pinned memory).
+__device__ __forceinline__ void post_decision(const Pts& p, int it, bool done, bool failed) {
+  if (p.hflag)
+    __hip_atomic_store(p.hflag + it, (p.htoken << 2) | (failed ? 3 : done ? 2 : 1), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 __device__ __forceinline__ double keep_factor(double ice, double topo, double rhow, double rhoi) {
   // (topo > 0) | (ice > |topo| rho_w / rho_i), as a 0/1 multiplier (solver.py:655-660)
@@ -573,6 +592,7 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
   const bool conv = xi <= c.eps1;
   const bool fail = !conv && it + 1 >= MAX_INNER;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    post_decision(p, it, conv || fail, fail);
     s->st.numiter = it + 1;
     s->st.xi = xi;
     s->st.conserv = conserv;
@@ -724,6 +744,7 @@ __global__ void k_decide_aa(Pts p, Consts c, int it) {
     s->st.done = 1;
     mask = mix = push = accept = false;
   }
+  post_decision(p, it, s->st.done != 0, s->st.failed != 0);
   if (push) {
     s->aa_head = (s->aa_head + 1) % (c.aa_depth + 1);
     s->aa_n = min(s->aa_n + 1, c.aa_depth + 1);
@@ -1049,23 +1070,32 @@ __global__ void __launch_bounds__(256) k_finish(Pts p, Consts c, int rates, doub
       p.r2[idx] += p.omphi[k * c.nl + 2] * d;
     }
   }
-  if (threadIdx.x != 0) return;
-  if (rates) {
-    const double d00 = ((-(c.rhoi / c.rhow) * s->dice00) / c.dt - q[4]) / q[3];
-    s->rate00 = d00;
-    dsl_lm[0] = make_double2(d00, 0.0);
+  if (threadIdx.x == 0) {
+    if (rates) {
+      const double d00 = ((-(c.rhoi / c.rhow) * s->dice00) / c.dt - q[4]) / q[3];
+      s->rate00 = d00;
+      dsl_lm[0] = make_double2(d00, 0.0);
+    }
+    s->st.gmslc_ocn = q[0] / q[1];
+    s->st.gmslc_ocnBeta = q[2] / q[3];
+    s->st.ocean_area = q[1] * c.area;
+    s->st.oceanBeta_area = q[3] * c.area;
+    s->st.ice_volume = p.icestar_lm[0].x * c.area;
+    if (c.tpw) {
+      for (int k = 0; k < 9; ++k) s->p_il[k] += s->dil[k];
+      for (int k = 0; k < 3; ++k) s->p_m[k] += s->dm[k];
+      for (int k = 0; k < 6; ++k) s->p_lam[k] += s->dlam[k];
+    }
+    if (close) s->st.finished = 1;
   }
-  s->st.gmslc_ocn = q[0] / q[1];
-  s->st.gmslc_ocnBeta = q[2] / q[3];
-  s->st.ocean_area = q[1] * c.area;
-  s->st.oceanBeta_area = q[3] * c.area;
-  s->st.ice_volume = p.icestar_lm[0].x * c.area;
-  if (c.tpw) {
-    for (int k = 0; k < 9; ++k) s->p_il[k] += s->dil[k];
-    for (int k = 0; k < 3; ++k) s->p_m[k] += s->dm[k];
-    for (int k = 0; k < 6; ++k) s->p_lam[k] += s->dlam[k];
+  if (close && p.hstat) {  // the final status into the host slot, one word per lane
+    __syncthreads();
+    constexpr int nw = (int)(sizeof(slx_status) / sizeof(int));
+    static_assert(nw <= 256, "status larger than a block");
+    if (threadIdx.x < nw)
+      __hip_atomic_store(reinterpret_cast<int*>(p.hstat) + threadIdx.x,
+                         reinterpret_cast<const int*>(&s->st)[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  if (close) s->st.finished = 1;
 }
 
 // ---------------------------------------------------------------------- read-outs
@@ -1247,6 +1277,9 @@ __global__ void k_init_areas(Pts p, Consts c, const double2* c0, const double2* 
 // ============================================================================== host side
 static Pts make_pts(slx_solver* s, const double* ice_cur, const double2* oldice) {
   Pts p;
+  p.hflag = s->pipelined ? s->hflag_dev : nullptr;
+  p.htoken = s->step & 0x1fffffff;
+  p.hstat = s->hstat_dev;
   p.st = s->st;
   p.partial = s->partial;
   p.tinit = s->tinit;
@@ -1481,6 +1514,9 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   SLX_HIP(hipMalloc(&s->R, (size_t)std::max(K, 1) * S));
   SLX_HIP(hipMemset(s->R, 0, (size_t)std::max(K, 1) * S));
   SLX_HIP(hipHostMalloc(&s->pinned, sizeof(slx_status), hipHostMallocDefault));
+  SLX_HIP(hipHostMalloc((void**)&s->hflag, MAX_INNER * sizeof(int), hipHostMallocCoherent));
+  std::memset(s->hflag, 0, MAX_INNER * sizeof(int));
+  SLX_HIP(hipHostGetDevicePointer((void**)&s->hflag_dev, s->hflag, 0));
   *out = s;
   return 0;
 }
@@ -1497,6 +1533,7 @@ extern "C" int slx_solver_destroy(slx_solver* s) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (s->pinned) (void)hipHostFree(s->pinned);
+  if (s->hflag) (void)hipHostFree(s->hflag);
   delete s;
   return 0;
 }
@@ -1812,8 +1849,9 @@ extern "C" int slx_step_finish_readouts(slx_solver* s, const slx_readouts* ro, v
   return 0;
 }
 
-// Host bookkeeping once the step's finish has run on the device (status.finished): the newest
-// converged increment (written to h2 by the commit) becomes h1.
+// Host bookkeeping once the step's finish has run on the device (status.finished), or has been
+// enqueued after a posted convergence decision: the newest converged increment (written to h2 by
+// the commit) becomes h1.
 static void step_close(slx_solver* s) {
   if (s->extrap) {
     std::swap(s->h1, s->h2);
@@ -1832,44 +1870,110 @@ extern "C" int slx_solver_status(slx_solver* s, slx_status* out, void* stream) {
   return 0;
 }
 
-// Native step loop (the loop of run_ais.py:389-425 without per-step host outputs): per step, the
-// step length, begin, batches of inner iterations each followed by the predicated finish and one
-// status read, until the step has finished.  The first batch has *guess iterations (the previous
-// step's count); a short batch is followed by single iterations while fewer than 16 are enqueued,
-// then by batches of half the count so far.
+// Waits (spinning) until the device posts the decision of inner iteration it of the current step;
+// code = 1 continue, 2 converged, 3 failed.  The stream is polled now and then, so that work that
+// ended without posting (or an error) ends the wait.
+static int wait_decision(slx_solver* s, int it, hipStream_t st, int& code) {
+  const volatile int* f = s->hflag + it;
+  const int token = s->step & 0x1fffffff;
+  for (long spin = 1;; ++spin) {
+    int v = *f;
+    if ((v >> 2) == token && (v & 3)) {
+      code = v & 3;
+      return 0;
+    }
+    if ((spin & 4095) == 0) {
+      const hipError_t q = hipStreamQuery(st);
+      if (q == hipSuccess) {
+        v = *f;
+        if ((v >> 2) == token && (v & 3)) {
+          code = v & 3;
+          return 0;
+        }
+        slx_set_error("slx_run_steps: the stream drained without the decision of iteration " + std::to_string(it));
+        return -35;
+      }
+      if (q != hipErrorNotReady) {
+        slx_set_error(std::string("slx_run_steps: ") + hipGetErrorString(q));
+        return -36;
+      }
+    }
+    __builtin_ia32_pause();
+  }
+}
+
+// Native step loop (the loop of run_ais.py:389-425 without per-step host outputs), pipelined: the
+// kernel that decides an inner iteration (the refresh pass, or the Anderson branch kernel) posts
+// the decision to pinned host memory as it starts, and the host enqueues the next iteration only
+// on a "continue", while the rest of the current one still runs on the device.  No iteration is
+// enqueued past convergence and the host never waits for the device to drain between steps: on
+// "converged" the step's finish and the next step's start follow at once; each finish copies the
+// step's status into a pinned slot, read after the last step.  *guess is not needed by this
+// schedule (kept for the interface; set to the last step's count).
 extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, int nsteps, const double* dts,
                              const double* dt_prev, int* guess, slx_status* out, void* stream) {
   SLX_CHECK(s && ice && dts && dt_prev && guess && out, "slx_run_steps: null argument");
   SLX_CHECK(nsteps >= 0 && ice_stride >= s->c.npts, "slx_run_steps: bad nsteps or ice_stride");
+  if (nsteps == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  slx_status* hst = nullptr;
+  SLX_HIP(hipHostMalloc((void**)&hst, (size_t)nsteps * sizeof(slx_status), hipHostMallocCoherent));
+  std::memset(hst, 0, (size_t)nsteps * sizeof(slx_status));
+  slx_status* hst_dev = nullptr;
+  hipError_t e = hipHostGetDevicePointer((void**)&hst_dev, hst, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(hst);
+    SLX_HIP(e);
+  }
   double d1 = dt_prev[0], d2 = dt_prev[1];
-  int n = *guess > 0 ? *guess : 1;
-  for (int k = 0; k < nsteps; ++k) {
-    int rc = slx_solver_set_dt(s, dts[k]);
+  int rc = 0, failed_at = -1;
+  s->pipelined = 1;
+  for (int k = 0; k < nsteps && !rc; ++k) {
+    rc = slx_solver_set_dt(s, dts[k]);
     if (!rc) rc = slx_step_begin(s, ice + (long)k * ice_stride, dts[k], d1, d2, stream);
-    if (rc) return rc;
-    int total = 0, batch = n;
-    slx_status st;
-    for (;;) {
-      rc = slx_step_iterate(s, batch, stream);
-      if (!rc) rc = slx_step_finish(s, stream);
-      if (!rc) rc = slx_solver_status(s, &st, stream);
-      if (rc) return rc;
-      total += batch;
-      if (st.failed) {
-        out[k] = st;
-        slx_set_error("slx_run_steps: inner loop failed to converge at step " + std::to_string(st.step));
-        return -34;
-      }
-      if (st.finished) break;
-      batch = total < 16 ? 1 : std::max(1, total / 2);
+    if (!rc) rc = slx_step_iterate(s, 1, stream);
+    int code = 0;
+    for (int it = 0; !rc; ++it) {
+      rc = wait_decision(s, it, st, code);
+      if (rc || code != 1) break;
+      rc = slx_step_iterate(s, 1, stream);
     }
-    out[k] = st;
-    n = st.numiter > 0 ? st.numiter : 1;
+    if (rc) break;
+    s->hstat_dev = hst_dev + k;  // a failed step is finished too, as by the step-by-step path
+    rc = slx_step_finish(s, stream);
+    s->hstat_dev = nullptr;
+    if (!rc) step_close(s);
+    if (code == 3) {
+      failed_at = k;
+      break;
+    }
     d2 = d1;
     d1 = dts[k];
   }
-  *guess = n;
-  return 0;
+  s->pipelined = 0;
+  s->hstat_dev = nullptr;
+  const hipError_t se = hipStreamSynchronize(st);
+  if (!rc && se != hipSuccess) {
+    slx_set_error(std::string("slx_run_steps: ") + hipGetErrorString(se));
+    rc = -36;
+  }
+  if (!rc && failed_at >= 0) {
+    for (int k = 0; k <= failed_at; ++k) out[k] = hst[k];
+    slx_set_error("slx_run_steps: inner loop failed to converge at step " + std::to_string(out[failed_at].step));
+    rc = -34;
+  } else if (!rc) {
+    for (int k = 0; k < nsteps; ++k) {
+      out[k] = hst[k];
+      if (!out[k].finished) {
+        slx_set_error("slx_run_steps: step " + std::to_string(k) + " did not post its status");
+        rc = -37;
+        break;
+      }
+    }
+    if (!rc) *guess = std::max(1, out[nsteps - 1].numiter);
+  }
+  (void)hipHostFree(hst);
+  return rc;
 }
 
 extern "C" int slx_solver_fields(slx_solver* s, slx_fields* o) {

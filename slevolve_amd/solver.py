@@ -208,6 +208,7 @@ class SeaLevelSolver:
         self._dS = torch.empty((self.nl, self.nl), dtype=torch.complex128, device=dev)
         self._spec_tmp = torch.empty(sht.ncoef, dtype=torch.complex128, device=dev)
         self._guess_iters = 4
+        self._recent = []  # inner-iteration counts of the last four steps, newest first
         self._reuse, self._reuse_store = "off", {}
         self.reuse_stats = {"warm_steps": 0}
         self.prof = {}
@@ -280,7 +281,7 @@ class SeaLevelSolver:
             self.reuse_stats["warm_steps"] += 1
         else:
             _lib.call("slx_step_begin", h, ice.data_ptr(), self.dt, self._dt_hist[0], self._dt_hist[1], sp)
-        n = self._guess_iters
+        n = max([self._guess_iters] + self._recent)  # first batch: the largest count of the last four steps
         total = 0
         while True:
             _lib.call("slx_step_iterate", h, n, sp)
@@ -296,6 +297,7 @@ class SeaLevelSolver:
                 break
             n = 1 if total < 16 else max(1, total // 2)
         self._guess_iters = max(1, st.numiter)
+        self._recent = ([st.numiter] + self._recent)[:4]
         self._last_status = st
         self._dt_hist = [self.dt, self._dt_hist[0]]
         self.gr_counts = {"reguess": st.reguess, "ice_analyses": st.ice_analyses}
@@ -379,6 +381,7 @@ class SeaLevelSolver:
                   D.stream_ptr())
         self.iter += n
         self._guess_iters = max(1, guess.value)
+        self._recent = ([x.numiter for x in out][::-1] + self._recent)[:4]
         self.dt = float(d[-1])
         self._dt_hist = [float(d[-1]), float(d[-2]) if n >= 2 else self._dt_hist[0]]
         st = out[n - 1]
