@@ -145,11 +145,24 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       qg[idx] = (double)g;
     }
   }
-  std::vector<double2> tw(nlon);
+  // FFT twiddles: the quarter period exp(-2 pi i t / nlon), t < nlon / 4, from cos/sin; the other
+  // quadrants by exact rotations, (-i)^q; laid out per radix-4 stage (Ns > 1, in stage order) as
+  // W1[k], W2[k], W3[k] = exp(-2 pi i r k nlon / (4 Ns) / nlon), r = 1, 2, 3, k < Ns
+  std::vector<double2> tw4(nlon / 4), stw;
   const double pi2 = 8.0 * std::atan(1.0);
-  for (int t = 0; t < nlon; ++t) {
+  for (int t = 0; t < nlon / 4; ++t) {
     double a = -pi2 * (double)t / (double)nlon;
-    tw[t] = make_double2(std::cos(a), std::sin(a));
+    tw4[t] = make_double2(std::cos(a), std::sin(a));
+  }
+  auto twq = [&](int u) {
+    const int q = u / (nlon / 4), r = u % (nlon / 4);
+    const double2 w = tw4[r];
+    return q == 0 ? w : q == 1 ? make_double2(w.y, -w.x) : q == 2 ? make_double2(-w.x, -w.y) : make_double2(-w.y, w.x);
+  };
+  for (int Ns = (lg & 1) ? 2 : 1; Ns < nlon; Ns *= 4) {
+    if (Ns == 1) continue;
+    for (int r = 1; r <= 3; ++r)
+      for (int k = 0; k < Ns; ++k) stw.push_back(twq(r * k * (nlon / (4 * Ns))));
   }
   size_t wsz = (size_t)max_fields * nm * nh * sizeof(double2);
 #define ALLOC_COPY(ptr, vec)                                                                   \
@@ -168,7 +181,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   ALLOC_COPY(h->qc, qc);
   ALLOC_COPY(h->qg, qg);
   ALLOC_COPY(h->jb, jb);
-  ALLOC_COPY(h->tw, tw);
+  ALLOC_COPY(h->stw, stw);
   // degree segments.  With nlat <= 1024 the analysis runs on the matrix cores with 32-degree
   // segments in groups of 4 per block; otherwise one block per (order, segment) with S (a multiple
   // of 8) the largest of 256, 128, 64, 32 that still gives >= 2048 waves.  Synthesis chains are
@@ -355,7 +368,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
 
 extern "C" int slx_sht_destroy(slx_sht* h) {
   if (!h) return 0;
-  void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->tw,
+  void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->stw,
                   h->WS, h->WD, h->FN, h->FSo, h->seg_m, h->seg_l0, h->seg_first, h->seed_a, h->seed_b,
                   h->seed_s, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg, h->grp_desc};
   for (void* p : ptrs)

@@ -17,11 +17,13 @@ namespace slx {
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// complex products with their fused multiply-adds written out, so that every kernel rounds them
+// the same way whatever the compiler's contraction choices
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
-  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -(a.x * b.y)));
 }
 
 // -------------------------------------------------------------------------------------------
@@ -34,25 +36,13 @@ constexpr int RING_MAXNT = 512;
 
 // Complex FFT of length N = 2^log2N held in LDS (a), Stockham autosort with ping-pong buffer b:
 // one radix-2 stage first when log2N is odd, then radix-4 stages.  Forward uses
-// exp(-2 pi i k n / N), inverse exp(+...), both unnormalised.  tw4 = exp(-2 pi i t / N) for
-// t < N/4 (the callers stage it in LDS); the other quadrants follow by exact rotations:
-// exp(-2 pi i (t + q N/4) / N) = tw4[t] (-i)^q.  Returns the buffer holding the result.  Every
-// thread of the block (nt = N / P of them) must call it.
-__device__ __forceinline__ double2 tw_quarter(const double2* tw4, int u, int log2N) {
-  const int q = u >> (log2N - 2), r = u & ((1 << (log2N - 2)) - 1);
-  const double2 w = tw4[r];
-  return q == 0 ? w : q == 1 ? make_double2(w.y, -w.x) : q == 2 ? make_double2(-w.x, -w.y) : make_double2(-w.y, w.x);
-}
-
-// stage the quarter-period twiddle table (N/4 = (P/4) nt entries) in LDS
+// exp(-2 pi i k n / N), inverse exp(+...), both unnormalised.  stw is the per-stage twiddle table
+// (ShtDev::stw): for every radix-4 stage with Ns > 1, in stage order, W1[k], W2[k], W3[k] =
+// exp(-2 pi i r k N / (4 Ns) / N) for r = 1, 2, 3 and k < Ns, consecutive over k, so that a wave's
+// twiddle loads are contiguous (the table has fewer than N entries and stays in cache).  Returns
+// the buffer holding the result.  Every thread of the block (nt = N / P of them) must call it.
 template <int P>
-__device__ __forceinline__ void stage_tw4(double2* tw4, const double2* __restrict__ tw, int nt) {
-#pragma unroll
-  for (int r = 0; r < P / 4; ++r) tw4[threadIdx.x + r * nt] = tw[threadIdx.x + r * nt];
-}
-
-template <int P>
-__device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ tw, int N, int log2N,
+__device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ stw, int N, int log2N,
                             bool inverse) {
   const int tid = threadIdx.x, nt = N / P;
   int Ns = 1;
@@ -69,16 +59,15 @@ __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ 
     Ns = 2;
   }
   const int q = N / 4;
+  const double2* w = stw;  // this stage's W1 (W2, W3 follow, Ns apart)
   for (; Ns < N; Ns *= 4) {
-    const int tstride = N / (4 * Ns);
 #pragma unroll
     for (int r = 0; r < P / 4; ++r) {
       const int j = tid + r * nt;
       const int k = j & (Ns - 1);
       double2 v0 = a[j], v1 = a[j + q], v2 = a[j + 2 * q], v3 = a[j + 3 * q];
       if (Ns > 1) {
-        const int st = k * tstride;  // < N/4
-        const double2 w1 = tw[st], w2 = tw_quarter(tw, 2 * st, log2N), w3 = tw_quarter(tw, 3 * st, log2N);
+        const double2 w1 = w[k], w2 = w[Ns + k], w3 = w[2 * Ns + k];
         if (!inverse) {
           v1 = cmul(v1, w1);
           v2 = cmul(v2, w2);
@@ -105,6 +94,7 @@ __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ 
       b[o + 2 * Ns] = csub(s02, s13);
       b[o + 3 * Ns] = y3;
     }
+    if (Ns > 1) w += 3 * Ns;
     __syncthreads();
     double2* t = a; a = b; b = t;
   }
@@ -185,7 +175,7 @@ struct ShtDev {
   const double* s2m3;
   const int* moff;
   const int* jb;
-  const double2* tw;
+  const double2* stw;  // per-stage FFT twiddles (fft_lds)
   const int* l_of;  // degree of each packed index
   const int* m_of;  // order of each packed index
   const double* qc;  // scaled recurrence coefficients (matrix-core kernels)
@@ -242,7 +232,6 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_ana(ShtDev d, Loader ld, dou
   const int N = d.nlon, nt = N / P, tid = threadIdx.x;
   double2* a = smem;
   double2* b = smem + N;
-  double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
   const int j = xcd_ring(blockIdx.x, d.nh), f = blockIdx.y, js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
   double vn[P], vs[P];
@@ -251,7 +240,6 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_ana(ShtDev d, Loader ld, dou
     vn[k] = ld(f, rn + tid + k * nt);
     vs[k] = ld(f, rs + tid + k * nt);
   }
-  stage_tw4<P>(tws, d.tw, nt);
 #pragma unroll
   for (int k = 0; k < P; ++k) a[tid + k * nt] = make_double2(vn[k], vs[k]);
   if constexpr (has_commit<Loader>::value) {
@@ -262,7 +250,7 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_ana(ShtDev d, Loader ld, dou
     }
   }
   __syncthreads();
-  double2* Z = fft_lds<P>(a, b, tws, N, d.log2n, false);
+  double2* Z = fft_lds<P>(a, b, d.stw, N, d.log2n, false);
   ring_fold(Z, WS, WD, (long)f * d.nm * d.nh, d.nh, j, d.lmax, N, d.wn[j], d.ws[j], nt);
 }
 
@@ -499,14 +487,12 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_syn(ShtDev d, const double2*
   const int N = d.nlon, L = d.lmax, nt = N / P, tid = threadIdx.x;
   double2* a = smem;
   double2* b = smem + N;
-  double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
   const int j = xcd_ring(blockIdx.x, d.nh), f = blockIdx.y, js = d.nlat - 1 - j;
   RingCoef<P> rc;
   rc.load(FN, FS, (long)f * d.nm * d.nh, d.nh, j, L, nt);
-  stage_tw4<P>(tws, d.tw, nt);
   rc.scatter(a, N, L, nt);
   __syncthreads();
-  double2* z = fft_lds<P>(a, b, tws, N, d.log2n, true);
+  double2* z = fft_lds<P>(a, b, d.stw, N, d.log2n, true);
   const long rn = (long)j * N, rs = (long)js * N;
 #pragma unroll
   for (int k = 0; k < P; ++k) {

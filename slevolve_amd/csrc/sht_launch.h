@@ -24,7 +24,7 @@ inline ShtDev dev_of(const slx_sht* h) {
   d.s2m3 = h->s2m3;
   d.moff = h->moff;
   d.jb = h->jb;
-  d.tw = h->tw;
+  d.stw = h->stw;
   d.l_of = h->l_of;
   d.m_of = h->m_of;
   d.qc = h->qc;
@@ -159,10 +159,9 @@ int allow_lds(size_t lds) {
   return 0;
 }
 
-// Ring passes: points per thread (4 up to nlon = 2048, else 8), LDS = two rows + the quarter
-// twiddle table.
+// Ring passes: points per thread (4 up to nlon = 2048, else 8), LDS = two rows.
 inline int ring_points(const slx_sht* h) { return h->nlon <= 4 * RING_MAXNT ? 4 : 8; }
-inline size_t ring_lds(const slx_sht* h) { return (2 * (size_t)h->nlon + h->nlon / 4) * sizeof(double2); }
+inline size_t ring_lds(const slx_sht* h) { return 2 * (size_t)h->nlon * sizeof(double2); }
 
 template <int P, class Loader>
 int launch_fft_ana_p(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {

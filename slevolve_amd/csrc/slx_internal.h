@@ -55,7 +55,7 @@ struct slx_sht {
   int* moff = nullptr;    // [nm+1]
   int* l_of = nullptr;    // [ncoef] degree of each packed index
   int* jb = nullptr;      // [nm] first ring (from the pole) with non-negligible P~_lm, multiple of 64
-  double2* tw = nullptr;  // [nlon] exp(-2 pi i t / nlon)
+  double2* stw = nullptr;  // per-stage FFT twiddles (fft_lds), < nlon entries
   int* m_of = nullptr;    // [ncoef] order of each packed index
   // scaled recurrence of the matrix-core kernels, restarted at every 32-degree segment start
   // ls = max(l0, m + 2) from the unscaled state (P~_{ls-2}, P~_{ls-1}):

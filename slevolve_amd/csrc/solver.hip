@@ -483,7 +483,6 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const doubl
   const int N = d.nlon, L = d.lmax, nt = N / P, tid = threadIdx.x;
   double2* a = smem;
   double2* b = smem + N;
-  double2* tws = smem + 2 * N;  // quarter-period twiddles, staged once per block
   const int j = xcd_ring(blockIdx.x, d.nh), js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
   // every global read of the pass first: the ring pair's Fourier coefficients, then C* and tO of
@@ -499,10 +498,9 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const doubl
     tn[k] = oc.tO(rn + i, cn[k]);
     ts[k] = oc.tO(rs + i, cs[k]);
   }
-  stage_tw4<P>(tws, d.tw, nt);
   rc.scatter(a, N, L, nt);
   __syncthreads();
-  double2* z = fft_lds<P>(a, b, tws, N, d.log2n, true);
+  double2* z = fft_lds<P>(a, b, d.stw, N, d.log2n, true);
   double2* w = (z == a) ? b : a;
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -521,7 +519,7 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const doubl
       for (int k = 0; k < P; ++k) z[tid + k * nt] = make_double2(cn[k], cs[k]);
       __syncthreads();
     }
-    double2* Z = fft_lds<P>(z, w, tws, N, d.log2n, false);
+    double2* Z = fft_lds<P>(z, w, d.stw, N, d.log2n, false);
     ring_fold(Z, WS, WD, f * fstride, d.nh, j, L, N, wn, ws, nt);
     __syncthreads();
     if (Z != z) {  // keep z as the buffer the next field is loaded into
