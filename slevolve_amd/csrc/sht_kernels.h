@@ -157,9 +157,14 @@ struct Pred {
   const int* f1 = nullptr;
   int w1 = 0;
   __device__ __forceinline__ bool active() const {
-    if (f0 && ((*(volatile const int*)f0 != 0) != (w0 != 0))) return false;
-    if (f1 && ((*(volatile const int*)f1 != 0) != (w1 != 0))) return false;
-    return true;
+    if (!f0 && !f1) return true;
+    // both flags are loaded, from valid addresses, before either is tested (one memory round
+    // trip, not two); the flags were written by earlier kernels of the stream
+    const int* p0 = f0 ? f0 : f1;
+    const int* p1 = f1 ? f1 : f0;
+    const int v0 = __hip_atomic_load(p0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int v1 = __hip_atomic_load(p1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (!f0 || (v0 != 0) == (w0 != 0)) && (!f1 || (v1 != 0) == (w1 != 0));
   }
 };
 

@@ -635,8 +635,9 @@ struct LoadReguess {
   int it, checkmarine;
   __device__ bool gate() const {
     const int slot = it & 1;
-    const bool regs = checkmarine && st->nacc2[slot] < GR_MAXACC;
-    const bool changed = regs && st->nflip[slot] > 0;
+    const int nacc = st->nacc2[slot], nflip = st->nflip[slot];  // loaded together
+    const bool regs = checkmarine && nacc < GR_MAXACC;
+    const bool changed = regs && nflip > 0;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
       st->ice_changed = changed ? 1 : 0;
       st->nacc2[slot ^ 1] = st->nacc2[slot] + (changed ? 1 : 0);
