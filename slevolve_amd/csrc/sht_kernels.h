@@ -1117,9 +1117,12 @@ __device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], L
   }
 }
 
+// One field with 2 rings per lane is compiled for 5 waves per SIMD (91 VGPRs, no spills; the LDS of
+// 5 blocks fits a CU): 11 % faster than at the 4 waves its natural 98 registers allow.
 template <int NF, int R>
-__global__ void __launch_bounds__(256) k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride,
-                                                    double2* __restrict__ FN, double2* __restrict__ FS, Pred pr) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NF == 1 && R == 2) ? 5 : 1, 8)))
+k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, double2* __restrict__ FN,
+             double2* __restrict__ FS, Pred pr) {
   static_assert(NF == 1 || NF == 2, "one or two fields per launch");
   if (!pr.active()) return;
   constexpr int S = MF_S;
