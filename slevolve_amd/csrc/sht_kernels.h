@@ -1063,8 +1063,9 @@ __global__ void __launch_bounds__(256) k_coef_fill(ShtDev d, Coef cf, double2* _
 // segments 4g .. 4g + 3 of one order), columns j = field x (re, im).  A lane walks the
 // recurrence of its ring through its slot; at step t the K index runs over the degrees
 // l0_k + t, all of the parity of t, so even and odd steps accumulate into separate E and O
-// outputs.  Block = (order pair p, L - p) x 16 R rings (R rings per lane, 4 waves); the segment
-// groups of both orders are dealt to the waves, whose E/O partials are summed in LDS in a fixed
+// outputs.  Block = (order pair p, L - p) x 16 R rings (R rings per lane, 4 waves; a 1-D grid dealt
+// so that the ring chunks of one order pair share an XCD); the segment groups of both orders are
+// dealt to the waves, whose E/O partials are summed in LDS in a fixed
 // order.  Recurrence and spectral coefficients of both orders are staged in LDS, zero past L.
 // Outputs FN = 1.41421356 (E + O), FS = 1.41421356 (E - O), [f][m][j].
 // Degree steps of one segment group (synthesis).  Parities of t go to separate accumulators
@@ -1132,7 +1133,11 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   double* qcl = reinterpret_cast<double*>(lsy);  // [2][nlp] scaled recurrence coefficients
   double2* cl = lsy + nlp;                       // [2][NF][nlp] coefficients times qg
   double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
-  const int p = blockIdx.x;
+  // XCD-aware block order: block b runs on XCD b % 8; the ny ring chunks of one order pair go to
+  // the same XCD, so that its L2 fetches the pair's staged coefficients once
+  const int ny = (nh + 16 * R - 1) / (16 * R), bt = blockIdx.x >> 3;
+  const int yb = bt % ny, p = (blockIdx.x & 7) + 8 * (bt / ny);
+  if (p >= (L + 2) / 2) return;
   const int mord[2] = {p, L - p};
   const int nord = (L - p == p) ? 1 : 2;
   for (int o = 0; o < 2; ++o) {
@@ -1159,7 +1164,7 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   double xr[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    jr[r] = blockIdx.y * 16 * R + r * 16 + 4 * blk + ri;
+    jr[r] = yb * 16 * R + r * 16 + 4 * blk + ri;
     okr[r] = jr[r] < nh;
     xr[r] = d.x[okr[r] ? jr[r] : 0];
   }
@@ -1244,7 +1249,7 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
       e += part[(((w * 2 + o) * 2 + 0) * R + r) * 64 + lane];
       q += part[(((w * 2 + o) * 2 + 1) * R + r) * 64 + lane];
     }
-    const int j = blockIdx.y * 16 * R + r * 16 + 4 * blk + rowr;
+    const int j = yb * 16 * R + r * 16 + 4 * blk + rowr;
     if (j < nh) {
       const long idx = fb * fstride + (long)m * nh + j;
       reinterpret_cast<double*>(FN)[2 * idx + comp] = SLX_SQRT2_LEGACY * (e + q);

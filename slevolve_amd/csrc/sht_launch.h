@@ -189,7 +189,9 @@ int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
 }
 template <int NF, int R>
 int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long off, Pred pr, hipStream_t s) {
-  dim3 grid((h->lmax + 2) / 2, (h->nh + 16 * R - 1) / (16 * R));
+  // 1-D grid: 8 XCD lanes x ring chunks x ceil(order pairs / 8) (blocks past the last pair exit)
+  const int ny = (h->nh + 16 * R - 1) / (16 * R), np = (h->lmax + 2) / 2;
+  const dim3 grid(8 * ny * ((np + 7) / 8));
   const int nl = h->lmax + 1, nlp = ((nl + 4 * MF_S - 1) / (4 * MF_S)) * (4 * MF_S);
   const size_t lds = std::max((size_t)nlp * sizeof(double2) + (size_t)2 * NF * nlp * sizeof(double2),
                               (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
