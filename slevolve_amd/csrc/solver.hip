@@ -337,19 +337,29 @@ __global__ void __launch_bounds__(64) k_begin(Pts p, Consts c) {
   }
 }
 
-// grounded ice on the previous converged topography, beta, C* (solver.py:853-887)
-__global__ void k_ice_grid(Pts p, Consts c) {
-  const long n = c.npts;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double ice = p.ice_cur[i];
-    double ist = c.checkmarine ? ice * keep_factor(ice, p.topo_prev[i], c.rhow, c.rhoi) : ice;
-    double b = ist < EPS0 ? 1.0 : 0.0;
-    double cs = p.cxy[i] * b;
-    p.icestar[i] = ist;
-    p.beta[i] = b;
-    p.cstar[i] = cs;
+// Grounded ice on the previous converged topography, beta, C* (solver.py:853-887), read by the
+// 2-field ring pass of slx_step_begin: field 0 = I* (the ice grid, marine-checked on the previous
+// step's topography), field 1 = C* = C beta with beta = (I* < eps); the field-0 blocks store I*
+// and beta, the field-1 blocks C*.
+struct LoadIceGrid {
+  const double *ice, *topo_prev, *cxy;
+  double *icestar, *beta, *cstar;
+  double rhow, rhoi;
+  int checkmarine;
+  __device__ double operator()(int f, long i) const {
+    const double v = ice[i];
+    const double ist = checkmarine ? v * keep_factor(v, topo_prev[i], rhow, rhoi) : v;
+    return f == 0 ? ist : cxy[i] * (ist < EPS0 ? 1.0 : 0.0);
   }
-}
+  __device__ void commit(int f, long i, double val) const {
+    if (f == 0) {
+      icestar[i] = val;
+      beta[i] = val < EPS0 ? 1.0 : 0.0;
+    } else {
+      cstar[i] = val;
+    }
+  }
+};
 
 // first guess of the ocean-load increment (solver.py:888-892; solver_opt.py:504-527);
 // so = sum |load_inc|
@@ -1657,9 +1667,10 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
   if (rc) return rc;
   if (warm_ocean)
     SLX_HIP(hipMemcpyAsync(s->cxy, warm_ocean, (size_t)c.npts * sizeof(double), hipMemcpyDeviceToDevice, st));
-  k_ice_grid<<<grid_blocks(c.npts), 256, 0, st>>>(p, c);
-  SLX_LAUNCH_CHECK();
-  rc = launch_fft_ana(s->h, LoadPtrs<2>{{s->icestar, s->cstar}}, 2, Pred{}, st);
+  rc = launch_fft_ana(s->h,
+                      LoadIceGrid{ice_cur, s->topo_prev, s->cxy, s->icestar, s->beta, s->cstar, c.rhow, c.rhoi,
+                                  c.checkmarine},
+                      2, Pred{}, st);
   if (rc) return rc;
   rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->icestar_lm, s->cstar_lm}}, Pred{}, st);
   if (rc) return rc;
