@@ -877,6 +877,106 @@ k_leg_ana_ck(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, cons
   }
 }
 
+// Incremental analysis of one field over a sparse set of rings: out += the analysis of the folded
+// spectra in WS/WD field 0 restricted to the rings whose ring_tok equals token (a grounded re-guess
+// changes I* at a few grid points; its ring pass writes those rings' spectra of the change).  One
+// wave per block, persistent over the groups in the XCD-aware order of k_leg_ana_ck; the listed
+// rings (ascending) are walked 16 R at a time, their partial sums added in that order, and each
+// coefficient is updated by the one lane that emits it.  Same degree steps as k_leg_ana_ck.
+template <int R>
+__global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS,
+                                                  const double2* __restrict__ WD, double2* __restrict__ out,
+                                                  const int* __restrict__ ring_tok, int token, Pred pr) {
+  if (!pr.active()) return;
+  constexpr int S = MF_S;
+  __shared__ double redw[8 * 64];  // the wave's partials of one ring chunk
+  __shared__ double qcs[4 * S];
+  __shared__ double qgs[4 * S];
+  __shared__ int list[512];  // nh <= 512 on the matrix-core path
+  const int L = d.lmax, nh = d.nh, lane = threadIdx.x;
+  int nact = 0;
+  for (int b0 = 0; b0 < nh; b0 += 64) {
+    const int b = b0 + lane;
+    const bool on = b < nh && ring_tok[b] == token;
+    const unsigned long long bal = __ballot(on);
+    if (on) list[nact + __popcll(bal & ((1ull << lane) - 1ull))] = b;
+    nact += __popcll(bal);
+  }
+  if (nact == 0) return;
+  __syncthreads();
+  const int per = (gr.ngrp + 7) >> 3;
+  const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
+  const int gend = min(gr.ngrp, (x + 1) * per);
+  const int blk = (lane >> 2) & 3, slot = lane & 3, kk = lane >> 4;
+  const bool colok = slot < 2;  // B columns of field 0: (re, im)
+  const int nchunk = (nact + 16 * R - 1) / (16 * R);
+  for (int gi = x * per + (blockIdx.x >> 3); gi < gend; gi += nbx) {
+    const int4 ds = gr.desc[gi];
+    const int m = ds.x, s0 = ds.y, mo = ds.w;
+    const int nsm = (L - m + S) / S;
+    for (int i = lane; i < 4 * S; i += 64) {
+      const int off = (s0 + i / S) * S + i % S;
+      const bool use = off <= L - m;
+      qcs[i] = use ? d.qc[mo + off] : 0.0;
+      qgs[i] = use ? d.qg[mo + off] : 0.0;
+    }
+    const int segi = s0 + slot;
+    const bool sval = segi < nsm;
+    const bool head = sval && segi == 0;
+    const long sx = (long)(ds.z + (sval ? slot : 0)) * nh;
+    const double s23 = d.s2m3[m];
+    __syncthreads();
+    double accv[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) accv[v] = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) {
+      double xr[R], wv[2][R];
+      LegState st[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int jl = ch * 16 * R + r * 16 + 4 * blk + kk;
+        const bool ok = jl < nact;
+        const int jj = ok ? list[jl] : 0;
+        xr[r] = d.x[jj];
+        const bool on = ok && sval;
+        double a0 = 0.0, b0 = 0.0;
+        int c0 = 0;
+        if (on && head) {
+          b0 = d.pmm[(long)m * nh + jj];
+          c0 = d.pms[(long)m * nh + jj];
+        } else if (on) {
+          a0 = g.sa[sx + jj];
+          b0 = g.sb[sx + jj];
+          c0 = g.ss[sx + jj];
+        }
+        mf_unscale(a0, b0, c0, segi + 1 < nsm ? g.ss[sx + nh + jj] : 0);
+        st[r] = LegState{a0, b0, 0};
+        const double2 vs = WS[(long)m * nh + jj], vd = WD[(long)m * nh + jj];
+        const bool use = ok && colok;
+        wv[0][r] = use ? ((slot & 1) ? vs.y : vs.x) : 0.0;
+        wv[1][r] = use ? ((slot & 1) ? vd.y : vd.x) : 0.0;
+      }
+      if (s0 == 0)
+        ana_ck_steps<R, true>(st, xr, wv, qcs, slot, head, s23, redw, lane);
+      else
+        ana_ck_steps<R, false>(st, xr, wv, qcs, slot, head, s23, redw, lane);
+#pragma unroll
+      for (int v = 0; v < 8; ++v) accv[v] += redw[v * 64 + lane];  // this lane's own slots
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int t = 8 * (v >> 1) + 4 * ((lane >> 3) & 1) + 2 * ((lane >> 2) & 1) + (v & 1);
+      const int row = lane >> 4, cj = lane & 3;
+      const int off = (s0 + row) * S + t;
+      if ((s0 + row) < nsm && off <= L - m && cj < 2) {
+        double* o = reinterpret_cast<double*>(out) + 2 * (long)(mo + off) + cj;
+        *o = *o + (accv[v] * qgs[row * S + t]) / SLX_SQRT2_LEGACY;
+      }
+    }
+    __syncthreads();  // the tables are refilled by the next group
+  }
+}
+
 // -------------------------------------------------------------------------------------------
 // Synthesis contraction with wave-uniform degree streams.  Block = (order pair m1 = p, m2 = L - p)
 // x 64 northern rings, 4 waves.  The chains (Ssyn-degree segments, seeded on the device) of the

@@ -112,6 +112,11 @@ struct slx_solver {
   int* hflag = nullptr;
   int* hflag_dev = nullptr;
   int pipelined = 0;
+  // incremental grounded re-guess (matrix-core path): per-ring token of the last re-analysis that
+  // changed the ring [nh], the next token, and whether the incremental form is used
+  int* ring_tok = nullptr;
+  int next_tok = 0;
+  int ice_incr = 0;
   slx_status* hstat_dev = nullptr;
 };
 
@@ -669,6 +674,44 @@ struct LoadReguess {
     beta[i] = cand < EPS0 ? 1.0 : 0.0;
   }
 };
+
+// Grounded re-guess ring pass, incremental form: when gate() accepts the re-guess (as for
+// LoadReguess), every block stores I* and beta of its ring pair exactly as LoadReguess does and
+// forms the change dI* = I*new - I*old; a ring pair whose rows did not change ends there, the
+// others transform the change, fold it into WS/WD field 0 (their own columns) and tag the ring
+// with token for k_leg_ana_sp, which adds the analysis of the change to I*_lm.
+template <int P>
+__global__ void __launch_bounds__(RING_MAXNT) k_reguess_ring(ShtDev d, LoadReguess lr, double2* __restrict__ WS,
+                                                             double2* __restrict__ WD, int* __restrict__ ring_tok,
+                                                             int token, Pred pr) {
+  if (!pr.active() || !lr.gate()) return;
+  extern __shared__ double2 smem[];
+  const int N = d.nlon, nt = N / P, tid = threadIdx.x;
+  double2* a = smem;
+  double2* b = smem + N;
+  const int j = xcd_ring(blockIdx.x, d.nh), js = d.nlat - 1 - j;
+  const long rn = (long)j * N, rs = (long)js * N;
+  double dn[P], dv[P];
+  int any = 0;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const long in = rn + tid + k * nt, is = rs + tid + k * nt;
+    const double on = lr.icestar[in], os = lr.icestar[is];
+    const double cn = lr(0, in), cs = lr(0, is);
+    lr.commit(0, in, cn);
+    lr.commit(0, is, cs);
+    dn[k] = cn - on;
+    dv[k] = cs - os;
+    any |= (dn[k] != 0.0 || dv[k] != 0.0) ? 1 : 0;
+  }
+  if (!__syncthreads_or(any)) return;
+#pragma unroll
+  for (int k = 0; k < P; ++k) a[tid + k * nt] = make_double2(dn[k], dv[k]);
+  __syncthreads();
+  double2* Z = fft_lds<P>(a, b, d.stw, N, d.log2n, false);
+  ring_fold(Z, WS, WD, 0, d.nh, j, d.lmax, N, d.wn[j], d.ws[j], nt);
+  if (tid == 0) ring_tok[j] = token;
+}
 
 // ---------------------------------------------------------------------- Anderson mixing
 // Inner iteration with inner = "anderson" (solver_opt.py:705-775): xi as above in every block;
@@ -1526,6 +1569,14 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   SLX_HIP(hipHostMalloc((void**)&s->hflag, MAX_INNER * sizeof(int), hipHostMallocCoherent));
   std::memset(s->hflag, 0, MAX_INNER * sizeof(int));
   SLX_HIP(hipHostGetDevicePointer((void**)&s->hflag_dev, s->hflag, 0));
+  // the incremental re-guess needs the matrix-core tables (nh <= 512); SLX_ICE_REANALYSIS=full
+  // keeps the full re-analysis
+  {
+    const char* e = std::getenv("SLX_ICE_REANALYSIS");
+    s->ice_incr = (h->ana_mfma && !(e && std::string(e) == "full")) ? 1 : 0;
+  }
+  SLX_HIP(hipMalloc((void**)&s->ring_tok, (size_t)h->nh * sizeof(int)));
+  SLX_HIP(hipMemset(s->ring_tok, 0, (size_t)h->nh * sizeof(int)));
   *out = s;
   return 0;
 }
@@ -1543,6 +1594,7 @@ extern "C" int slx_solver_destroy(slx_solver* s) {
     if (q) (void)hipFree(q);
   if (s->pinned) (void)hipHostFree(s->pinned);
   if (s->hflag) (void)hipHostFree(s->hflag);
+  if (s->ring_tok) (void)hipFree(s->ring_tok);
   delete s;
   return 0;
 }
@@ -1689,6 +1741,38 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
 // n inner iterations with Anderson mixing (solver_opt.py:532-535,705-775): the same ring and
 // Legendre passes (C* analysed only after a mask update), then the decision lane and the
 // kernels of the chosen branch, each gated by its iteration token.
+// incremental grounded re-guess: the ring pass of the change, then the sparse contraction (a
+// persistent grid of one-wave blocks, sized once to what is resident)
+template <int P>
+static int launch_reguess_ring_p(slx_solver* s, const LoadReguess& ld, int token, Pred pr, hipStream_t st) {
+  const slx_sht* h = s->h;
+  const size_t lds = ring_lds(h);
+  if (int rc = allow_lds<&k_reguess_ring<P>>(lds)) return rc;
+  k_reguess_ring<P><<<h->nh, h->nlon / P, lds, st>>>(dev_of(h), ld, h->WS, h->WD, s->ring_tok, token, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+static int launch_reguess_incremental(slx_solver* s, const LoadReguess& ld, Pred pr_ring, Pred pr_ana, hipStream_t st) {
+  const slx_sht* h = s->h;
+  const int token = ++s->next_tok;
+  int rc = ring_points(h) == 4 ? launch_reguess_ring_p<4>(s, ld, token, pr_ring, st)
+                               : launch_reguess_ring_p<8>(s, ld, token, pr_ring, st);
+  if (rc) return rc;
+  static int cap = 0;
+  if (cap == 0) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    SLX_HIP(hipGetDevice(&dev));
+    SLX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    SLX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_leg_ana_sp<2>), 64, 0));
+    cap = 8 * std::max(1, ncu * std::max(per_cu, 1) / 8);
+  }
+  const int grid = std::min(cap, 8 * ((h->ngrp + 7) / 8));
+  GrpDev gr{h->ngrp, h->grp_m, h->grp_s0, h->grp_desc};
+  k_leg_ana_sp<2><<<grid, 64, 0, st>>>(dev_of(h), seg_of(h), gr, h->WS, h->WD, s->icestar_lm, s->ring_tok, token, pr_ana);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
 // ring pass of an inner iteration (k_fft_synana) on the solver's grids
 template <int P>
 static int launch_fft_synana_p(slx_solver* s, int nf, const int* cst, Pred pr, hipStream_t st) {
@@ -1780,8 +1864,13 @@ extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
     SLX_LAUNCH_CHECK();
     if (c.checkmarine) {
       const LoadReguess ld{s->ice_cur, s->topo, s->icestar, s->beta, c.rhow, c.rhoi, s->st, it_step, c.checkmarine};
-      rc = launch_fft_ana(s->h, ld, 1, Pred{done, 0}, st);
-      if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
+      if (s->ice_incr) {  // the re-guess changes I* at a few points: analyse the change only
+        rc = launch_reguess_incremental(s, ld, Pred{done, 0}, Pred{done, 0, changed, 1}, st);
+      } else {
+        rc = launch_fft_ana(s->h, ld, 1, Pred{done, 0}, st);
+        if (!rc)
+          rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
+      }
       if (rc) return rc;
     }
   }
