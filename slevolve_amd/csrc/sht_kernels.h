@@ -894,12 +894,15 @@ __global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr
   __shared__ double qgs[4 * S];
   __shared__ int list[512];  // nh <= 512 on the matrix-core path
   const int L = d.lmax, nh = d.nh, lane = threadIdx.x;
+  int tk[8];  // every ring's token loaded first (one round trip), then listed
+#pragma unroll
+  for (int u = 0; u < 8; ++u) tk[u] = (u * 64 + lane < nh) ? ring_tok[u * 64 + lane] : 0;
   int nact = 0;
-  for (int b0 = 0; b0 < nh; b0 += 64) {
-    const int b = b0 + lane;
-    const bool on = b < nh && ring_tok[b] == token;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const bool on = u * 64 + lane < nh && tk[u] == token;
     const unsigned long long bal = __ballot(on);
-    if (on) list[nact + __popcll(bal & ((1ull << lane) - 1ull))] = b;
+    if (on) list[nact + __popcll(bal & ((1ull << lane) - 1ull))] = u * 64 + lane;
     nact += __popcll(bal);
   }
   if (nact == 0) return;

@@ -676,8 +676,8 @@ struct LoadReguess {
 };
 
 // Grounded re-guess ring pass, incremental form: when gate() accepts the re-guess (as for
-// LoadReguess), every block stores I* and beta of its ring pair exactly as LoadReguess does and
-// forms the change dI* = I*new - I*old; a ring pair whose rows did not change ends there, the
+// LoadReguess), every block stores I* and beta of its ring pair where they change (as LoadReguess
+// does everywhere) and forms the change dI* = I*new - I*old; a ring pair whose rows did not change ends there, the
 // others transform the change, fold it into WS/WD field 0 (their own columns) and tag the ring
 // with token for k_leg_ana_sp, which adds the analysis of the change to I*_lm.
 template <int P>
@@ -698,8 +698,9 @@ __global__ void __launch_bounds__(RING_MAXNT) k_reguess_ring(ShtDev d, LoadRegue
     const long in = rn + tid + k * nt, is = rs + tid + k * nt;
     const double on = lr.icestar[in], os = lr.icestar[is];
     const double cn = lr(0, in), cs = lr(0, is);
-    lr.commit(0, in, cn);
-    lr.commit(0, is, cs);
+    // I* and beta = (I* < eps) are always stored together, so only changed points need storing
+    if (cn != on) lr.commit(0, in, cn);
+    if (cs != os) lr.commit(0, is, cs);
     dn[k] = cn - on;
     dv[k] = cs - os;
     any |= (dn[k] != 0.0 || dv[k] != 0.0) ? 1 : 0;
