@@ -1835,45 +1835,59 @@ static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
   return 0;
 }
 
-extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
-  SLX_CHECK(s != nullptr, "null solver");
-  hipStream_t st = (hipStream_t)stream;
+// An inner iteration (extrapolated or plain start) in two parts: the head (B18 coefficients and the
+// synthesis contraction) and the tail (the fused ring pass through the grounded re-guess).  The
+// native step loop enqueues the next iteration's head before the current one's decision is known;
+// both parts are predicated on the device's done flag.
+static int iter_head(slx_solver* s, hipStream_t st) {
+  const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
+  CoefB18 cb{s->icestar_lm, oldice, s->p_ice, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->deg,
+             s->st, s->c, 1};
+  return launch_leg_syn<1>(s->h, cb, Pred{&s->st->st.done, 0}, st);
+}
+
+static int iter_tail(slx_solver* s, hipStream_t st) {
   const Consts& c = s->c;
   const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
   Pts p = make_pts(s, s->ice_cur, oldice);
   const int* done = &s->st->st.done;
   const int* changed = &s->st->ice_changed;
-  if (c.aa) return iterate_anderson(s, n, st);
-  for (int it = 0; it < n; ++it) {
-    const bool first = (s->iters_enqueued == 0);
-    const int it_step = s->iters_enqueued;
-    s->iters_enqueued += 1;
-    CoefB18 cb{s->icestar_lm, oldice, s->p_ice, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->deg,
-               s->st, c, 1};
-    int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
-    if (rc) return rc;
-    rc = launch_fft_synana(s, first ? 1 : 2, nullptr, Pred{done, 0}, st);
-    if (rc) return rc;
-    if (first)
-      rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
-    else
-      rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
-    if (rc) return rc;
-    k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step, s->load_inc);
-    SLX_LAUNCH_CHECK();
-    k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
-    SLX_LAUNCH_CHECK();
-    if (c.checkmarine) {
-      const LoadReguess ld{s->ice_cur, s->topo, s->icestar, s->beta, c.rhow, c.rhoi, s->st, it_step, c.checkmarine};
-      if (s->ice_incr) {  // the re-guess changes I* at a few points: analyse the change only
-        rc = launch_reguess_incremental(s, ld, Pred{done, 0}, Pred{done, 0, changed, 1}, st);
-      } else {
-        rc = launch_fft_ana(s->h, ld, 1, Pred{done, 0}, st);
-        if (!rc)
-          rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
-      }
-      if (rc) return rc;
+  const bool first = (s->iters_enqueued == 0);
+  const int it_step = s->iters_enqueued;
+  s->iters_enqueued += 1;
+  int rc = launch_fft_synana(s, first ? 1 : 2, nullptr, Pred{done, 0}, st);
+  if (rc) return rc;
+  if (first)
+    rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
+  else
+    rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
+  if (rc) return rc;
+  k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step, s->load_inc);
+  SLX_LAUNCH_CHECK();
+  k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
+  SLX_LAUNCH_CHECK();
+  if (c.checkmarine) {
+    const LoadReguess ld{s->ice_cur, s->topo, s->icestar, s->beta, c.rhow, c.rhoi, s->st, it_step, c.checkmarine};
+    if (s->ice_incr) {  // the re-guess changes I* at a few points: analyse the change only
+      rc = launch_reguess_incremental(s, ld, Pred{done, 0}, Pred{done, 0, changed, 1}, st);
+    } else {
+      rc = launch_fft_ana(s->h, ld, 1, Pred{done, 0}, st);
+      if (!rc)
+        rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
     }
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int slx_step_iterate(slx_solver* s, int n, void* stream) {
+  SLX_CHECK(s != nullptr, "null solver");
+  hipStream_t st = (hipStream_t)stream;
+  if (s->c.aa) return iterate_anderson(s, n, st);
+  for (int it = 0; it < n; ++it) {
+    int rc = iter_head(s, st);
+    if (!rc) rc = iter_tail(s, st);
+    if (rc) return rc;
   }
   return 0;
 }
@@ -2005,9 +2019,12 @@ static int wait_decision(slx_solver* s, int it, hipStream_t st, int& code) {
 // Native step loop (the loop of run_ais.py:389-425 without per-step host outputs), pipelined: the
 // kernel that decides an inner iteration (the refresh pass, or the Anderson branch kernel) posts
 // the decision to pinned host memory as it starts, and the host enqueues the next iteration only
-// on a "continue", while the rest of the current one still runs on the device.  No iteration is
-// enqueued past convergence and the host never waits for the device to drain between steps: on
-// "converged" the step's finish and the next step's start follow at once; each finish copies the
+// on a "continue", while the rest of the current one still runs on the device; the head of the
+// iteration after that (B18 coefficients and synthesis, iter_head) is enqueued ahead of the
+// decision as well, so the device need not wait for the host between iterations (after
+// convergence that head runs predicated off).  No iteration tail is enqueued past convergence and
+// the host never waits for the device to drain between steps: on "converged" the step's finish and
+// the next step's start follow at once; each finish copies the
 // step's status into a pinned slot, read after the last step.  *guess is not needed by this
 // schedule (kept for the interface; set to the last step's count).
 extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, int nsteps, const double* dts,
@@ -2031,12 +2048,24 @@ extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, 
   for (int k = 0; k < nsteps && !rc; ++k) {
     rc = slx_solver_set_dt(s, dts[k]);
     if (!rc) rc = slx_step_begin(s, ice + (long)k * ice_stride, dts[k], d1, d2, stream);
-    if (!rc) rc = slx_step_iterate(s, 1, stream);
     int code = 0;
-    for (int it = 0; !rc; ++it) {
-      rc = wait_decision(s, it, st, code);
-      if (rc || code != 1) break;
-      rc = slx_step_iterate(s, 1, stream);
+    if (s->c.aa) {  // Anderson mixing: one iteration enqueued per posted decision
+      if (!rc) rc = slx_step_iterate(s, 1, stream);
+      for (int it = 0; !rc; ++it) {
+        rc = wait_decision(s, it, st, code);
+        if (rc || code != 1) break;
+        rc = slx_step_iterate(s, 1, stream);
+      }
+    } else {  // iteration 0 and the head of iteration 1; then, per continue, a tail and the next head
+      if (!rc) rc = iter_head(s, st);
+      if (!rc) rc = iter_tail(s, st);
+      if (!rc) rc = iter_head(s, st);
+      for (int it = 0; !rc; ++it) {
+        rc = wait_decision(s, it, st, code);
+        if (rc || code != 1) break;
+        rc = iter_tail(s, st);
+        if (!rc) rc = iter_head(s, st);
+      }
     }
     if (rc) break;
     s->hstat_dev = hst_dev + k;  // a failed step is finished too, as by the step-by-step path
