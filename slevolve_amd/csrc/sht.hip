@@ -318,8 +318,9 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       h->ana_ck = 0;
     }
     // synthesis kernel: scalar-fed VALU, or matrix cores with 2 or 4 rings per lane (the latter
-    // only with the matrix-core segments).  By default a fixed rule per shape (matrix cores, 2
-    // rings per lane up to nh = 256, else 4; scalar without the matrix-core segments);
+    // only with the matrix-core segments).  By default a fixed rule (matrix cores with 2 rings per
+    // lane: its 16 KB of partial sums leave the packed staging area the LDS limit, 5 blocks per CU
+    // up to lmax 1023; scalar without the matrix-core segments);
     // SLX_SYN_VARIANT = scalar | mfma2 | mfma4 fixes the choice, mfma times the two ring counts and
     // auto all three.
     {
@@ -332,7 +333,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
       else if (want == "mfma4") cand = {{1, 4}};
       else if (want == "mfma") cand = {{1, 2}, {1, 4}};
       else if (want == "auto") cand = {{0, 4}, {1, 2}, {1, 4}};
-      else cand = {{1, nh <= 256 ? 2 : 4}};
+      else cand = {{1, 2}};
       float bs = 1e30f;
       Cand bc = cand[0];
       for (const Cand& c : cand) {

@@ -1169,7 +1169,8 @@ __global__ void __launch_bounds__(256) k_coef_fill(ShtDev d, Coef cf, double2* _
 // outputs.  Block = (order pair p, L - p) x 16 R rings (R rings per lane, 4 waves; a 1-D grid dealt
 // so that the ring chunks of one order pair share an XCD); the segment groups of both orders are
 // dealt to the waves, whose E/O partials are summed in LDS in a fixed
-// order.  Recurrence and spectral coefficients of both orders are staged in LDS, zero past L.
+// order.  Recurrence and spectral coefficients of both orders are staged in LDS, one order after
+// the other in groups of 4 S degrees, zero past each order's last degree.
 // Outputs FN = 1.41421356 (E + O), FS = 1.41421356 (E - O), [f][m][j].
 // Degree steps of one segment group (synthesis).  Parities of t go to separate accumulators
 // aE (even l - m) and aO (odd); consecutive MFMAs alternate between them.
@@ -1232,10 +1233,6 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   constexpr int S = MF_S;
   extern __shared__ double2 lsy[];
   const int L = d.lmax, nl = L + 1, nh = d.nh;
-  const int nlp = ((nl + 4 * S - 1) / (4 * S)) * (4 * S);
-  double* qcl = reinterpret_cast<double*>(lsy);  // [2][nlp] scaled recurrence coefficients
-  double2* cl = lsy + nlp;                       // [2][NF][nlp] coefficients times qg
-  double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
   // XCD-aware block order: block b runs on XCD b % 8; the ny ring chunks of one order pair go to
   // the same XCD, so that its L2 fetches the pair's staged coefficients once
   const int ny = (nh + 16 * R - 1) / (16 * R), bt = blockIdx.x >> 3;
@@ -1243,16 +1240,28 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   if (p >= (L + 2) / 2) return;
   const int mord[2] = {p, L - p};
   const int nord = (L - p == p) ? 1 : 2;
-  for (int o = 0; o < 2; ++o) {
-    const int m = mord[o], mo = d.moff[m], cnt = nl - m;
-    for (int i = threadIdx.x; i < nlp; i += 256) {
-      const bool v = (o < nord) && (i < cnt);
-      qcl[o * nlp + i] = v ? d.qc[mo + i] : 0.0;
-      const double gq = v ? d.qg[mo + i] : 0.0;
+  const int ng0 = ((nl - mord[0] + S - 1) / S + 3) / 4;
+  const int ng1 = nord == 2 ? ((nl - mord[1] + S - 1) / S + 3) / 4 : 0;
+  // staging, packed: order p's ng0 groups of 4 S degrees, then order L - p's ng1 groups, zero past
+  // each order's last degree; np2 = 4 S (ng0 + ng1) <= 4 S (ceil((L + 2) / (4 S)) + 1)
+  const int np2 = 4 * S * (ng0 + ng1), base1 = 4 * S * ng0;
+  double* qcl = reinterpret_cast<double*>(lsy);  // [np2] scaled recurrence coefficients
+  double2* cl = lsy + np2 / 2;                   // [NF][np2] coefficients times qg
+  double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
+  {
+    const int mo0 = d.moff[mord[0]], mo1 = d.moff[mord[1]];
+    const int cnt0 = nl - mord[0], cnt1 = nl - mord[1];
+    for (int i = threadIdx.x; i < np2; i += 256) {
+      const bool second = i >= base1;
+      const int k = second ? i - base1 : i;
+      const bool v = k < (second ? cnt1 : cnt0);
+      const int src = (second ? mo1 : mo0) + k;
+      qcl[i] = v ? d.qc[src] : 0.0;
+      const double gq = v ? d.qg[src] : 0.0;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const double2 c = v ? cf[f * cstride + mo + i] : make_double2(0.0, 0.0);
-        cl[(o * NF + f) * nlp + i] = make_double2(c.x * gq, c.y * gq);
+        const double2 c = v ? cf[f * cstride + src] : make_double2(0.0, 0.0);
+        cl[f * np2 + i] = make_double2(c.x * gq, c.y * gq);
       }
     }
   }
@@ -1271,8 +1280,6 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
     okr[r] = jr[r] < nh;
     xr[r] = d.x[okr[r] ? jr[r] : 0];
   }
-  const int ng0 = ((nl - mord[0] + S - 1) / S + 3) / 4;
-  const int ng1 = nord == 2 ? ((nl - mord[1] + S - 1) / S + 3) / 4 : 0;
   double EO[2][2][R];  // [order][E, O][ring]
 #pragma unroll
   for (int o = 0; o < 2; ++o)
@@ -1310,8 +1317,9 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
       aE[r] = aO[r] = 0.0;
     }
     const double s23 = d.s2m3[m];
-    const double* qco = qcl + o * nlp + sg * S;
-    const double* clo = reinterpret_cast<const double*>(cl + (o * NF + (colok ? fb : 0)) * nlp + sg * S) + comp;
+    const int ob = (o ? base1 : 0) + sg * S;
+    const double* qco = qcl + ob;
+    const double* clo = reinterpret_cast<const double*>(cl + (colok ? fb : 0) * np2 + ob) + comp;
     if (gq == 0)
       syn_mf_steps<R, true>(aE, aO, st, xr, qco, clo, colok, head, s23);
     else

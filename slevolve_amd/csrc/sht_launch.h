@@ -192,8 +192,10 @@ int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long o
   // 1-D grid: 8 XCD lanes x ring chunks x ceil(order pairs / 8) (blocks past the last pair exit)
   const int ny = (h->nh + 16 * R - 1) / (16 * R), np = (h->lmax + 2) / 2;
   const dim3 grid(8 * ny * ((np + 7) / 8));
-  const int nl = h->lmax + 1, nlp = ((nl + 4 * MF_S - 1) / (4 * MF_S)) * (4 * MF_S);
-  const size_t lds = std::max((size_t)nlp * sizeof(double2) + (size_t)2 * NF * nlp * sizeof(double2),
+  // staged entries of an order pair: ceil(n0 / 4S) + ceil(n1 / 4S) groups of 4S degrees with
+  // n0 + n1 = L + 2, at most ceil((L + 2) / 4S) + 1 groups
+  const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
+  const size_t lds = std::max((size_t)np2 * (sizeof(double) + NF * sizeof(double2)),
                               (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
   static size_t lds_allowed = 65536;
   if (lds > lds_allowed) {

@@ -127,7 +127,8 @@ def test_ring_passes_at_the_row_length_limits(nlat, nlon, L):
 
 def test_kernel_choice_is_a_rule_per_shape(monkeypatch):
     """Without SLX_ANA_VARIANT / SLX_SYN_VARIANT every handle of a shape takes the same kernels
-    (2 rings per lane up to nh = 256), so two handles give bit-identical transforms."""
+    (2 rings per lane for the synthesis; for the analysis up to nh = 256), so two handles give
+    bit-identical transforms."""
     from slevolve_amd import SHT
 
     monkeypatch.delenv("SLX_ANA_VARIANT", raising=False)
