@@ -227,6 +227,13 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
     SLX_LAUNCH_CHECK();
     SLX_HIP(hipDeviceSynchronize());
   }
+  if (h->ana_mfma) {
+    const long n = (long)h->nseg * nh;
+    SLX_HIP(hipMalloc(&h->seed_ab, (size_t)n * sizeof(double2)));
+    k_seed_unscale<<<(unsigned)((n + 255) / 256), 256>>>(dev_of(h), seg_of(h), h->seed_ab);
+    SLX_LAUNCH_CHECK();
+    SLX_HIP(hipDeviceSynchronize());
+  }
   // matrix-core analysis groups: 4 consecutive segments of one order
   {
     std::vector<int> gm, gs;
@@ -371,7 +378,7 @@ extern "C" int slx_sht_destroy(slx_sht* h) {
   if (!h) return 0;
   void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->stw,
                   h->WS, h->WD, h->FN, h->FSo, h->seg_m, h->seg_l0, h->seg_first, h->seed_a, h->seed_b,
-                  h->seed_s, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg, h->grp_desc};
+                  h->seed_s, h->seed_ab, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg, h->grp_desc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;

@@ -295,6 +295,7 @@ struct SegDev {
   const double* sa;  // [nseg][nh] P~_{l0-2}
   const double* sb;  // [nseg][nh] P~_{l0-1}
   const int* ss;     // [nseg][nh] scale count
+  const double2* sab;  // [nseg][nh] start states of the matrix-core kernels (k_seed_unscale)
   const int* first;  // [nm] index of the first segment of order m
 };
 
@@ -578,6 +579,31 @@ __device__ __forceinline__ void mf_unscale(double& a, double& b, int sc, int sc_
   }
 }
 
+// The start state (P~_{l0-2}, P~_{l0-1}) of every segment as the matrix-core kernels take it, one
+// double2 per (segment, ring): the sectoral pair (0, P~_m) for the first segment of an order, the
+// recorded seed otherwise, each through mf_unscale.  Built once per handle, so that a kernel lane
+// starts its segment with one 16-byte load.
+static __global__ void k_seed_unscale(ShtDev d, SegDev g, double2* sab) {
+  const int nh = d.nh;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)g.nseg * nh) return;
+  const int seg = (int)(i / nh), j = (int)(i - (long)seg * nh);
+  const int m = g.seg_m[seg], segi = seg - g.first[m];
+  const int nsm = (d.lmax - m + g.S) / g.S;
+  double a0 = 0.0, b0;
+  int c0;
+  if (segi == 0) {
+    b0 = d.pmm[(long)m * nh + j];
+    c0 = d.pms[(long)m * nh + j];
+  } else {
+    a0 = g.sa[i];
+    b0 = g.sb[i];
+    c0 = g.ss[i];
+  }
+  mf_unscale(a0, b0, c0, segi + 1 < nsm ? g.ss[i + nh] : 0);
+  sab[i] = make_double2(a0, b0);
+}
+
 // Degree steps t of one segment group for R rings of this lane (slot = its segment row).
 // HEADG: the group holds the first segment of its order (lanes with head = true start with
 // P~_m, P~_{m+1}).  MFMAs of two consecutive steps are interleaved so that consecutive matrix
@@ -667,19 +693,8 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
     const bool ok = j < nh;
     const int jj = ok ? j : 0;
     xr[r] = d.x[jj];
-    const bool on = ok && sval;
-    double a0 = 0.0, b0 = 0.0;
-    int c0 = 0;
-    if (on && head) {
-      b0 = d.pmm[(long)m * nh + jj];
-      c0 = d.pms[(long)m * nh + jj];
-    } else if (on) {
-      a0 = g.sa[sx + jj];
-      b0 = g.sb[sx + jj];
-      c0 = g.ss[sx + jj];
-    }
-    mf_unscale(a0, b0, c0, segi + 1 < nsm ? g.ss[sx + nh + jj] : 0);
-    st[r] = LegState{a0, b0, 0};
+    const double2 ab = (ok && sval) ? g.sab[sx + jj] : make_double2(0.0, 0.0);
+    st[r] = LegState{ab.x, ab.y, 0};
     const double2 vs = WS[fb * fstride + (long)m * nh + jj], vd = WD[fb * fstride + (long)m * nh + jj];
     const bool use = ok && colok;
     wv[0][r] = use ? ((col & 1) ? vs.y : vs.x) : 0.0;
@@ -837,19 +852,8 @@ k_leg_ana_ck(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, cons
       const bool ok = j < nh;
       const int jj = ok ? j : 0;
       xr[r] = d.x[jj];
-      const bool on = ok && sval;
-      double a0 = 0.0, b0 = 0.0;
-      int c0 = 0;
-      if (on && head) {
-        b0 = d.pmm[(long)m * nh + jj];
-        c0 = d.pms[(long)m * nh + jj];
-      } else if (on) {
-        a0 = g.sa[sx + jj];
-        b0 = g.sb[sx + jj];
-        c0 = g.ss[sx + jj];
-      }
-      mf_unscale(a0, b0, c0, segi + 1 < nsm ? g.ss[sx + nh + jj] : 0);
-      st[r] = LegState{a0, b0, 0};
+      const double2 ab = (ok && sval) ? g.sab[sx + jj] : make_double2(0.0, 0.0);
+      st[r] = LegState{ab.x, ab.y, 0};
       const double2 vs = WS[fb * fstride + (long)m * nh + jj], vd = WD[fb * fstride + (long)m * nh + jj];
       const bool use = ok && colok;
       wv[0][r] = use ? ((col & 1) ? vs.y : vs.x) : 0.0;
@@ -941,19 +945,8 @@ __global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr
         const bool ok = jl < nact;
         const int jj = ok ? list[jl] : 0;
         xr[r] = d.x[jj];
-        const bool on = ok && sval;
-        double a0 = 0.0, b0 = 0.0;
-        int c0 = 0;
-        if (on && head) {
-          b0 = d.pmm[(long)m * nh + jj];
-          c0 = d.pms[(long)m * nh + jj];
-        } else if (on) {
-          a0 = g.sa[sx + jj];
-          b0 = g.sb[sx + jj];
-          c0 = g.ss[sx + jj];
-        }
-        mf_unscale(a0, b0, c0, segi + 1 < nsm ? g.ss[sx + nh + jj] : 0);
-        st[r] = LegState{a0, b0, 0};
+        const double2 ab = (ok && sval) ? g.sab[sx + jj] : make_double2(0.0, 0.0);
+        st[r] = LegState{ab.x, ab.y, 0};
         const double2 vs = WS[(long)m * nh + jj], vd = WD[(long)m * nh + jj];
         const bool use = ok && colok;
         wv[0][r] = use ? ((slot & 1) ? vs.y : vs.x) : 0.0;
@@ -1301,19 +1294,8 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int jj = okr[r] ? jr[r] : 0;
-      const bool on = okr[r] && sval;
-      double a0 = 0.0, b0 = 0.0;
-      int c0 = 0;
-      if (on && head) {
-        b0 = d.pmm[(long)m * nh + jj];
-        c0 = d.pms[(long)m * nh + jj];
-      } else if (on) {
-        a0 = g.sa[sx + jj];
-        b0 = g.sb[sx + jj];
-        c0 = g.ss[sx + jj];
-      }
-      mf_unscale(a0, b0, c0, sg + 1 < nsm ? g.ss[sx + nh + jj] : 0);
-      st[r] = LegState{a0, b0, 0};
+      const double2 ab = (okr[r] && sval) ? g.sab[sx + jj] : make_double2(0.0, 0.0);
+      st[r] = LegState{ab.x, ab.y, 0};
       aE[r] = aO[r] = 0.0;
     }
     const double s23 = d.s2m3[m];

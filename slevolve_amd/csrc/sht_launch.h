@@ -61,6 +61,7 @@ inline SegDev seg_of(const slx_sht* h) {
   g.sa = h->seed_a;
   g.sb = h->seed_b;
   g.ss = h->seed_s;
+  g.sab = h->seed_ab;
   g.first = h->seg_first;
   return g;
 }

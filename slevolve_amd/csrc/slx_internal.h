@@ -69,6 +69,7 @@ struct slx_sht {
   int *seg_m = nullptr, *seg_l0 = nullptr, *seg_first = nullptr;
   double *seed_a = nullptr, *seed_b = nullptr;
   int* seed_s = nullptr;
+  double2* seed_ab = nullptr;  // [nseg][nh] start states unscaled for the matrix-core kernels
   // analysis on the matrix cores: groups of 4 consecutive segments of one order
   int ana_mfma = 0, ana_r = 2, ngrp = 0;
   int ana_ck = 0;         // chunked-accumulator persistent analysis kernel (else one group per block)
