@@ -665,8 +665,9 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   const int per = (gr.ngrp + 7) >> 3;
   const int gi = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (gi >= gr.ngrp) return;
-  const int m = gr.m[gi], s0 = gr.s0[gi];
-  const int L = d.lmax, nh = d.nh, mo = d.moff[m];
+  const int4 ds = gr.desc[gi];  // (m, s0, seed row of segment s0, packed offset of order m)
+  const int m = ds.x, s0 = ds.y;
+  const int L = d.lmax, nh = d.nh, mo = ds.w;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
   const int nsm = (L - m + S) / S;  // segments of order m
   for (int i = threadIdx.x; i < 4 * S; i += blockDim.x) {
@@ -680,7 +681,7 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   const int segi = s0 + slot;
   const bool sval = segi < nsm;
   const bool head = sval && segi == 0;
-  const long sx = (long)(g.first[m] + (sval ? segi : 0)) * nh;
+  const long sx = (long)(ds.z + (sval ? slot : 0)) * nh;
   const long fstride = (long)d.nm * nh;
   const int col = slot;  // B column of this lane (same lane bits as its A row)
   const bool colok = (col >> 1) < NF;
@@ -1280,15 +1281,17 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
     for (int e = 0; e < 2; ++e)
 #pragma unroll
       for (int r = 0; r < R; ++r) EO[o][e][r] = 0.0;
+  const int first0 = g.first[mord[0]], first1 = g.first[mord[1]];
+  const double s23o0 = d.s2m3[mord[0]], s23o1 = d.s2m3[mord[1]];
   for (int it = wave; it < ng0 + ng1; it += 4) {
     const int o = it < ng0 ? 0 : 1;
     const int gq = it < ng0 ? it : it - ng0;
-    const int m = mord[o], mo = d.moff[m];
+    const int m = mord[o];
     const int nsm = (nl - m + S - 1) / S;
     const int sg = 4 * gq + slot;
     const bool sval = sg < nsm;
     const bool head = sval && sg == 0;
-    const long sx = (long)(g.first[m] + (sval ? sg : 0)) * nh;
+    const long sx = (long)((o ? first1 : first0) + (sval ? sg : 0)) * nh;
     LegState st[R];
     double aE[R], aO[R];
 #pragma unroll
@@ -1298,7 +1301,7 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
       st[r] = LegState{ab.x, ab.y, 0};
       aE[r] = aO[r] = 0.0;
     }
-    const double s23 = d.s2m3[m];
+    const double s23 = o ? s23o1 : s23o0;
     const int ob = (o ? base1 : 0) + sg * S;
     const double* qco = qcl + ob;
     const double* clo = reinterpret_cast<const double*>(cl + (colok ? fb : 0) * np2 + ob) + comp;

@@ -75,7 +75,7 @@ template <int NF, int R>
 int launch_leg_ana_mf(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
                       hipStream_t s) {
   const int nw = (h->nh + 16 * R - 1) / (16 * R);
-  GrpDev gr{h->ngrp, h->grp_m, h->grp_s0};
+  GrpDev gr{h->ngrp, h->grp_m, h->grp_s0, h->grp_desc};
   dim3 grid(8 * ((h->ngrp + 7) / 8));
   const size_t lds = (nw > 1) ? (size_t)nw * 8 * 64 * sizeof(double) : 0;
   k_leg_ana_mf<NF, R><<<grid, 64 * nw, lds, s>>>(dev_of(h), seg_of(h), gr, WS, WD, out, pr);
