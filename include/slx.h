@@ -136,7 +136,10 @@ int slx_solver_init(slx_solver* s, const double* topo0, const double* ice0, void
 int slx_solver_set_dt(slx_solver* s, double dt_years);
 /* Start a step with the ice grid ice_cur (device): modal advance, grounded-ice projection,
  * ice / ocean analyses and the first guess (eustatic, or extrapolated with the step lengths
- * dt0 = this step, dt1, dt2 = the two previous ones when the solver extrapolates). */
+ * dt0 = this step, dt1, dt2 = the two previous ones when the solver extrapolates).  With the
+ * extrapolated guess the ice analysis adds the analysis of the change of I* to the previous one
+ * and C* is analysed by the first inner iteration (unless the solver was created with
+ * SLX_ICE_REANALYSIS=full). */
 int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, double dt1, double dt2, void* stream);
 /* slx_step_begin with the "warm" start across outer passes of OptSeaLevelSolver(reuse="warm")
  * (solver_opt.py:400-402,517-528): before the ice/ocean block the ocean function becomes

@@ -117,6 +117,8 @@ struct slx_solver {
   int* ring_tok = nullptr;
   int next_tok = 0;
   int ice_incr = 0;
+  // the step started with the incremental ice pass: the first inner iteration analyses C* too
+  int cstar_pending = 0;
   slx_status* hstat_dev = nullptr;
 };
 
@@ -701,6 +703,47 @@ __global__ void __launch_bounds__(RING_MAXNT) k_reguess_ring(ShtDev d, LoadRegue
     // I* and beta = (I* < eps) are always stored together, so only changed points need storing
     if (cn != on) lr.commit(0, in, cn);
     if (cs != os) lr.commit(0, is, cs);
+    dn[k] = cn - on;
+    dv[k] = cs - os;
+    any |= (dn[k] != 0.0 || dv[k] != 0.0) ? 1 : 0;
+  }
+  if (!__syncthreads_or(any)) return;
+#pragma unroll
+  for (int k = 0; k < P; ++k) a[tid + k * nt] = make_double2(dn[k], dv[k]);
+  __syncthreads();
+  double2* Z = fft_lds<P>(a, b, d.stw, N, d.log2n, false);
+  ring_fold(Z, WS, WD, 0, d.nh, j, d.lmax, N, d.wn[j], d.ws[j], nt);
+  if (tid == 0) ring_tok[j] = token;
+}
+
+// Step-start ice pass, incremental form (used with the extrapolated start, whose first guess does
+// not read C*_lm; the first inner iteration then analyses C* with rO - tO): every block forms I* of
+// its ring pair from the step's ice on the previous converged topography (solver.py:853-857) as
+// LoadIceGrid does, stores I* and beta where they change and C* = C beta everywhere, and forms
+// dI* = I*new - I*old; a ring pair whose rows did not change ends there, the others transform the
+// change, fold it into WS/WD field 0 and tag the ring with token for k_leg_ana_sp, which adds the
+// analysis of the change to I*_lm.
+template <int P>
+__global__ void __launch_bounds__(RING_MAXNT) k_begin_ring(ShtDev d, LoadIceGrid lg, double2* __restrict__ WS,
+                                                           double2* __restrict__ WD, int* __restrict__ ring_tok,
+                                                           int token) {
+  extern __shared__ double2 smem[];
+  const int N = d.nlon, nt = N / P, tid = threadIdx.x;
+  double2* a = smem;
+  double2* b = smem + N;
+  const int j = xcd_ring(blockIdx.x, d.nh), js = d.nlat - 1 - j;
+  const long rn = (long)j * N, rs = (long)js * N;
+  double dn[P], dv[P];
+  int any = 0;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const long in = rn + tid + k * nt, is = rs + tid + k * nt;
+    const double on = lg.icestar[in], os = lg.icestar[is];
+    const double cn = lg(0, in), cs = lg(0, is);
+    if (cn != on) lg.commit(0, in, cn);
+    if (cs != os) lg.commit(0, is, cs);
+    lg.cstar[in] = lg.cxy[in] * (cn < EPS0 ? 1.0 : 0.0);
+    lg.cstar[is] = lg.cxy[is] * (cs < EPS0 ? 1.0 : 0.0);
     dn[k] = cn - on;
     dv[k] = cs - os;
     any |= (dn[k] != 0.0 || dv[k] != 0.0) ? 1 : 0;
@@ -1698,6 +1741,35 @@ extern "C" int slx_step_begin(slx_solver* s, const double* ice_cur, double dt0, 
   return slx_step_begin_warm(s, ice_cur, dt0, dt1, dt2, nullptr, nullptr, stream);
 }
 
+// sparse contraction of the ring pairs tagged with token (the change of I*, WS/WD field 0), added
+// to I*_lm: a persistent grid of one-wave blocks, sized once to what is resident
+static int launch_ice_sparse(slx_solver* s, int token, Pred pr, hipStream_t st) {
+  const slx_sht* h = s->h;
+  static int cap = 0;
+  if (cap == 0) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    SLX_HIP(hipGetDevice(&dev));
+    SLX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    SLX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_leg_ana_sp<2>), 64, 0));
+    cap = 8 * std::max(1, ncu * std::max(per_cu, 1) / 8);
+  }
+  const int grid = std::min(cap, 8 * ((h->ngrp + 7) / 8));
+  GrpDev gr{h->ngrp, h->grp_m, h->grp_s0, h->grp_desc};
+  k_leg_ana_sp<2><<<grid, 64, 0, st>>>(dev_of(h), seg_of(h), gr, h->WS, h->WD, s->icestar_lm, s->ring_tok, token, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
+template <int P>
+static int launch_begin_ring_p(slx_solver* s, const LoadIceGrid& lg, int token, hipStream_t st) {
+  const slx_sht* h = s->h;
+  const size_t lds = ring_lds(h);
+  if (int rc = allow_lds<&k_begin_ring<P>>(lds)) return rc;
+  k_begin_ring<P><<<h->nh, h->nlon / P, lds, st>>>(dev_of(h), lg, h->WS, h->WD, s->ring_tok, token);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+
 extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double dt0, double dt1, double dt2,
                                    const void* warm_dS, const double* warm_ocean, void* stream) {
   SLX_CHECK(s && ice_cur, "slx_step_begin: null argument");
@@ -1720,15 +1792,23 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
   if (rc) return rc;
   if (warm_ocean)
     SLX_HIP(hipMemcpyAsync(s->cxy, warm_ocean, (size_t)c.npts * sizeof(double), hipMemcpyDeviceToDevice, st));
-  rc = launch_fft_ana(s->h,
-                      LoadIceGrid{ice_cur, s->topo_prev, s->cxy, s->icestar, s->beta, s->cstar, c.rhow, c.rhoi,
-                                  c.checkmarine},
-                      2, Pred{}, st);
-  if (rc) return rc;
-  rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->icestar_lm, s->cstar_lm}}, Pred{}, st);
-  if (rc) return rc;
+  const int extrap_ready = (s->extrap && s->nhist >= 2) ? 1 : 0;
+  const LoadIceGrid lg{ice_cur, s->topo_prev, s->cxy, s->icestar, s->beta, s->cstar, c.rhow, c.rhoi, c.checkmarine};
+  // the eustatic first guess reads all of C*_lm, so only the extrapolated start defers C*
+  s->cstar_pending = (s->ice_incr && extrap_ready && !c.aa) ? 1 : 0;
+  if (s->cstar_pending) {
+    const int token = ++s->next_tok;
+    rc = ring_points(s->h) == 4 ? launch_begin_ring_p<4>(s, lg, token, st) : launch_begin_ring_p<8>(s, lg, token, st);
+    if (!rc) rc = launch_ice_sparse(s, token, Pred{}, st);
+    if (rc) return rc;
+  } else {
+    rc = launch_fft_ana(s->h, lg, 2, Pred{}, st);
+    if (rc) return rc;
+    rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->icestar_lm, s->cstar_lm}}, Pred{}, st);
+    if (rc) return rc;
+  }
   GuessArgs g;
-  g.extrap_ready = (s->extrap && s->nhist >= 2) ? 1 : 0;
+  g.extrap_ready = extrap_ready;
   g.uniform = (dt0 == dt1 && dt1 == dt2) ? 1 : 0;
   const double w = (dt0 + dt1) / (dt1 + dt2);
   g.c1 = dt0 / dt1 * (1.0 + w);
@@ -1759,19 +1839,7 @@ static int launch_reguess_incremental(slx_solver* s, const LoadReguess& ld, Pred
   int rc = ring_points(h) == 4 ? launch_reguess_ring_p<4>(s, ld, token, pr_ring, st)
                                : launch_reguess_ring_p<8>(s, ld, token, pr_ring, st);
   if (rc) return rc;
-  static int cap = 0;
-  if (cap == 0) {
-    int dev = 0, ncu = 0, per_cu = 0;
-    SLX_HIP(hipGetDevice(&dev));
-    SLX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    SLX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_leg_ana_sp<2>), 64, 0));
-    cap = 8 * std::max(1, ncu * std::max(per_cu, 1) / 8);
-  }
-  const int grid = std::min(cap, 8 * ((h->ngrp + 7) / 8));
-  GrpDev gr{h->ngrp, h->grp_m, h->grp_s0, h->grp_desc};
-  k_leg_ana_sp<2><<<grid, 64, 0, st>>>(dev_of(h), seg_of(h), gr, h->WS, h->WD, s->icestar_lm, s->ring_tok, token, pr_ana);
-  SLX_LAUNCH_CHECK();
-  return 0;
+  return launch_ice_sparse(s, token, pr_ana, st);
 }
 
 // ring pass of an inner iteration (k_fft_synana) on the solver's grids
@@ -1855,9 +1923,11 @@ static int iter_tail(slx_solver* s, hipStream_t st) {
   const bool first = (s->iters_enqueued == 0);
   const int it_step = s->iters_enqueued;
   s->iters_enqueued += 1;
-  int rc = launch_fft_synana(s, first ? 1 : 2, nullptr, Pred{done, 0}, st);
+  // C* was analysed at the step's start unless that pass was incremental
+  const bool rt_only = first && !s->cstar_pending;
+  int rc = launch_fft_synana(s, rt_only ? 1 : 2, nullptr, Pred{done, 0}, st);
   if (rc) return rc;
-  if (first)
+  if (rt_only)
     rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
   else
     rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);

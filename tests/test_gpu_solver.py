@@ -20,15 +20,21 @@ def oracle_solver(nlat, L, text, topo, ice0, **kw):
     return OSolver(OracleSHT(nlat, 2 * nlat, L), OLove(text, L, text=True), topo, ice0, norder=L, **kw)
 
 
-def compare(a, b, tol=1e-9):
+def compare(a, b, tol=1e-9, vscale=0.0):
+    """GPU step result a against the oracle's b.  vscale: the run's initial ice volume.  The
+    incremental ice passes carry I*_00 forward by adding the analysis of each change, so once the
+    grounded ice is gone its volume is a rounding residue of the earlier volumes, not an exact 0:
+    the ice volume is held to 1e-12 of max(|its value|, vscale)."""
     h = lambda t: t.cpu().numpy()  # noqa: E731
     assert a["numiter"] == b["numiter"], (a["numiter"], b["numiter"])
     e = {k: float(np.abs(h(a[k]) - b[k]).max()) for k in ("tgrid", "R", "G")}
     assert e["tgrid"] < tol and e["R"] < tol and e["G"] < tol, e
     assert np.array_equal(h(a["ocean"]), b["ocean"]) and np.array_equal(h(a["beta"]), b["beta"])
     assert np.abs(h(a["dS_converged"]) - b["dS_converged"]).max() < 1e-9
-    for k in ("ocean_area", "oceanBeta_area", "ice_volume"):
+    for k in ("ocean_area", "oceanBeta_area"):
         assert abs(a[k] - b[k]) <= 1e-12 * abs(b[k]) + 1e-3, (k, a[k], b[k])
+    assert abs(a["ice_volume"] - b["ice_volume"]) <= 1e-12 * max(abs(b["ice_volume"]), abs(vscale)) + 1e-3, \
+        ("ice_volume", a["ice_volume"], b["ice_volume"], vscale)
     for k in ("gmslc_ocn", "gmslc_ocnBeta"):
         assert abs(a[k] - b[k]) < 1e-9, (k, a[k], b[k])
     assert np.allclose(a["mm"], b["mm"], rtol=1e-9, atol=1e-20)
@@ -46,7 +52,7 @@ def test_quickstart_case(prem_text, topo_seed, checkmarine):
     for k in range(1, 9):
         print(f"[quickstart cm={checkmarine}] step {k}", flush=True)
         ra, rb = a.step(ice[k]), b.step(ice[k])
-        compare(ra, rb)
+        compare(ra, rb, vscale=b.ice_volume0)
         nit.append(ra["numiter"])
     assert nit == [6, 4, 6, 7, 7, 7, 7, 7]
     rsl = -(ra["tgrid"].cpu().numpy() - topo)
@@ -65,7 +71,7 @@ def test_small_case_floating_ice(prem_text):
     for k in range(1, 9):
         print(f"[small case] step {k}", flush=True)
         ra, rb = a.step(ice[k]), b.step(ice[k])
-        compare(ra, rb)
+        compare(ra, rb, vscale=b.ice_volume0)
         nit.append(ra["numiter"])
     assert nit == [4, 4, 5, 8, 6, 6, 5, 6]
     assert a.gr_counts == {"reguess": 36, "ice_analyses": 1}
@@ -77,7 +83,7 @@ def test_tpw_off_and_no_rg(prem_text, topo_seed):
     a = gpu_solver(64, 63, prem_text, topo, ice[0], **kw)
     b = oracle_solver(64, 63, prem_text, topo, ice[0], **kw)
     for k in range(1, 5):
-        compare(a.step(ice[k]), b.step(ice[k]))
+        compare(a.step(ice[k]), b.step(ice[k]), vscale=b.ice_volume0)
 
 
 def test_extrap_nonuniform_steps(prem_text, topo_seed):
@@ -87,7 +93,7 @@ def test_extrap_nonuniform_steps(prem_text, topo_seed):
     b = oracle_solver(128, 63, prem_text, topo, ice[0], **kw)
     for k, dt in enumerate([1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0, 200.0], start=1):
         print(f"[extrap vardt] step {k} dt {dt}", flush=True)
-        compare(a.step(ice[k], dt=dt), b.step(ice[k], dt=dt))
+        compare(a.step(ice[k], dt=dt), b.step(ice[k], dt=dt), vscale=b.ice_volume0)
 
 
 def test_reference_fixture_tgrid8(prem_text, topo_seed, tgrid8_change):
@@ -114,7 +120,7 @@ def test_lmax512_workload_vs_oracle():
     b = oracle_solver(w.nlat, w.lmax, w.love_text, w.topo, w.ice(0), **kw)
     for k in range(1, 4):
         print(f"[lmax512] step {k}", flush=True)
-        e = compare(a.step(w.ice(k)), b.step(w.ice(k)))
+        e = compare(a.step(w.ice(k)), b.step(w.ice(k)), vscale=b.ice_volume0)
         print(f"[lmax512] step {k} errors {e}", flush=True)
 
 
