@@ -187,6 +187,25 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return sh[0];
 }
 
+// NV block sums at once (blockDim 256, sh holds NV x 256 doubles) -> every thread; each value is
+// summed by the tree of block_sum, so the sums equal NV calls of it, with one barrier per level.
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double* sh) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) sh[q * 256 + t] = v[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) sh[q * 256 + t] += sh[q * 256 + t + s];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = sh[q * 256];
+}
+
 // rotation block (solver.py:910-975): products of inertia from the degree-2 load, Liouville
 // equations, centrifugal potential Lambda and the rotational sea-level / radial terms, from the
 // (2,0), (2,1) coefficients of the ocean-load increment dS_cur and the ice increment dI.
@@ -1103,7 +1122,7 @@ __global__ void k_commit(Pts p, Consts c, int extrap) {
 // row); with g0 (the dSL/dt grid without its degree 0) also the row sums of g0 C* on the final C*
 __global__ void __launch_bounds__(256) k_post_grid(Pts p, Consts c, const double* g0) {
   if (!finish_active(p.st)) return;
-  __shared__ double sh[256];
+  __shared__ double sh[5 * 256];
   const int j = blockIdx.x;
   const double cadd = p.st->st.conserv * c.K00;
   double a = 0.0, b = 0.0, e = 0.0, f = 0.0, r5 = 0.0;
@@ -1123,22 +1142,12 @@ __global__ void __launch_bounds__(256) k_post_grid(Pts p, Consts c, const double
     f += o * bt;
     if (g0) r5 += g0[k] * (o * bt);
   }
-  double r;
-  r = block_sum(a, sh);
-  if (threadIdx.x == 0) p.rows[j] = r;
-  __syncthreads();
-  r = block_sum(b, sh);
-  if (threadIdx.x == 0) p.rows[c.nlat + j] = r;
-  __syncthreads();
-  r = block_sum(e, sh);
-  if (threadIdx.x == 0) p.rows[2 * c.nlat + j] = r;
-  __syncthreads();
-  r = block_sum(f, sh);
-  if (threadIdx.x == 0) p.rows[3 * c.nlat + j] = r;
-  if (g0) {
-    __syncthreads();
-    r = block_sum(r5, sh);
-    if (threadIdx.x == 0) p.rows[4 * c.nlat + j] = r;
+  double r[5] = {a, b, e, f, r5};
+  block_sums<5>(r, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) p.rows[q * c.nlat + j] = r[q];
+    if (g0) p.rows[4 * c.nlat + j] = r[4];
   }
 }
 
@@ -1151,14 +1160,18 @@ __global__ void __launch_bounds__(256) k_finish(Pts p, Consts c, int rates, doub
   __shared__ double sh[256];
   DevState* s = p.st;
   if (!finish_active(s)) return;
+  __shared__ double sh5[5 * 256];
   double q[5];
   const int nq = rates ? 5 : 4;
-  for (int k = 0; k < nq; ++k) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
     double v = 0.0;
-    for (int j = threadIdx.x; j < c.nlat; j += blockDim.x) v += p.w00[j] * p.rows[k * c.nlat + j];
-    q[k] = block_sum(v, sh);
-    __syncthreads();
+    if (k < nq)
+      for (int j = threadIdx.x; j < c.nlat; j += blockDim.x) v += p.w00[j] * p.rows[k * c.nlat + j];
+    q[k] = v;
   }
+  block_sums<5>(q, sh5);
+  __syncthreads();
   if (c.ramp && c.tpw) {
     for (int idx = threadIdx.x; idx < 18 * c.nmod2; idx += blockDim.x) {
       const int k = idx / 18, e = idx % 18;
