@@ -598,25 +598,25 @@ __global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it, double2
 // Every block first forms xi = |sum|new| - sum|old|| / sum|old| (solver.py:1023-1030) from the
 // update's block partials in a fixed order (sum|old| = the previous iteration's partials, or the
 // guess's), so that all blocks take the same decision; block 0 publishes it.
+// sh: 2 x 256 doubles; both sums are reduced together, each by the tree of block_sum
 __device__ double convergence_xi(const Pts& p, int it, int nbu, int nbg, double* sh, double& sa, double& so) {
-  double v = 0.0;
-  for (int b = threadIdx.x; b < nbu; b += blockDim.x) v += p.partial[(it & 1) * PSLOT + b];
-  sa = block_sum(v, sh);
-  __syncthreads();
-  v = 0.0;
+  double v[2] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < nbu; b += blockDim.x) v[0] += p.partial[(it & 1) * PSLOT + b];
   if (it == 0) {
-    for (int b = threadIdx.x; b < nbg; b += blockDim.x) v += p.partial[2 * PSLOT + b];
+    for (int b = threadIdx.x; b < nbg; b += blockDim.x) v[1] += p.partial[2 * PSLOT + b];
   } else {
-    for (int b = threadIdx.x; b < nbu; b += blockDim.x) v += p.partial[((it - 1) & 1) * PSLOT + b];
+    for (int b = threadIdx.x; b < nbu; b += blockDim.x) v[1] += p.partial[((it - 1) & 1) * PSLOT + b];
   }
-  so = block_sum(v, sh);
+  block_sums<2>(v, sh);
+  sa = v[0];
+  so = v[1];
   if (fabs(sa - so) < EPS0 && fabs(so) < EPS0) return 0.0;
   if (fabs(so) < EPS0) return fabs((sa - so) / (EPS0 * 10));
   return fabs((sa - so) / so);
 }
 
 __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nbu, int nbg) {
-  __shared__ double sh[256];
+  __shared__ double sh[2 * 256];
   __shared__ int shf;
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
@@ -781,7 +781,7 @@ __global__ void __launch_bounds__(RING_MAXNT) k_begin_ring(ShtDev d, LoadIceGrid
 // when the iteration has converged, or the mask is not frozen, count the ocean-mask cells the
 // topography iterate would change and the grounded flips on it (no grid is written here).
 __global__ void __launch_bounds__(256) k_refresh_aa(Pts p, Consts c, int it, int nbu, int nbg) {
-  __shared__ double sh[256];
+  __shared__ double sh[2 * 256];
   __shared__ int shm, shf;
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
