@@ -175,35 +175,40 @@ __device__ __forceinline__ double2 ice_tot(const double2* pice, long i, double2 
   return make_double2(a.x + d.x, a.y + d.y);
 }
 
-// Block sum of v (blockDim 256) -> lane 0; fixed order.
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) sh[t] += sh[t + s];
-    __syncthreads();
-  }
-  return sh[0];
-}
-
-// NV block sums at once (blockDim 256, sh holds NV x 256 doubles) -> every thread; each value is
-// summed by the tree of block_sum, so the sums equal NV calls of it, with one barrier per level.
+// NV block sums at once (blockDim 256, sh holds NV x 256 doubles) -> every thread.  Fixed pairwise
+// tree: at stride s = 128, 64, ..., 1 element t < s takes element t + s; strides 128 and 64 through
+// LDS, the rest inside wave 0 with lane shuffles (lane t adds lane t + s's partial: the same pairs
+// in the same order, without a barrier per level).
 template <int NV>
 __device__ __forceinline__ void block_sums(double (&v)[NV], double* sh) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < NV; ++q) sh[q * 256 + t] = v[q];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) {
+  if (t < 128) {
 #pragma unroll
-      for (int q = 0; q < NV; ++q) sh[q * 256 + t] += sh[q * 256 + t + s];
-    }
-    __syncthreads();
+    for (int q = 0; q < NV; ++q) sh[q * 256 + t] += sh[q * 256 + t + 128];
   }
+  __syncthreads();
+  if (t < 64) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double x = sh[q * 256 + t] + sh[q * 256 + t + 64];
+#pragma unroll
+      for (int s = 32; s > 0; s >>= 1) x += __shfl_down(x, s, 64);
+      if (t == 0) sh[q * 256] = x;
+    }
+  }
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = sh[q * 256];
+}
+
+// Block sum of v (blockDim 256) -> every thread; the tree of block_sums.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  double a[1] = {v};
+  block_sums<1>(a, sh);
+  return a[0];
 }
 
 // rotation block (solver.py:910-975): products of inertia from the degree-2 load, Liouville
