@@ -20,7 +20,12 @@
 extern "C" {
 #endif
 
-#define SLX_ABI_VERSION 2
+#define SLX_ABI_VERSION 3
+
+/* Arithmetic of the Legendre contractions: FP64 throughout, or the FP64 degree recurrence feeding
+ * FP32 matrix-core products accumulated in FP32 (inputs, outputs and every other stage FP64). */
+#define SLX_PREC_FP64 0
+#define SLX_PREC_FP32 1
 
 typedef struct slx_sht slx_sht;
 typedef struct slx_solver slx_solver;
@@ -33,8 +38,10 @@ int slx_abi_version(void);
  * sht_ducc.py:70-132 (DuccSHT), reached through make_sht (sht_ducc.py:206-217).  Builds the
  * Gauss-Legendre nodes/weights (gaulw, sht.py:65-101), the sectoral P~_mm and recurrence
  * coefficients (legend, sht.py:105-144) and FFT twiddles, and work buffers for up to
- * max_fields (1..3) fields per call.  nlat even, nlon a power of two, lmax <= nlon/2. */
-int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_sht** out);
+ * max_fields (1..3) fields per call.  nlat even, nlon a power of two, lmax <= nlon/2.
+ * precision: SLX_PREC_FP64, or SLX_PREC_FP32 for the analysis / synthesis entry points below
+ * (the reference has FP64 only; the solver chooses its sweeps' precision itself). */
+int slx_sht_create(int nlat, int nlon, int lmax, int precision, int max_fields, slx_sht** out);
 int slx_sht_destroy(slx_sht* h);
 /* dims (host, 9 ints or NULL) = {nlat, nlon, lmax, ncoef, max_fields, analysis kernel (-1 VALU,
  * 0 matrix cores one group per block, 1 persistent chunked), its rings per lane, synthesis kernel
@@ -97,6 +104,14 @@ typedef struct slx_solver_params {
    * iterations, at most max_mask_refresh mask refreshes per step, history kept across a refresh
    * if anderson_keep */
   int anderson, anderson_depth, anderson_freeze, max_mask_refresh, anderson_keep;
+  /* Mixed-precision sweeps (no counterpart in the reference, whose transforms are FP64): with
+   * precision = SLX_PREC_FP32 the inner iterations of a step take the FP32-product contractions
+   * until one follows an iteration that changed no ocean-function cell and no grounded-ice flag,
+   * or whose convergence ratio xi was <= eps_switch (> epsilon1); from then on FP64 sweeps until
+   * xi <= epsilon1 after an FP64 sweep.  The step start, re-guesses, radial displacement, modal
+   * state, mass integrals and the convergence norm stay FP64.  Not with anderson = 1. */
+  int precision;
+  double eps_switch;
 } slx_solver_params;
 
 typedef struct slx_status {
@@ -108,6 +123,8 @@ typedef struct slx_status {
   double ocean_area0, oceanBeta_area0, ice_volume0;
   /* inner = "anderson" counters over the run (OptSeaLevelSolver.inner_stats) */
   int aa_mixed, aa_plain, aa_fallback, aa_singular, aa_mask_refresh, aa_frozen_iters;
+  int sweeps32;  /* inner iterations of the step that took the FP32-product contractions */
+  int pad_;
 } slx_status;
 
 typedef struct slx_fields { /* device pointers owned by the solver */

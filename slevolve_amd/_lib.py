@@ -24,7 +24,8 @@ class SlxSolverParams(C.Structure):
                 ("ttl", P_double), ("ttlekhl", P_double), ("ttlhe", P_double),
                 ("ramp", c_int), ("a_hl", P_double), ("a_thl", P_double), ("ttlle", P_double), ("lTE2", c_double),
                 ("anderson", c_int), ("anderson_depth", c_int), ("anderson_freeze", c_int),
-                ("max_mask_refresh", c_int), ("anderson_keep", c_int)]
+                ("max_mask_refresh", c_int), ("anderson_keep", c_int),
+                ("precision", c_int), ("eps_switch", c_double)]
 
 
 class SlxStatus(C.Structure):
@@ -36,7 +37,7 @@ class SlxStatus(C.Structure):
                 ("gmslc_ocnBeta", c_double), ("ice_volume", c_double),
                 ("ocean_area0", c_double), ("oceanBeta_area0", c_double), ("ice_volume0", c_double),
                 ("aa_mixed", c_int), ("aa_plain", c_int), ("aa_fallback", c_int), ("aa_singular", c_int),
-                ("aa_mask_refresh", c_int), ("aa_frozen_iters", c_int)]
+                ("aa_mask_refresh", c_int), ("aa_frozen_iters", c_int), ("sweeps32", c_int), ("pad_", c_int)]
 
 
 class SlxFields(C.Structure):
@@ -56,7 +57,7 @@ class SlxReadouts(C.Structure):
 SIGNATURES = {
     "slx_last_error": (C.c_char_p, []),
     "slx_abi_version": (c_int, []),
-    "slx_sht_create": (c_int, [c_int, c_int, c_int, c_int, C.POINTER(c_void_p)]),
+    "slx_sht_create": (c_int, [c_int, c_int, c_int, c_int, c_int, C.POINTER(c_void_p)]),
     "slx_sht_destroy": (c_int, [c_void_p]),
     "slx_sht_info": (c_int, [c_void_p, P_int, P_double, P_double]),
     "slx_sht_analysis": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
@@ -80,6 +81,10 @@ SIGNATURES = {
                               C.POINTER(SlxStatus), c_void_p]),
     "slx_solver_fields": (c_int, [c_void_p, C.POINTER(SlxFields)]),
 }
+
+
+PREC_FP64, PREC_FP32 = 0, 1
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_FP32, "fp32": PREC_FP32}
 
 
 class SlxError(RuntimeError):

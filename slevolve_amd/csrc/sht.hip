@@ -50,8 +50,9 @@ static int gauss_legendre(int nlat, std::vector<double>& z_out, std::vector<doub
   return 0;
 }
 
-extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_sht** out) {
+extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int max_fields, slx_sht** out) {
   SLX_CHECK(out != nullptr, "slx_sht_create: null handle pointer");
+  SLX_CHECK(precision == SLX_PREC_FP64 || precision == SLX_PREC_FP32, "precision must be 0 (FP64) or 1 (FP32 products)");
   SLX_CHECK(nlat >= 2 && nlat % 2 == 0, "nlat must be even");
   SLX_CHECK(nlon >= 8 && (nlon & (nlon - 1)) == 0, "nlon must be a power of two >= 8");
   SLX_CHECK(nlon <= 4096, "nlon must be <= 4096 (a ring pass holds two rows and a work buffer in LDS)");
@@ -67,6 +68,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int max_fields, slx_
   h->nm = lmax + 1;
   h->ncoef = (lmax + 1) * (lmax + 2) / 2;
   h->max_fields = max_fields;
+  h->precision = precision;
   int lg = 0;
   while ((1 << lg) < nlon) ++lg;
   h->log2n = lg;
@@ -411,6 +413,15 @@ extern "C" int slx_sht_analysis_contraction(slx_sht* h, int nf, void* coef_v, vo
   SLX_CHECK(h && coef, "slx_sht_analysis_contraction: null argument");
   SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_analysis_contraction: nf out of range");
   hipStream_t s = (hipStream_t)stream;
+  if (h->precision == SLX_PREC_FP32) {
+    switch (nf) {
+      case 1: return launch_leg_ana32<1>(h, h->WS, h->WD, OutPtrs<1>{{coef}}, Pred{}, s);
+      case 2: return launch_leg_ana32<2>(h, h->WS, h->WD, OutPtrs<2>{{coef, coef + h->ncoef}}, Pred{}, s);
+      default:
+        return launch_leg_ana32<3>(h, h->WS, h->WD, OutPtrs<3>{{coef, coef + h->ncoef, coef + 2 * (long)h->ncoef}},
+                                   Pred{}, s);
+    }
+  }
   switch (nf) {
     case 1: return launch_leg_ana<1>(h, h->WS, h->WD, OutPtrs<1>{{coef}}, Pred{}, s);
     case 2: return launch_leg_ana<2>(h, h->WS, h->WD, OutPtrs<2>{{coef, coef + h->ncoef}}, Pred{}, s);
@@ -435,7 +446,9 @@ extern "C" int slx_sht_synthesis(slx_sht* h, const void* coef_v, int nf, double*
   SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_synthesis: nf out of range");
   hipStream_t s = (hipStream_t)stream;
   for (int f = 0; f < nf; ++f) {
-    const int rc = launch_leg_syn<1>(h, CoefPlain{coef + (long)f * h->ncoef, 0}, Pred{}, s, f);
+    const CoefPlain cp{coef + (long)f * h->ncoef, 0};
+    const int rc = h->precision == SLX_PREC_FP32 ? launch_leg_syn32<1>(h, cp, Pred{}, s, f)
+                                                 : launch_leg_syn<1>(h, cp, Pred{}, s, f);
     if (rc) return rc;
   }
   return launch_fft_syn(h, StorePlain{grid, (long)h->nlat * h->nlon}, nf, Pred{}, s);

@@ -4,6 +4,7 @@
 #include <type_traits>
 
 #include "sht_kernels.h"
+#include "sht_kernels_x32.h"
 
 namespace slx {
 
@@ -233,5 +234,80 @@ int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field 
   k_leg_syn_s<NF, 1><<<grid, 256, 0, s>>>(dev_of(h), seg_of(h), h->rec, src, cstride, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();
   return 0;
+}
+
+// Reduced-precision contractions (FP64 recurrence, FP32 matrix-core products).  Analysis: one block
+// of X32_NW waves per super-group of 4 segment groups (dealt XCD-aware as the FP64 kernels'
+// groups), R ring chains per lane with 4 R X32_NW >= nh.
+constexpr int X32_SYN_R = 2;  // rings per lane of the synthesis
+template <int NF>
+int launch_leg_ana_x32(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
+                       hipStream_t s) {
+  const int nsg = (h->ngrp + 3) / 4;
+  GrpDev gr{h->ngrp, h->grp_m, h->grp_s0, h->grp_desc};
+  const int rneed = (h->nh + 4 * X32_NW - 1) / (4 * X32_NW);
+  const dim3 grid(8 * ((nsg + 7) / 8)), blk(64 * X32_NW);
+  const ShtDev d = dev_of(h);
+  const SegDev g = seg_of(h);
+  if (rneed <= 1)
+    k_leg_ana_x32<NF, 1><<<grid, blk, 0, s>>>(d, g, gr, WS, WD, out, pr);
+  else if (rneed <= 2)
+    k_leg_ana_x32<NF, 2><<<grid, blk, 0, s>>>(d, g, gr, WS, WD, out, pr);
+  else if (rneed <= 4)
+    k_leg_ana_x32<NF, 4><<<grid, blk, 0, s>>>(d, g, gr, WS, WD, out, pr);
+  else if (rneed <= 8)
+    k_leg_ana_x32<NF, 8><<<grid, blk, 0, s>>>(d, g, gr, WS, WD, out, pr);
+  else
+    k_leg_ana_x32<NF, 16><<<grid, blk, 0, s>>>(d, g, gr, WS, WD, out, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+template <int NF>
+int launch_leg_syn_x32(const slx_sht* h, const double2* src, long cstride, long off, Pred pr, hipStream_t s) {
+  constexpr int R = X32_SYN_R;
+  const int ny = (h->nh + 16 * R - 1) / (16 * R), np = (h->lmax + 2) / 2;
+  const dim3 grid(8 * ny * ((np + 7) / 8));
+  const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
+  const size_t lds = std::max((size_t)np2 * (sizeof(double) + NF * sizeof(float2)),
+                              (size_t)4 * 2 * 2 * R * 64 * sizeof(float));
+  if (int rc = allow_lds<&k_leg_syn_x32<NF, R>>(lds)) return rc;
+  k_leg_syn_x32<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+// FP32-product analysis of NF fields from the folded spectra in WS/WD (matrix-core path only)
+template <int NF>
+int launch_leg_ana32(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr, hipStream_t s) {
+  if constexpr (NF == 3) {
+    const long fs = (long)h->nm * h->nh;
+    int rc = launch_leg_ana_x32<2>(h, WS, WD, OutPtrs<2>{{out.p[0], out.p[1]}}, pr, s);
+    if (rc) return rc;
+    return launch_leg_ana_x32<1>(h, WS + 2 * fs, WD + 2 * fs, OutPtrs<1>{{out.p[2]}}, pr, s);
+  } else {
+    return launch_leg_ana_x32<NF>(h, WS, WD, out, pr, s);
+  }
+}
+// FP32-product synthesis contraction of NF fields (coefficient loaders as launch_leg_syn)
+template <int NF, class Coef>
+int launch_leg_syn32(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field = 0) {
+  const double2* src;
+  long cstride;
+  if constexpr (std::is_same<Coef, CoefPlain>::value) {
+    src = cf.c;
+    cstride = cf.fstride;
+  } else {
+    k_coef_fill<NF, Coef><<<(h->ncoef + 255) / 256, 256, 0, s>>>(dev_of(h), cf, h->CS, h->ncoef, h->ncoef, pr);
+    SLX_LAUNCH_CHECK();
+    src = h->CS;
+    cstride = h->ncoef;
+  }
+  const long off = (long)field * h->nm * h->nh;
+  if constexpr (NF <= 2) {
+    return launch_leg_syn_x32<NF>(h, src, cstride, off, pr, s);
+  } else {
+    int rc = launch_leg_syn_x32<2>(h, src, cstride, off, pr, s);
+    if (rc) return rc;
+    return launch_leg_syn_x32<1>(h, src + 2 * cstride, cstride, off + 2L * h->nm * h->nh, pr, s);
+  }
 }
 }  // namespace slx

@@ -45,6 +45,8 @@ void slx_set_error(const std::string& msg);
 // moff[m] + (l - m), moff[m] = m (L+1) - m (m-1)/2, complex as double2.
 struct slx_sht {
   int nlat = 0, nlon = 0, lmax = 0, nh = 0, nm = 0, ncoef = 0, log2n = 0, max_fields = 0;
+  int precision = 0;      // SLX_PREC_FP64, or SLX_PREC_FP32: the transform entry points take the
+                          // FP32-product contractions
   double* x = nullptr;    // [nh] mu_j of the northern rings (north -> equator)
   double* wn = nullptr;   // [nh] w_j / nlon
   double* ws = nullptr;   // [nh] w_{nlat-1-j} / nlon

@@ -49,6 +49,10 @@ struct DevState {
   int aa_kind;       // last mix: 0 mixed, 1 plain, 2 fallback, 3 singular
   double aa_fn[6];   // residual norm of each history slot
   double aa_gam[5];
+  // mixed-precision sweeps: 1 = iteration of that parity takes FP64 contractions, 0 = FP32 products;
+  // ocean-function cells the refresh of the iteration of that parity changed
+  int sweep64[2];
+  int mchg[2];
 };
 
 // Constants evaluated on the host once, in the reference's operation order.
@@ -64,6 +68,8 @@ struct Consts {
   int aa, aa_depth, aa_freeze, aa_maxref, aa_keep;  // inner = "anderson"
   double dt;                                 // length of the current step (years)
   double c_il, c_m, c_tsl, c_trr, c_thl;     // sum_k (1 - phi_k) a_k at degree 2 (solver.py:362-367)
+  int mixed;                                 // FP32-product sweeps while xi > eps_sw
+  double eps_sw;
 };
 
 struct slx_solver {
@@ -365,6 +371,10 @@ __global__ void __launch_bounds__(64) k_begin(Pts p, Consts c) {
     s->aa_head = 0;
     s->aa_tok_mask = s->aa_tok_accept = s->aa_tok_push = s->aa_tok_mix = -1;
     s->aa_mdiff[0] = s->aa_mdiff[1] = 0;
+    s->sweep64[0] = c.mixed ? 0 : 1;
+    s->sweep64[1] = 1;
+    s->mchg[0] = s->mchg[1] = 0;
+    s->st.sweeps32 = 0;
   }
 }
 
@@ -452,6 +462,7 @@ struct CoefB18 {
   DevState* st;
   Consts c;
   int eval;
+  int it = -1;  // inner iteration whose sweep this assembles (mixed precision: its block 0 decides)
   // only the block that stores the rotation (block 0) and those holding the degree-2, m <= 2
   // coefficients need it
   __device__ bool prologue_needed(int lo, int hi) const {
@@ -460,6 +471,12 @@ struct CoefB18 {
   }
   __device__ void prologue(double* pro) const {
     if (threadIdx.x != 0) return;
+    if (c.mixed && it > 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+      // mixed precision: the sweep after one that changed neither the ocean function nor a
+      // grounded-ice flag (the masks are final) and every later one take FP64 contractions
+      const int prev = (it - 1) & 1;
+      if (!st->sweep64[it & 1] && st->mchg[prev] == 0 && st->nflip[prev] == 0) st->sweep64[it & 1] = 1;
+    }
     if (eval) {
       RotOut o;
       if (c.tpw) {
@@ -622,16 +639,20 @@ __device__ double convergence_xi(const Pts& p, int it, int nbu, int nbg, double*
 
 __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nbu, int nbg) {
   __shared__ double sh[2 * 256];
-  __shared__ int shf;
+  __shared__ int shf, shm;
   DevState* s = p.st;
   if (*(volatile int*)&s->st.done) return;
   double sa, so;
   const double xi = convergence_xi(p, it, nbu, nbg, sh, sa, so);
   const double conserv = conserv_of(p, c);
-  const bool conv = xi <= c.eps1;
+  // a step converges on an FP64 sweep only; FP32-product sweeps run while xi > eps_sw
+  const int fp64 = *(volatile const int*)&s->sweep64[it & 1];
+  const bool conv = xi <= c.eps1 && fp64;
   const bool fail = !conv && it + 1 >= MAX_INNER;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     post_decision(p, it, conv || fail, fail);
+    s->sweep64[(it + 1) & 1] = (fp64 || xi <= c.eps_sw) ? 1 : 0;
+    if (!fp64) s->st.sweeps32 += 1;
     s->st.numiter = it + 1;
     s->st.xi = xi;
     s->st.conserv = conserv;
@@ -641,16 +662,19 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
     if (fail) s->st.failed = 1;
   }
   if (conv || fail) return;
-  if (threadIdx.x == 0) shf = 0;
+  if (threadIdx.x == 0) shf = shm = 0;
+  if (c.mixed && blockIdx.x == 0 && threadIdx.x == 0) s->mchg[(it + 1) & 1] = 0;
   __syncthreads();
   const long n = c.npts;
   const double cadd = conserv * c.K00;
   const bool regs = c.checkmarine && s->nacc2[it & 1] < GR_MAXACC;
-  int flips = 0;
+  int flips = 0, mflips = 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double t = p.tinit[i] - (p.dsl[i] + cadd);
+    const double cn = t < 0.0 ? 1.0 : 0.0;
+    if (c.mixed && cn != p.cxy[i]) mflips += 1;
     p.topo[i] = t;
-    p.cxy[i] = t < 0.0 ? 1.0 : 0.0;
+    p.cxy[i] = cn;
     if (regs) {
       const double ice = p.ice_cur[i];
       const double cand = ice * keep_factor(ice, t, c.rhow, c.rhoi);
@@ -659,8 +683,10 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
     }
   }
   if (flips) atomicAdd(&shf, flips);
+  if (mflips) atomicAdd(&shm, mflips);
   __syncthreads();
   if (threadIdx.x == 0 && shf) atomicAdd(&s->nflip[it & 1], shf);
+  if (threadIdx.x == 0 && shm) atomicAdd(&s->mchg[it & 1], shm);
 }
 
 // The grounded-ice re-guess of an inner iteration, applied by the ice ring pass
@@ -1506,6 +1532,11 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   c.aa_freeze = p->anderson_freeze;
   c.aa_maxref = p->max_mask_refresh;
   c.aa_keep = p->anderson_keep ? 1 : 0;
+  SLX_CHECK(p->precision == SLX_PREC_FP64 || p->precision == SLX_PREC_FP32, "precision must be 0 or 1");
+  c.mixed = p->precision == SLX_PREC_FP32 ? 1 : 0;
+  c.eps_sw = p->eps_switch;
+  SLX_CHECK(!c.mixed || (!c.aa && p->eps_switch > p->epsilon1 && h->ana_mfma),
+            "mixed-precision sweeps need inner extrap or off and eps_switch > epsilon1");
   SLX_CHECK(!c.aa || (p->extrap && p->anderson_depth >= 1 && p->anderson_depth <= 5 && p->anderson_freeze >= 1 &&
                       p->max_mask_refresh >= 0),
             "anderson needs extrap = 1, depth 1..5, freeze >= 1, max_mask_refresh >= 0");
@@ -1929,7 +1960,19 @@ static int iter_head(slx_solver* s, hipStream_t st) {
   const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
   CoefB18 cb{s->icestar_lm, oldice, s->p_ice, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->deg,
              s->st, s->c, 1};
-  return launch_leg_syn<1>(s->h, cb, Pred{&s->st->st.done, 0}, st);
+  const int* done = &s->st->st.done;
+  if (!s->c.mixed) return launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
+  cb.it = s->iters_enqueued;
+  // mixed sweeps: the coefficients once, then the FP64 or the FP32-product contraction as the
+  // device decided for this iteration (head of iteration iters_enqueued)
+  const slx_sht* h = s->h;
+  k_coef_fill<1, CoefB18><<<(h->ncoef + 255) / 256, 256, 0, st>>>(dev_of(h), cb, h->CS, h->ncoef, h->ncoef,
+                                                                   Pred{done, 0});
+  SLX_LAUNCH_CHECK();
+  const int* sw = &s->st->sweep64[s->iters_enqueued & 1];
+  int rc = launch_leg_syn<1>(h, CoefPlain{h->CS, h->ncoef}, Pred{done, 0, sw, 1}, st);
+  if (!rc) rc = launch_leg_syn32<1>(h, CoefPlain{h->CS, h->ncoef}, Pred{done, 0, sw, 0}, st);
+  return rc;
 }
 
 static int iter_tail(slx_solver* s, hipStream_t st) {
@@ -1945,10 +1988,22 @@ static int iter_tail(slx_solver* s, hipStream_t st) {
   const bool rt_only = first && !s->cstar_pending;
   int rc = launch_fft_synana(s, rt_only ? 1 : 2, nullptr, Pred{done, 0}, st);
   if (rc) return rc;
-  if (rt_only)
-    rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
-  else
-    rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
+  if (!c.mixed) {
+    if (rt_only)
+      rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0}, st);
+    else
+      rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0}, st);
+  } else {  // the FP64 or the FP32-product contraction, as decided for this iteration
+    const int* sw = &s->st->sweep64[it_step & 1];
+    const Pred p64{done, 0, sw, 1}, p32{done, 0, sw, 0};
+    if (rt_only) {
+      rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, p64, st);
+      if (!rc) rc = launch_leg_ana32<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, p32, st);
+    } else {
+      rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, p64, st);
+      if (!rc) rc = launch_leg_ana32<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, p32, st);
+    }
+  }
   if (rc) return rc;
   k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step, s->load_inc);
   SLX_LAUNCH_CHECK();

@@ -2,7 +2,10 @@
 ``SHT`` (python/slevolve/sht.py:148-343) and ``make_sht`` (python/slevolve/sht_ducc.py:206-217).
 
 ``analysis(grid[nlat, nlon] float64, out=None) -> coeffs[lmax+1, lmax+1] complex128`` ([l, m],
-zero above the diagonal) and ``synthesis(coeffs, out=None) -> grid``.  Inputs may be NumPy arrays
+zero above the diagonal) and ``synthesis(coeffs, out=None) -> grid``.  ``raw=True`` exchanges
+V[l, m] = (-1)^m U[l, m] instead, as the ducc backend's raw mode does (sht_ducc.py:101-113).
+``precision="mixed"`` makes these two calls use the FP32-product contractions (FP64 recurrence,
+FP32 matrix-core products; about 1e-6 relative); the default is FP64.  Inputs may be NumPy arrays
 (results come back as NumPy arrays) or torch tensors (results stay on the device).  Without
 ``out`` a persistent internal buffer is returned, as in the reference.  The work runs in the HIP
 kernels of libslx.so (ring FFTs + per-order FP64 Legendre contractions); there is no CPU path.
@@ -21,10 +24,8 @@ SQRT2_LEGACY = 1.41421356
 
 
 class SHT:
-    raw = False  # the (-1)^m sign-free mode of the ducc backend is not used: U convention only
-
     def __init__(self, nlat: int, nlon: int, lmax: int, workers: int = 1, chunk: int = 16,
-                 device=None, max_fields: int = 3):
+                 device=None, max_fields: int = 3, raw: bool = False, precision: str = "fp64"):
         if nlat % 2 != 0:
             raise ValueError("nlat must be even")
         if lmax > nlon // 2:
@@ -37,12 +38,16 @@ class SHT:
         self.workers = self.nthreads = int(workers)
         self.chunk = int(chunk)
         self.max_fields = int(max_fields)
+        self.raw = bool(raw)
+        if precision not in _lib.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}, got {precision!r}")
+        self.precision = precision
         self.dev = D.device(device)
         lib = _lib.load()
         h = C.c_void_p()
         with torch.cuda.device(self.dev):
-            _lib.check(lib.slx_sht_create(self.nlat, self.nlon, self.lmax, self.max_fields, C.byref(h)),
-                       "slx_sht_create")
+            _lib.check(lib.slx_sht_create(self.nlat, self.nlon, self.lmax, _lib.PRECISIONS[precision],
+                                          self.max_fields, C.byref(h)), "slx_sht_create")
         self._h = h
         dims = (C.c_int * 9)()
         x = np.zeros(self.nlat)
@@ -58,6 +63,11 @@ class SHT:
         self._tri = torch.zeros((self.max_fields, self.ncoef), dtype=torch.complex128, device=self.dev)
         self._coef_out = torch.zeros((self.nm, self.nm), dtype=torch.complex128, device=self.dev)
         self._grid_out = torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=self.dev)
+        self._msign = None
+        if self.raw:  # (-1)^m per column of the dense [l, m] layout
+            sg = torch.ones(self.nm, dtype=torch.complex128, device=self.dev)
+            sg[1::2] = -1.0
+            self._msign = sg
 
     @property
     def handle(self):
@@ -90,15 +100,21 @@ class SHT:
         _lib.call("slx_sht_synthesis", self._h, coefs.data_ptr(), nf, out.data_ptr(), D.stream_ptr())
         return out
 
-    def to_dense(self, tri: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    def to_dense(self, tri: torch.Tensor, out: torch.Tensor | None = None, raw: bool | None = None) -> torch.Tensor:
+        """Packed U triangle -> dense [l, m] (V = (-1)^m U for a raw-mode handle unless raw=False)."""
         if out is None:
             out = torch.empty((self.nm, self.nm), dtype=torch.complex128, device=self.dev)
         _lib.call("slx_sht_tri_to_dense", self._h, tri.data_ptr(), out.data_ptr(), 1, D.stream_ptr())
+        if (self.raw if raw is None else raw) and self._msign is not None:
+            out.mul_(self._msign)
         return out
 
     def to_packed(self, dense: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Dense [l, m] (V for a raw-mode handle) -> packed U triangle."""
         if out is None:
             out = torch.empty((self.ncoef,), dtype=torch.complex128, device=self.dev)
+        if self.raw:
+            dense = dense * self._msign
         _lib.call("slx_sht_dense_to_tri", self._h, dense.data_ptr(), out.data_ptr(), 1, D.stream_ptr())
         return out
 
@@ -146,13 +162,15 @@ class SHT:
 
 
 def make_sht(backend: str, nlat: int, nlon: int, lmax: int, workers: int = 1, chunk: int = 16,
-             raw: bool = False, device=None):
-    """Factory with the reference's signature; every backend name maps to the GPU transform."""
-    if raw:
-        raise ValueError("raw (sign-free) mode is not provided; use the U convention")
+             raw: bool = False, device=None, precision: str = "fp64"):
+    """Factory with the reference's signature (sht_ducc.py:206-217); every backend name maps to the
+    GPU transform.  raw=True (the ducc backend's raw mode) exchanges V = (-1)^m U; as in the
+    reference it is refused for the "numpy" backend."""
     if backend not in ("numpy", "ducc", "hip", "gpu"):
         raise ValueError(f"unknown SHT backend {backend!r} (expected 'numpy', 'ducc' or 'hip')")
-    return SHT(nlat, nlon, lmax, workers=workers, chunk=chunk, device=device)
+    if raw and backend == "numpy":
+        raise ValueError("raw mode needs the ducc backend")
+    return SHT(nlat, nlon, lmax, workers=workers, chunk=chunk, device=device, raw=raw, precision=precision)
 
 
 __all__ = ["SHT", "make_sht", "SQRT2_LEGACY"]

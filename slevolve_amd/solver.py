@@ -11,9 +11,16 @@ its reused buffers).
 Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap" |
 "anderson" (Anderson mixing with the frozen ocean mask, OptSeaLevelSolver), the
 second-order load placement (ramp), the analytic rates (step(rates=True)), the horizontal read-out
-(solver.horizontal = True, result key "horiz") and the "warm" cross-pass reuse of OptSeaLevelSolver
-(per-step converged states kept on the device).  Axial-rotation variants and the other reuse
-modes of the reference are not provided and raise.
+(solver.horizontal = True, result key "horiz"), the "warm" cross-pass reuse of OptSeaLevelSolver
+(per-step converged states kept on the device) and its raw-mode spectra (opt "raw" with a raw-mode
+SHT: dense spectral outputs in V = (-1)^m U).  Axial-rotation variants and the other reuse modes of
+the reference are not provided and raise.
+
+``precision="mixed"`` (no counterpart in the reference) runs the early inner sweeps of every step
+with the FP32-product Legendre contractions; a sweep takes the FP64 contractions once the previous
+one changed no ocean-function cell and no grounded-ice flag, or once the convergence ratio is at or
+below ``eps_switch`` (default 100 epsilon1), and so do all later sweeps of the step; the step
+converges on an FP64 sweep only.  Everything else stays FP64.
 """
 from __future__ import annotations
 
@@ -114,7 +121,8 @@ def _status_record(st):
     return {"iter": st.step, "numiter": st.numiter, "ocean_area": st.ocean_area, "oceanBeta_area": st.oceanBeta_area,
             "gmslc_ocn": st.gmslc_ocn, "gmslc_ocnBeta": st.gmslc_ocnBeta, "ice_volume": st.ice_volume,
             "il": np.ctypeslib.as_array(st.il).reshape(3, 3).copy(), "mm": np.ctypeslib.as_array(st.mm).copy(),
-            "lambda": lam[..., 0] + 1j * lam[..., 1], "reguess": st.reguess, "ice_analyses": st.ice_analyses}
+            "lambda": lam[..., 0] + 1j * lam[..., 1], "reguess": st.reguess, "ice_analyses": st.ice_analyses,
+            "sweeps32": st.sweeps32}
 
 
 class SeaLevelSolver:
@@ -122,9 +130,13 @@ class SeaLevelSolver:
 
     def __init__(self, sht, love, topo0, ice0, *, norder=512, dt=1, tpw=True, checkmarine=True, calcRG=True,
                  epsilon1=1.0e-5, planet=None, mem_threads=1, ramp=False, axial_rotation="off",
-                 axial_tau_c=None, inner="off", _anderson=(3, 2, 3, False)):
+                 axial_tau_c=None, inner="off", _anderson=(3, 2, 3, False), precision="fp64", eps_switch=None):
         if axial_rotation != "off":
             raise NotImplementedError("axial_rotation other than 'off' is not provided")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        if precision == "mixed" and inner == "anderson":
+            raise ValueError("precision='mixed' needs inner 'extrap' or 'off'")
         if inner not in INNER_MODES:
             raise ValueError(f"inner must be one of {INNER_MODES}, got {inner!r}")
         if not isinstance(sht, SHT):
@@ -133,11 +145,17 @@ class SeaLevelSolver:
             raise ValueError("norder must equal the transform's lmax")
         if sht.max_fields < 3:
             raise ValueError("the solver needs an SHT with max_fields >= 3")
+        if sht.raw and not getattr(self, "_raw_ok", False):
+            raise ValueError("a raw-mode SHT needs OptSeaLevelSolver(opt={'raw', ...}) (solver_opt.py:299-303)")
         self.sht, self.love = sht, love
         self.norder, self.nl = int(norder), int(norder) + 1
         self.dt = float(dt)
         self.tpw, self.checkmarine, self.calcRG = bool(tpw), bool(checkmarine), bool(calcRG)
         self.epsilon1 = float(epsilon1)
+        self.precision = precision
+        self.eps_switch = float(100.0 * self.epsilon1 if eps_switch is None else eps_switch)
+        if precision == "mixed" and not self.eps_switch > self.epsilon1:
+            raise ValueError("eps_switch must exceed epsilon1")
         self.inner = inner
         self.horizontal = False  # set True for the horizontal read-out (result key "horiz")
         self.ramp = bool(ramp)
@@ -176,7 +194,8 @@ class SeaLevelSolver:
             a_hl=dp(h["a_hl"]), a_thl=dp(h["a_thl"]), ttlle=dp(h["ttlle"]),
             lTE2=float(love.lTE[2]) if norder >= 2 else 0.0, anderson=int(inner == "anderson"),
             anderson_depth=int(_anderson[0]), anderson_freeze=int(_anderson[1]), max_mask_refresh=int(_anderson[2]),
-            anderson_keep=int(bool(_anderson[3])))
+            anderson_keep=int(bool(_anderson[3])), precision=_lib.PRECISIONS[precision],
+            eps_switch=self.eps_switch)
         g = lambda: torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=dev)  # noqa: E731
         self.topoxy, self.cxy, self.beta, self.rrxy, self.Gxy = g(), g(), g(), g(), g()
         outs = _lib.SlxOutputs(tgrid=self.topoxy.data_ptr(), ocean=self.cxy.data_ptr(), beta=self.beta.data_ptr(),
@@ -394,9 +413,11 @@ class SeaLevelSolver:
 
 class OptSeaLevelSolver(SeaLevelSolver):
     """``OptSeaLevelSolver`` signature (solver_opt.py:268-296).  The optimisation switches ``opt``
-    are accepted (the device path always uses the fused forms); ``inner`` may be "off", "extrap"
-    or "anderson" (with anderson_depth 1..5, anderson_freeze, max_mask_refresh, anderson_keep;
-    ``inner_stats`` counts the mixing outcomes over the run)."""
+    (a set, or the --opt string) are validated as the reference does: unknown names raise, and
+    "raw" must come with a raw-mode SHT, whose V = (-1)^m U convention the dense spectral outputs
+    then use (the device path itself always runs the fused forms in the U convention); ``inner``
+    may be "off", "extrap" or "anderson" (with anderson_depth 1..5, anderson_freeze,
+    max_mask_refresh, anderson_keep; ``inner_stats`` counts the mixing outcomes over the run)."""
 
     def __init__(self, *args, opt=(), inner="off", anderson_depth=3, max_mask_refresh=3, anderson_freeze=2,
                  anderson_keep=False, reuse="off", reuse_store=None, **kw):
@@ -404,7 +425,14 @@ class OptSeaLevelSolver(SeaLevelSolver):
             raise ValueError(f"inner must be one of {INNER_MODES}, got {inner!r}")
         if reuse not in ("off", "warm"):
             raise NotImplementedError(f"reuse={reuse!r} is not provided (off | warm)")
-        self.opt = set(opt)
+        self.opt = parse_opts(opt) if isinstance(opt, str) else set(opt)
+        unknown = self.opt - set(ALL_OPTS)
+        if unknown:
+            raise ValueError(f"unknown opt {sorted(unknown)} (known: {','.join(ALL_OPTS)})")
+        sht = args[0] if args else kw.get("sht")
+        if ("raw" in self.opt) != bool(getattr(sht, "raw", False)):
+            raise ValueError("--opt raw must be used with a raw-mode SHT backend")  # solver_opt.py:299-301
+        self._raw_ok = "raw" in self.opt
         self.anderson_depth, self.anderson_freeze = int(anderson_depth), int(anderson_freeze)
         self.max_mask_refresh, self.anderson_keep = int(max_mask_refresh), bool(anderson_keep)
         super().__init__(*args, inner=inner, _anderson=(anderson_depth, anderson_freeze, max_mask_refresh,
