@@ -112,6 +112,9 @@ typedef struct slx_solver_params {
    * state, mass integrals and the convergence norm stay FP64.  Not with anderson = 1. */
   int precision;
   double eps_switch;
+  /* inner iterations after which a step fails (the reference aborts at 9999, solver.py:1049-1050);
+   * <= 0 or > 9999 means 9999.  A failed step is not committed. */
+  int max_inner;
 } slx_solver_params;
 
 typedef struct slx_status {
@@ -196,8 +199,9 @@ int slx_solver_status(slx_solver* s, slx_status* out, void* stream);
  * iteration is enqueued when the device has posted the decision of the previous one (to pinned host
  * memory, as the deciding kernel starts), so no iteration runs past convergence and the host never
  * waits for the stream to drain before the last step.  Outputs of intermediate steps are
- * overwritten; the solver's grids hold the last step's.  Returns -34 (out[0..k] filled) when step
- * k's inner loop fails to converge. */
+ * overwritten; the solver's grids hold the last step's.  Returns -34 when step k's inner loop fails
+ * to converge: out[0..k-1] hold the finished steps, out[k] the failed step's status (failed = 1);
+ * step k is not committed, as the reference raises before its commit. */
 int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, int nsteps, const double* dts,
                   const double* dt_prev, int* guess, slx_status* out, void* stream);
 int slx_solver_fields(slx_solver* s, slx_fields* out);

@@ -10,6 +10,7 @@ _EXPORTS = {
     "SeaLevelSolver": ".solver",
     "OptSeaLevelSolver": ".solver",
     "Planet": ".solver",
+    "InnerLoopError": ".solver",
     "LoveNumbers": ".love",
     "SHT": ".sht",
     "make_sht": ".sht",

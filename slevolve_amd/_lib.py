@@ -25,7 +25,7 @@ class SlxSolverParams(C.Structure):
                 ("ramp", c_int), ("a_hl", P_double), ("a_thl", P_double), ("ttlle", P_double), ("lTE2", c_double),
                 ("anderson", c_int), ("anderson_depth", c_int), ("anderson_freeze", c_int),
                 ("max_mask_refresh", c_int), ("anderson_keep", c_int),
-                ("precision", c_int), ("eps_switch", c_double)]
+                ("precision", c_int), ("eps_switch", c_double), ("max_inner", c_int)]
 
 
 class SlxStatus(C.Structure):

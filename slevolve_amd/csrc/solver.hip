@@ -24,7 +24,7 @@ using namespace slx;
 
 static constexpr double EPS0 = 2.220446049250313e-16;
 static constexpr int GR_MAXACC = 3;
-static constexpr int MAX_INNER = 9999;
+static constexpr int MAX_INNER = 9999;  // the reference's abort count (solver.py:1049-1050)
 static constexpr double PI_LEGACY = 3.1415926535898;
 static constexpr int PSLOT = 1024;  // block-partial slots per reduction (3 of them: 2 iteration parities + guess)
 
@@ -70,6 +70,7 @@ struct Consts {
   double c_il, c_m, c_tsl, c_trr, c_thl;     // sum_k (1 - phi_k) a_k at degree 2 (solver.py:362-367)
   int mixed;                                 // FP32-product sweeps while xi > eps_sw
   double eps_sw;
+  int max_inner;                             // inner iterations before a step fails (<= MAX_INNER)
 };
 
 struct slx_solver {
@@ -118,6 +119,13 @@ struct slx_solver {
   int* hflag = nullptr;
   int* hflag_dev = nullptr;
   int pipelined = 0;
+  // token of the step in progress in the posted decisions: a per-handle count of started steps that
+  // no re-initialisation resets, so that a decision posted before slx_solver_init never matches
+  // a later step
+  unsigned epoch = 0;
+  // per-step status slots of the native step loop (coherent pinned host memory, grown on demand)
+  slx_status* run_stat = nullptr;
+  int run_stat_cap = 0;
   // incremental grounded re-guess (matrix-core path): per-ring token of the last re-analysis that
   // changed the ring [nh], the next token, and whether the incremental form is used
   int* ring_tok = nullptr;
@@ -648,7 +656,7 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
   // a step converges on an FP64 sweep only; FP32-product sweeps run while xi > eps_sw
   const int fp64 = *(volatile const int*)&s->sweep64[it & 1];
   const bool conv = xi <= c.eps1 && fp64;
-  const bool fail = !conv && it + 1 >= MAX_INNER;
+  const bool fail = !conv && it + 1 >= c.max_inner;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     post_decision(p, it, conv || fail, fail);
     s->sweep64[(it + 1) & 1] = (fp64 || xi <= c.eps_sw) ? 1 : 0;
@@ -885,7 +893,7 @@ __global__ void k_decide_aa(Pts p, Consts c, int it) {
     }
     mask = accept = true;
   }
-  if (!s->st.done && it + 1 >= MAX_INNER) {
+  if (!s->st.done && it + 1 >= c.max_inner) {
     s->st.failed = 1;
     s->st.done = 1;
     mask = mix = push = accept = false;
@@ -1118,8 +1126,11 @@ __global__ void k_aa_plain(Pts p, Consts c, int it) {
 
 // ---------------------------------------------------------------------- step finish
 // commit (solver.py:1080-1095): load increment without degree 0 into P; ice, ocean, previous ice
+// (a step whose inner loop failed is not committed: the reference raises before its commit,
+// solver.py:1049-1050)
 __device__ __forceinline__ bool finish_active(const DevState* s) {
-  return *(volatile const int*)&s->st.done && !*(volatile const int*)&s->st.finished;
+  return *(volatile const int*)&s->st.done && !*(volatile const int*)&s->st.finished &&
+         !*(volatile const int*)&s->st.failed;
 }
 
 __global__ void k_commit(Pts p, Consts c, int extrap) {
@@ -1418,7 +1429,7 @@ __global__ void k_init_areas(Pts p, Consts c, const double2* c0, const double2* 
 static Pts make_pts(slx_solver* s, const double* ice_cur, const double2* oldice) {
   Pts p;
   p.hflag = s->pipelined ? s->hflag_dev : nullptr;
-  p.htoken = s->step & 0x1fffffff;
+  p.htoken = (int)(s->epoch & 0x1fffffffu);
   p.hstat = s->hstat_dev;
   p.st = s->st;
   p.partial = s->partial;
@@ -1535,6 +1546,7 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   SLX_CHECK(p->precision == SLX_PREC_FP64 || p->precision == SLX_PREC_FP32, "precision must be 0 or 1");
   c.mixed = p->precision == SLX_PREC_FP32 ? 1 : 0;
   c.eps_sw = p->eps_switch;
+  c.max_inner = p->max_inner > 0 ? std::min(p->max_inner, MAX_INNER) : MAX_INNER;
   SLX_CHECK(!c.mixed || (!c.aa && p->eps_switch > p->epsilon1 && h->ana_mfma),
             "mixed-precision sweeps need inner extrap or off and eps_switch > epsilon1");
   SLX_CHECK(!c.aa || (p->extrap && p->anderson_depth >= 1 && p->anderson_depth <= 5 && p->anderson_freeze >= 1 &&
@@ -1687,6 +1699,7 @@ extern "C" int slx_solver_destroy(slx_solver* s) {
     if (q) (void)hipFree(q);
   if (s->pinned) (void)hipHostFree(s->pinned);
   if (s->hflag) (void)hipHostFree(s->hflag);
+  if (s->run_stat) (void)hipHostFree(s->run_stat);
   if (s->ring_tok) (void)hipFree(s->ring_tok);
   delete s;
   return 0;
@@ -1828,6 +1841,7 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
   hipStream_t st = (hipStream_t)stream;
   SLX_CHECK(s->closed, "slx_step_begin: the previous step has not finished (check slx_solver_status)");
   s->step += 1;
+  s->epoch += 1;
   s->closed = 0;
   s->iters_enqueued = 0;
   s->ice_cur = ice_cur;
@@ -2132,7 +2146,7 @@ extern "C" int slx_solver_status(slx_solver* s, slx_status* out, void* stream) {
 // ended without posting (or an error) ends the wait.
 static int wait_decision(slx_solver* s, int it, hipStream_t st, int& code) {
   const volatile int* f = s->hflag + it;
-  const int token = s->step & 0x1fffffff;
+  const int token = (int)(s->epoch & 0x1fffffffu);
   for (long spin = 1;; ++spin) {
     int v = *f;
     if ((v >> 2) == token && (v & 3)) {
@@ -2176,15 +2190,18 @@ extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, 
   SLX_CHECK(nsteps >= 0 && ice_stride >= s->c.npts, "slx_run_steps: bad nsteps or ice_stride");
   if (nsteps == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  slx_status* hst = nullptr;
-  SLX_HIP(hipHostMalloc((void**)&hst, (size_t)nsteps * sizeof(slx_status), hipHostMallocCoherent));
+  if (nsteps > s->run_stat_cap) {  // grown on demand; later calls of at most this length allocate nothing
+    SLX_HIP(hipStreamSynchronize(st));
+    if (s->run_stat) SLX_HIP(hipHostFree(s->run_stat));
+    s->run_stat = nullptr;
+    s->run_stat_cap = 0;
+    SLX_HIP(hipHostMalloc((void**)&s->run_stat, (size_t)nsteps * sizeof(slx_status), hipHostMallocCoherent));
+    s->run_stat_cap = nsteps;
+  }
+  slx_status* hst = s->run_stat;
   std::memset(hst, 0, (size_t)nsteps * sizeof(slx_status));
   slx_status* hst_dev = nullptr;
-  hipError_t e = hipHostGetDevicePointer((void**)&hst_dev, hst, 0);
-  if (e != hipSuccess) {
-    (void)hipHostFree(hst);
-    SLX_HIP(e);
-  }
+  SLX_HIP(hipHostGetDevicePointer((void**)&hst_dev, hst, 0));
   double d1 = dt_prev[0], d2 = dt_prev[1];
   int rc = 0, failed_at = -1;
   s->pipelined = 1;
@@ -2211,14 +2228,14 @@ extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, 
       }
     }
     if (rc) break;
-    s->hstat_dev = hst_dev + k;  // a failed step is finished too, as by the step-by-step path
-    rc = slx_step_finish(s, stream);
-    s->hstat_dev = nullptr;
-    if (!rc) step_close(s);
-    if (code == 3) {
+    if (code == 3) {  // not finished, not committed; its status is read after the stream drains
       failed_at = k;
       break;
     }
+    s->hstat_dev = hst_dev + k;
+    rc = slx_step_finish(s, stream);
+    s->hstat_dev = nullptr;
+    if (!rc) step_close(s);
     d2 = d1;
     d1 = dts[k];
   }
@@ -2230,7 +2247,8 @@ extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, 
     rc = -36;
   }
   if (!rc && failed_at >= 0) {
-    for (int k = 0; k <= failed_at; ++k) out[k] = hst[k];
+    for (int k = 0; k < failed_at; ++k) out[k] = hst[k];
+    SLX_HIP(hipMemcpy(&out[failed_at], &s->st->st, sizeof(slx_status), hipMemcpyDeviceToHost));
     slx_set_error("slx_run_steps: inner loop failed to converge at step " + std::to_string(out[failed_at].step));
     rc = -34;
   } else if (!rc) {
@@ -2244,7 +2262,6 @@ extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, 
     }
     if (!rc) *guess = std::max(1, out[nsteps - 1].numiter);
   }
-  (void)hipHostFree(hst);
   return rc;
 }
 

@@ -125,12 +125,24 @@ def _status_record(st):
             "sweeps32": st.sweeps32}
 
 
+class InnerLoopError(RuntimeError):
+    """An inner fixed point did not converge within the iteration limit (solver.py:1049-1050).
+    ``records`` holds the scalar records of the steps a ``run`` finished before the failing one,
+    ``status`` the failing step's (not committed, as in the reference)."""
+
+    def __init__(self, msg, records=(), status=None):
+        super().__init__(msg)
+        self.records = list(records)
+        self.status = status
+
+
 class SeaLevelSolver:
     """Exact-memory sea-level solver; constructor and ``step`` as in solver.py:521-1144."""
 
     def __init__(self, sht, love, topo0, ice0, *, norder=512, dt=1, tpw=True, checkmarine=True, calcRG=True,
                  epsilon1=1.0e-5, planet=None, mem_threads=1, ramp=False, axial_rotation="off",
-                 axial_tau_c=None, inner="off", _anderson=(3, 2, 3, False), precision="fp64", eps_switch=None):
+                 axial_tau_c=None, inner="off", _anderson=(3, 2, 3, False), precision="fp64", eps_switch=None,
+                 _max_inner=9999):
         if axial_rotation != "off":
             raise NotImplementedError("axial_rotation other than 'off' is not provided")
         if precision not in ("fp64", "mixed"):
@@ -195,7 +207,7 @@ class SeaLevelSolver:
             lTE2=float(love.lTE[2]) if norder >= 2 else 0.0, anderson=int(inner == "anderson"),
             anderson_depth=int(_anderson[0]), anderson_freeze=int(_anderson[1]), max_mask_refresh=int(_anderson[2]),
             anderson_keep=int(bool(_anderson[3])), precision=_lib.PRECISIONS[precision],
-            eps_switch=self.eps_switch)
+            eps_switch=self.eps_switch, max_inner=int(_max_inner))
         g = lambda: torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=dev)  # noqa: E731
         self.topoxy, self.cxy, self.beta, self.rrxy, self.Gxy = g(), g(), g(), g(), g()
         outs = _lib.SlxOutputs(tgrid=self.topoxy.data_ptr(), ocean=self.cxy.data_ptr(), beta=self.beta.data_ptr(),
@@ -310,8 +322,9 @@ class SeaLevelSolver:
             else:
                 _lib.call("slx_step_finish_readouts", h, C.byref(ro), sp)
             st = self._status()
-            if st.failed:
-                raise RuntimeError(f"inner loop failed to converge, xi={st.xi}")
+            if st.failed:  # the step is not committed (its finish kernels skip a failed step)
+                raise InnerLoopError(f"inner loop failed to converge after {st.numiter} iterations, xi={st.xi}",
+                                     status=_status_record(st))
             if st.finished:
                 break
             n = 1 if total < 16 else max(1, total // 2)
@@ -395,18 +408,31 @@ class SeaLevelSolver:
         guess = C.c_int(self._guess_iters)
         out = (_lib.SlxStatus * n)()
         stride = self.nlat * self.nlon
-        _lib.call("slx_run_steps", self._s_handle, C.c_void_p(ice.data_ptr() + k0 * stride * 8), stride, n,
-                  d.ctypes.data_as(_lib.P_double), prev.ctypes.data_as(_lib.P_double), C.byref(guess), out,
-                  D.stream_ptr())
+        rc = _lib.load().slx_run_steps(self._s_handle, C.c_void_p(ice.data_ptr() + k0 * stride * 8), stride, n,
+                                       d.ctypes.data_as(_lib.P_double), prev.ctypes.data_as(_lib.P_double),
+                                       C.byref(guess), out, D.stream_ptr())
+        if rc == -34:  # step k0 + j failed: steps before it finished and were committed
+            j = next(i for i in range(n) if out[i].failed)
+            if j > 0:
+                self._advance_bookkeeping(out, d, j)
+            self.iter += 1  # the failed step counts as started, as in the reference's step()
+            raise InnerLoopError(_lib.load().slx_last_error().decode(errors="replace"),
+                                 records=[_status_record(x) for x in out[:j]], status=_status_record(out[j]))
+        _lib.check(rc, "slx_run_steps")
+        return self._advance_bookkeeping(out, d, n, guess=guess.value)
+
+    def _advance_bookkeeping(self, out, d, n, guess=None):
+        """Host state after n finished steps of the native loop; returns their records."""
+        outs = list(out)[:n]
         self.iter += n
-        self._guess_iters = max(1, guess.value)
-        self._recent = ([x.numiter for x in out][::-1] + self._recent)[:4]
-        self.dt = float(d[-1])
-        self._dt_hist = [float(d[-1]), float(d[-2]) if n >= 2 else self._dt_hist[0]]
-        st = out[n - 1]
+        self._guess_iters = max(1, guess if guess is not None else outs[-1].numiter)
+        self._recent = ([x.numiter for x in outs][::-1] + self._recent)[:4]
+        self.dt = float(d[n - 1])
+        self._dt_hist = [float(d[n - 1]), float(d[n - 2]) if n >= 2 else self._dt_hist[0]]
+        st = outs[-1]
         self._last_status = st
         self.gr_counts = {"reguess": st.reguess, "ice_analyses": st.ice_analyses}
-        rec = [_status_record(x) for x in out]
+        rec = [_status_record(x) for x in outs]
         self.il, self.mm, self.lam = rec[-1]["il"], rec[-1]["mm"], rec[-1]["lambda"]
         return rec
 
@@ -457,4 +483,5 @@ class OptSeaLevelSolver(SeaLevelSolver):
                 "mask_refresh": st.aa_mask_refresh, "frozen_iters": st.aa_frozen_iters}
 
 
-__all__ = ["SeaLevelSolver", "OptSeaLevelSolver", "Planet", "PI", "EPS0", "GR_MAXACC", "parse_opts"]
+__all__ = ["SeaLevelSolver", "OptSeaLevelSolver", "InnerLoopError", "Planet", "PI", "EPS0", "GR_MAXACC",
+           "parse_opts"]

@@ -44,4 +44,4 @@ def test_struct_layouts_match_the_header():
     """The ctypes mirrors of slx_solver_params and slx_status end where the C structs do."""
     assert ctypes.sizeof(_lib.SlxStatus) % 8 == 0
     assert _lib.SlxStatus.sweeps32.offset + 8 == ctypes.sizeof(_lib.SlxStatus)
-    assert _lib.SlxSolverParams.eps_switch.offset + 8 == ctypes.sizeof(_lib.SlxSolverParams)
+    assert _lib.SlxSolverParams.max_inner.offset + 8 == ctypes.sizeof(_lib.SlxSolverParams)
