@@ -55,6 +55,16 @@ int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* coef, void* s
  * ring spectra left in the handle's work buffers by the last analysis of nf fields (no
  * counterpart in the reference; lets a profiler time the contraction by itself). */
 int slx_sht_analysis_contraction(slx_sht* h, int nf, void* coef, void* stream);
+/* slx_sht_analysis that allocates nothing and uses no work buffer of the handle: the folded ring
+ * spectra go to a static device array of SLX_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends), which limits 2 nf (lmax+1) (nlat/2) complex doubles to that size (24 MiB: up to
+ * three fields at lmax 512 on 512 x 1024, one at lmax 1023 on 1024 x 2048); larger requests return -22
+ * with a message.  No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid it).  Every call on
+ * one GPU shares that GPU's array, so calls that overlap in time on one GPU (other streams or
+ * host threads) race.  Same results as slx_sht_analysis, bit for bit. */
+#define SLX_NOALLOC_SCRATCH_BYTES (24u << 20)
+int slx_sht_analysis_noalloc(slx_sht* h, const double* grid, int nf, void* coef, void* stream);
 /* Replaces SHT.synthesis (sht.py:305-343) / DuccSHT.synthesis (sht_ducc.py:156-187): coef
  * (device) nf x [ncoef] -> grid (device) nf x [nlat][nlon].  Im of the m = 0 (and Nyquist)
  * coefficients is ignored, as irfft does. */

@@ -62,6 +62,7 @@ SIGNATURES = {
     "slx_sht_info": (c_int, [c_void_p, P_int, P_double, P_double]),
     "slx_sht_analysis": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "slx_sht_analysis_contraction": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "slx_sht_analysis_noalloc": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "slx_sht_synthesis": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "slx_sht_tri_to_dense": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "slx_sht_dense_to_tri": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "sht_launch.h"
@@ -438,6 +439,66 @@ extern "C" int slx_sht_analysis(slx_sht* h, const double* grid, int nf, void* co
   int rc = launch_fft_ana(h, LoadPlain{grid, gstride}, nf, Pred{}, (hipStream_t)stream);
   if (rc) return rc;
   return slx_sht_analysis_contraction(h, nf, coef, stream);
+}
+
+// ------------------------------------------------------------------ allocation-free analysis
+// The folded-spectra work area of slx_sht_analysis_noalloc lives in this file-scope device array
+// (one copy per GPU, held until the process ends).  Host code never passes `scratch` itself to a
+// kernel: its address on each device comes from hipGetSymbolAddress, taken once per device with
+// that device current and kept in scratch_ptr.
+static __device__ __align__(256) double2 scratch[SLX_NOALLOC_SCRATCH_BYTES / sizeof(double2)];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static int scratch_of_current_device(double2** out) {
+  int dev = 0;
+  SLX_HIP(hipGetDevice(&dev));
+  SLX_CHECK(dev >= 0 && dev < kMaxDevices, "slx_sht_analysis_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    SLX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SLX_CHECK(ptr != nullptr, "slx_sht_analysis_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<double2*>(g_scratch_ptr[dev]);
+  return 0;
+}
+
+extern "C" int slx_sht_analysis_noalloc(slx_sht* h, const double* grid, int nf, void* coef_v, void* stream) {
+  double2* coef = static_cast<double2*>(coef_v);
+  SLX_CHECK(h && grid && coef, "slx_sht_analysis_noalloc: null argument");
+  SLX_CHECK(nf >= 1 && nf <= h->max_fields, "slx_sht_analysis_noalloc: nf out of range");
+  const size_t need = 2 * (size_t)nf * h->nm * h->nh * sizeof(double2);
+  if (need > SLX_NOALLOC_SCRATCH_BYTES) {
+    slx_set_error("slx_sht_analysis_noalloc: " + std::to_string(nf) + " field(s) at lmax " + std::to_string(h->lmax) +
+                  ", nlat " + std::to_string(h->nlat) + " need " + std::to_string(need) +
+                  " bytes of work area, more than the " + std::to_string((size_t)SLX_NOALLOC_SCRATCH_BYTES) +
+                  " of the static scratch");
+    return -22;
+  }
+  double2* ws = nullptr;
+  if (int rc = scratch_of_current_device(&ws)) return rc;
+  double2* wd = ws + (size_t)nf * h->nm * h->nh;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_fft_ana(h, LoadPlain{grid, (long)h->nlat * h->nlon}, nf, Pred{}, s, ws, wd);
+  if (rc) return rc;
+  const long n = h->ncoef;
+  if (h->precision == SLX_PREC_FP32) {
+    switch (nf) {
+      case 1: return launch_leg_ana32<1>(h, ws, wd, OutPtrs<1>{{coef}}, Pred{}, s);
+      case 2: return launch_leg_ana32<2>(h, ws, wd, OutPtrs<2>{{coef, coef + n}}, Pred{}, s);
+      default: return launch_leg_ana32<3>(h, ws, wd, OutPtrs<3>{{coef, coef + n, coef + 2 * n}}, Pred{}, s);
+    }
+  }
+  switch (nf) {
+    case 1: return launch_leg_ana<1>(h, ws, wd, OutPtrs<1>{{coef}}, Pred{}, s);
+    case 2: return launch_leg_ana<2>(h, ws, wd, OutPtrs<2>{{coef, coef + n}}, Pred{}, s);
+    default: return launch_leg_ana<3>(h, ws, wd, OutPtrs<3>{{coef, coef + n, coef + 2 * n}}, Pred{}, s);
+  }
 }
 
 extern "C" int slx_sht_synthesis(slx_sht* h, const void* coef_v, int nf, double* grid, void* stream) {

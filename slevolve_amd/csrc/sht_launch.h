@@ -165,17 +165,22 @@ int allow_lds(size_t lds) {
 inline int ring_points(const slx_sht* h) { return h->nlon <= 4 * RING_MAXNT ? 4 : 8; }
 inline size_t ring_lds(const slx_sht* h) { return 2 * (size_t)h->nlon * sizeof(double2); }
 
+// (WS, WD: the folded-spectra work area, the handle's unless given)
 template <int P, class Loader>
-int launch_fft_ana_p(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {
+int launch_fft_ana_p(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s, double2* WS, double2* WD) {
   const size_t lds = ring_lds(h);
   if (int rc = allow_lds<&k_fft_ana<P, Loader>>(lds)) return rc;
-  k_fft_ana<P, Loader><<<dim3(h->nh, nf), h->nlon / P, lds, s>>>(dev_of(h), ld, h->WS, h->WD, pr);
+  k_fft_ana<P, Loader><<<dim3(h->nh, nf), h->nlon / P, lds, s>>>(dev_of(h), ld, WS, WD, pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }
 template <class Loader>
-int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s) {
-  return ring_points(h) == 4 ? launch_fft_ana_p<4>(h, ld, nf, pr, s) : launch_fft_ana_p<8>(h, ld, nf, pr, s);
+int launch_fft_ana(const slx_sht* h, Loader ld, int nf, Pred pr, hipStream_t s, double2* WS = nullptr,
+                   double2* WD = nullptr) {
+  if (!WS) WS = h->WS;
+  if (!WD) WD = h->WD;
+  return ring_points(h) == 4 ? launch_fft_ana_p<4>(h, ld, nf, pr, s, WS, WD)
+                             : launch_fft_ana_p<8>(h, ld, nf, pr, s, WS, WD);
 }
 template <int P, class Store>
 int launch_fft_syn_p(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
