@@ -1,21 +1,31 @@
 #!/usr/bin/env python3
-"""Solver steps/sec of the sea-level time step on one MI355X.
+"""Solver steps/sec of the sea-level time step on one MI355X, FP64 and mixed precision, at
+lmax 512 and lmax 1023, with the accuracy condition of the metric measured in the same run.
 
-Workload (default "lmax512"): Gauss-Legendre 512 x 1024 grid, lmax 512, 30 viscous modes per
-degree, 550 steps of 1 yr (seeded synthetic Love numbers, topography to degree 256, a polar cap and
-a retreating marine sector), FP64, extrapolated inner start, rotation and marine check on,
-epsilon1 = 1e-5.  A "step" is one solver step (modal advance, inner fixed point to convergence,
-radial displacement, commit).  All ice grids are uploaded before the timed region; W warm-up steps
-(SeaLevelSolver.step), then K timed steps through the native step loop (SeaLevelSolver.run)
-bracketed by device synchronisation.  Prints one JSON line.
+Headline (``value``): workload "lmax512" = Gauss-Legendre 512 x 1024 grid, lmax 512, 30 viscous
+modes per degree, 550 steps of 1 yr (seeded synthetic Love numbers, topography to degree 256, a
+polar cap and a retreating marine sector), FP64, extrapolated inner start, rotation and marine
+check on, epsilon1 = 1e-5.  A "step" is one solver step (modal advance, inner fixed point to
+convergence, radial displacement, commit).  All ice grids are uploaded before the timed region;
+W warm-up steps (SeaLevelSolver.step), then K timed steps through the native step loop
+(SeaLevelSolver.run), each timed segment bracketed by device synchronisation; the run is cut into
+segments at the steps where the oracle's long-run fixture stores fields (tests/golden), and the
+fields are read there outside the timed region.
 
-Also reported: the roofline of the dominant kernel (the per-order Legendre analysis contraction),
-timed live with HIP events on the stream it is launched on, with the HBM traffic per launch from
-the committed rocprofv3 counter summary; the CPU oracle's steps/s on a bounded sample of the same
-workload on this host; and, in a default run, the 1024 x 2048, lmax 1023 grid (240 x 0.5 kyr,
-three ice caps) in FP64 under "also".
+Also measured in a default run (every leg in full whatever --steps is, unless --no-also):
+  * lmax512 FP64 over the whole workload (when --steps shortens the headline);
+  * lmax512 and lmax1023 (1024 x 2048, 240 x 0.5 kyr, three caps) in FP64 and in mixed precision
+    (FP32-product Legendre contractions in the early sweeps of each step, FP64 to convergence);
+  * accuracy: FP64 runs against the oracle's stored fields and per-step iteration counts; mixed
+    runs against the FP64 GPU run of the same steps (max |dSL|, |dR| in m, max |dm|/|m|, steps
+    whose iteration count differs);
+  * the roofline of the dominant kernel (the per-order Legendre analysis contraction), timed live
+    with HIP events on the stream it is launched on, with the HBM traffic per launch from the
+    committed rocprofv3 counter summary;
+  * the CPU baseline on a bounded sample of the same workload on this host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lmax512|lmax1023|lmax63]
+                    [--precision fp64|mixed] [--no-also] [--no-cpu]
 """
 import argparse
 import json
@@ -29,6 +39,7 @@ sys.path.insert(0, ROOT)
 METRIC = "solver steps/sec at lmax=512/1023, mixed vs FP64 GPU, at max|ΔSL|≤1e-6 m"
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 def parse():
@@ -37,12 +48,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's steps - warmup)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="lmax512", choices=("lmax63", "lmax512", "lmax1023"))
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU oracle sample")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"), help="precision of the headline")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-also", action="store_true", help="only the headline leg")
     ap.add_argument("--inner", default="extrap", choices=("off", "extrap", "anderson"),
                     help="inner start / acceleration (the metric is quoted with extrap)")
-    ap.add_argument("--kernels-only", action="store_true",
-                    help="only launch the dominant kernel (20x) and a 256 MiB device copy (for PMC counter runs)")
+    ap.add_argument("--kernels-only", default=None,
+                    choices=("ana2", "ana2x32", "syn1", "syn1x32", "modal"),
+                    help="only launch one kernel 20x (the 2-field analysis contraction, FP64 or FP32-product; the "
+                         "1-field synthesis contraction; the modal advance) and a 256 MiB device copy, for PMC runs")
     return ap.parse_args()
 
 
@@ -61,21 +76,19 @@ def measured_traffic(workload, path_kernel, kernel):
     return None, None
 
 
-def cpu_baseline(w, ice, budget):
+def cpu_baseline(w, budget):
     """The FP64 NumPy oracle on this host: set-up, then as many steps as fit in `budget` seconds
     (at least 1) of the same workload; steps/s over the timed steps."""
-    import numpy as np
-
     from oracle.love import LoveNumbers as OLove
     from oracle.sht import OracleSHT
     from oracle.solver import SeaLevelSolver as OSolver
 
-    slv = OSolver(OracleSHT(w.nlat, w.nlon, w.lmax), OLove(w.love_text, w.lmax, text=True), w.topo, ice(0),
+    slv = OSolver(OracleSHT(w.nlat, w.nlon, w.lmax), OLove(w.love_text, w.lmax, text=True), w.topo, w.ice(0),
                   norder=w.lmax, dt=w.dt, epsilon1=1e-5, inner="extrap")
     t0 = time.perf_counter()
     n = 0
     while True:
-        slv.step(ice(n + 1))
+        slv.step(w.ice(n + 1))
         n += 1
         if time.perf_counter() - t0 > budget or n >= w.nsteps:
             break
@@ -90,46 +103,167 @@ def cpu_baseline(w, ice, budget):
                       f"{os.environ.get('OMP_NUM_THREADS', 'default')}), {el:.1f} s"}
 
 
-def run_workload(w, warmup, steps, dev, inner="extrap"):
-    """W warm-up steps, then K timed steps of workload w with every ice grid resident in HBM,
-    bracketed by device synchronisation.  Returns (steps/s, seconds, K, numiter list, transform)."""
+def fixture(name):
+    """The oracle's long-run fixture of workload `name` (tests/golden/oracle_<name>_long.npz) or None."""
     import numpy as np
+
+    path = os.path.join(GOLDEN, f"oracle_{name}_long.npz")
+    return dict(np.load(path)) if os.path.exists(path) else None
+
+
+class Leg:
+    """One timed run of a workload: warm-up steps through step(), then the timed steps through the
+    native loop in segments that end at the checkpoint steps, where tgrid, R and m are kept."""
+
+    def __init__(self, w, ice_dev, dev, precision, inner, warmup, steps, checkpoints):
+        import numpy as np
+        import torch
+
+        from slevolve_amd import SHT, LoveNumbers, OptSeaLevelSolver
+
+        last = warmup + steps
+        sht = SHT(w.nlat, w.nlon, w.lmax, device=dev)
+        slv = OptSeaLevelSolver(sht, LoveNumbers(w.love_text, w.lmax, text=True), w.topo, w.ice(0), norder=w.lmax,
+                                dt=w.dt, tpw=True, checkmarine=True, calcRG=True, epsilon1=1e-5, inner=inner,
+                                precision=precision)
+        self.numiter, self.sweeps32, self.fields = [], [], {}
+        cps = sorted({k for k in checkpoints if warmup < k <= last} | {last})
+        for k in range(1, warmup + 1):
+            r = slv.step(ice_dev[k])
+            self.numiter.append(r["numiter"])
+            self.sweeps32.append(slv._last_status.sweeps32)
+            if k in checkpoints:
+                self.fields[k] = (r["tgrid"].clone(), r["R"].clone(), np.array(r["mm"]))
+        el, k0 = 0.0, warmup + 1
+        for cp in cps:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rec = slv.run(ice_dev, k0, cp + 1)
+            torch.cuda.synchronize()
+            el += time.perf_counter() - t0
+            self.numiter += [x["numiter"] for x in rec]
+            self.sweeps32 += [x["sweeps32"] for x in rec]
+            self.fields[cp] = (slv.topoxy.clone(), slv.rrxy.clone(), np.array(rec[-1]["mm"]))
+            k0 = cp + 1
+        self.elapsed, self.steps, self.warmup = el, steps, warmup
+        self.sps = steps / el
+        self.timed_numiter = self.numiter[warmup:]
+        self.sht_kernels = dict(sht.kernels)
+        del slv, sht
+        torch.cuda.synchronize()
+        print(f"[bench] {w.name} {precision}: {steps} steps in {el:.3f} s ({self.sps:.1f} steps/s), numiter mean "
+              f"{np.mean(self.timed_numiter):.2f}", file=sys.stderr, flush=True)
+
+    def summary(self, w, precision, inner):
+        import numpy as np
+
+        out = {"workload": describe(w, inner, precision), "value": self.sps, "unit": "steps/s", "steps": self.steps,
+               "warmup": self.warmup, "ms_per_step": 1e3 * self.elapsed / self.steps,
+               "numiter_mean": float(np.mean(self.timed_numiter)), "numiter_total": int(np.sum(self.timed_numiter))}
+        if precision == "mixed":
+            n = max(1, int(np.sum(self.timed_numiter)))
+            out["fp32_sweep_fraction"] = float(np.sum(self.sweeps32[self.warmup:]) / n)
+        return out
+
+
+def vs_oracle(leg, fx):
+    """FP64 leg against the oracle's long-run fixture: max |dtgrid|, |dR| on the stored subsample at
+    the checkpoints reached, steps whose iteration count differs."""
+    import numpy as np
+
+    if fx is None:
+        return {"note": "no oracle fixture for this workload"}
+    n = min(len(leg.numiter), len(fx["numiter"]))
+    res = {"reference": "FP64 NumPy oracle (tests/golden long-run fixture)", "steps_compared": n,
+           "numiter_mismatch_steps": int(np.sum(np.asarray(leg.numiter[:n]) != fx["numiter"][:n])),
+           "checkpoints": []}
+    s = int(fx["sub"])
+    for i, k in enumerate(fx["check"]):
+        k = int(k)
+        if k not in leg.fields:
+            continue
+        t, r, mm = leg.fields[k]
+        dt = float(np.abs(t.cpu().numpy()[::s, ::s] - fx["tgrid_sub"][i]).max())
+        dr = float(np.abs(r.cpu().numpy()[::s, ::s] - fx["R_sub"][i]).max())
+        dm = float(np.abs(mm - fx["mm"][k - 1]).max() / max(np.abs(fx["mm"][k - 1]).max(), 1e-30))
+        res["checkpoints"].append({"step": k, "max_dSL_m": dt, "max_dR_m": dr, "max_dm_rel": dm})
+    if res["checkpoints"]:
+        res["max_dSL_m"] = max(c["max_dSL_m"] for c in res["checkpoints"])
+        res["max_dR_m"] = max(c["max_dR_m"] for c in res["checkpoints"])
+        res["max_dm_rel"] = max(c["max_dm_rel"] for c in res["checkpoints"])
+    return res
+
+
+def vs_fp64(mixed, fp64):
+    """Mixed leg against the FP64 GPU leg of the same steps (full grids at the common checkpoints)."""
+    import numpy as np
+
+    n = min(len(mixed.numiter), len(fp64.numiter))
+    res = {"reference": "FP64 GPU run of the same steps", "steps_compared": n,
+           "numiter_mismatch_steps": int(np.sum(np.asarray(mixed.numiter[:n]) != np.asarray(fp64.numiter[:n]))),
+           "checkpoints": []}
+    for k in sorted(set(mixed.fields) & set(fp64.fields)):
+        (ta, ra, ma), (tb, rb, mb) = mixed.fields[k], fp64.fields[k]
+        res["checkpoints"].append({"step": k, "max_dSL_m": float((ta - tb).abs().max().item()),
+                                   "max_dR_m": float((ra - rb).abs().max().item()),
+                                   "max_dm_rel": float(np.abs(ma - mb).max() / max(np.abs(mb).max(), 1e-30))})
+    for key in ("max_dSL_m", "max_dR_m", "max_dm_rel"):
+        res[key] = max((c[key] for c in res["checkpoints"]), default=None)
+    res["meets_1e-6_m"] = bool(res["max_dSL_m"] is not None and res["max_dSL_m"] <= 1e-6)
+    return res
+
+
+def describe(w, inner="extrap", precision="fp64"):
+    prec = "FP64" if precision == "fp64" else "mixed (FP32-product contractions in early sweeps, FP64 to convergence)"
+    return (f"{w.name}: {w.nlat}x{w.nlon} GL grid, lmax {w.lmax}, {w.nmodes} modes/degree, "
+            f"{w.nsteps} x {w.dt:g} yr, {prec}, inner {inner}, tpw+checkmarine on, eps1 1e-5")
+
+
+def upload(w, nsteps, dev):
     import torch
 
-    from slevolve_amd import SHT, LoveNumbers, OptSeaLevelSolver
-
-    steps = steps if steps is not None else w.nsteps - warmup
-    nsteps = warmup + steps
-    if nsteps > w.nsteps:
-        raise SystemExit(f"warmup + steps = {nsteps} exceeds the workload's {w.nsteps} steps")
     t0 = time.perf_counter()
     ice_dev = torch.empty((nsteps + 1, w.nlat, w.nlon), dtype=torch.float64, device=dev)
     for k in range(nsteps + 1):
         ice_dev[k].copy_(torch.from_numpy(w.ice(k)))
     print(f"[bench] {w.name}: inputs on the device ({ice_dev.numel() * 8 / 1e9:.2f} GB, "
           f"{time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    sht = SHT(w.nlat, w.nlon, w.lmax, device=dev)
-    love = LoveNumbers(w.love_text, w.lmax, text=True)
-    slv = OptSeaLevelSolver(sht, love, w.topo, w.ice(0), norder=w.lmax, dt=w.dt, tpw=True, checkmarine=True,
-                            calcRG=True, epsilon1=1e-5, inner=inner)
-    for k in range(1, warmup + 1):
-        slv.step(ice_dev[k])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rec = slv.run(ice_dev, warmup + 1, nsteps + 1)  # the native step loop
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    numiter = [r["numiter"] for r in rec]
-    print(f"[bench] {w.name}: {steps} steps in {el:.3f} s, numiter mean {np.mean(numiter):.2f}", file=sys.stderr,
-          flush=True)
-    del slv, ice_dev
-    torch.cuda.synchronize()
-    return steps / el, el, steps, numiter, sht
+    return ice_dev
 
 
-def describe(w, inner="extrap"):
-    return (f"{w.name}: {w.nlat}x{w.nlon} GL grid, lmax {w.lmax}, {w.nmodes} modes/degree, "
-            f"{w.nsteps} x {w.dt:g} yr, FP64, inner {inner}, tpw+checkmarine on, eps1 1e-5")
+def kernels_only(w, which, dev):
+    """Launch one contraction 20x (and a 256 MiB copy as the counter scale), for rocprofv3 --pmc."""
+    import numpy as np
+    import torch
+
+    from slevolve_amd import SHT
+    from slevolve_amd import _device as D
+    from slevolve_amd import _lib
+
+    if which == "modal":
+        from slevolve_amd import LoveNumbers
+        from slevolve_amd.memory import ExactMemory
+
+        mem = ExactMemory(LoveNumbers(w.love_text, w.lmax, text=True), w.lmax, w.dt, device=dev)
+        mem._p_load.copy_(torch.randn_like(mem._p_load))
+        for _ in range(20):
+            mem.advance_and_read_packed()
+    else:
+        sht = SHT(w.nlat, w.nlon, w.lmax, device=dev, precision="mixed" if which.endswith("x32") else "fp64")
+        grids = torch.from_numpy(np.stack([w.topo, w.ice(1)])).to(dev)
+        coef = torch.zeros((2, sht.ncoef), dtype=torch.complex128, device=dev)
+        sht.analysis_packed(grids, coef)
+        for _ in range(20):
+            if which.startswith("ana"):
+                sht.analysis_contraction(2, coef)
+            else:
+                _lib.call("slx_sht_synthesis", sht.handle, coef.data_ptr(), 1, grids.data_ptr(), D.stream_ptr())
+    src = torch.ones(32 * 2**20, dtype=torch.float64, device=dev)  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_only": which, "copy_bytes": src.numel() * 8}))
 
 
 def main():
@@ -145,25 +279,20 @@ def main():
     from slevolve_amd import synthetic as syn
 
     w = syn.workload(a.workload)
-    if a.kernels_only:
-        dev = torch.device("cuda", 0)
-        sht = SHT(w.nlat, w.nlon, w.lmax, device=dev)
-        grids = torch.from_numpy(np.stack([w.topo, w.ice(1)])).to(dev)
-        coef = torch.zeros((2, sht.ncoef), dtype=torch.complex128, device=dev)
-        sht.analysis_packed(grids, coef)
-        for _ in range(20):
-            sht.analysis_contraction(2, coef)
-        src = torch.ones(32 * 2**20, dtype=torch.float64, device=dev)  # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"kernels_only": True, "copy_bytes": src.numel() * 8}))
-        return
     dev = torch.device("cuda", 0)
-    sps, el, steps, numiter, sht = run_workload(w, a.warmup, a.steps, dev, a.inner)
+    if a.kernels_only:
+        kernels_only(w, a.kernels_only, dev)
+        return
+    steps = a.steps if a.steps is not None else w.nsteps - a.warmup
+    if a.warmup + steps > w.nsteps:
+        raise SystemExit(f"warmup + steps = {a.warmup + steps} exceeds the workload's {w.nsteps} steps")
+    fx = fixture(w.name)
+    cps = set(int(k) for k in fx["check"]) if fx is not None else set()
+    ice_dev = upload(w, w.nsteps, dev)
+    head = Leg(w, ice_dev, dev, a.precision, a.inner, a.warmup, steps, cps)
     # ---- dominant kernel: the per-order analysis contraction of a 2-field sweep, timed with HIP
     # events on the current stream (the stream the library launches on)
+    sht = SHT(w.nlat, w.nlon, w.lmax, device=dev)
     grids = torch.from_numpy(np.stack([w.topo, w.ice(1)])).to(dev)
     coef = torch.zeros((2, sht.ncoef), dtype=torch.complex128, device=dev)
     sht.analysis_packed(grids, coef)
@@ -186,27 +315,56 @@ def main():
             "kernel": f"{kname} (per-order FP64 Legendre analysis contraction on the matrix cores, 2 fields; "
                       f"variant chosen by shape)",
             "kernel_us": t_kernel * 1e6, "flops_per_launch": flops}
+    del sht, grids, coef
     out = {
-        "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
-        "ms_per_step": 1e3 * el / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": describe(w, a.inner), "numiter_mean": float(np.mean(numiter)),
-                   "numiter_total": int(np.sum(numiter)), "ice": w.note},
+        "metric": METRIC, "value": head.sps, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * head.elapsed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f64+f32", "data": "synthetic",
+        "config": {"workload": describe(w, a.inner, a.precision),
+                   "numiter_mean": float(np.mean(head.timed_numiter)),
+                   "numiter_total": int(np.sum(head.timed_numiter)), "ice": w.note},
         "roofline": roof,
-        "mixed": "not provided: on this GPU FP32 vector math runs at the FP64 rate and the contraction is "
-                 "bound by its FP64 recurrence, so reduced-precision sweeps would not be cheaper",
     }
-    del sht
-    torch.cuda.synchronize()
-    if a.steps is None and a.workload == "lmax512":
-        # the metric's second grid (lmax 1023, 1024 x 2048, 240 x 0.5 kyr), FP64, every step but 5
-        w3 = syn.workload("lmax1023")
-        sps3, el3, steps3, nit3, sht3 = run_workload(w3, 5, None, dev, a.inner)
-        del sht3
-        out["also"] = [{"workload": describe(w3, a.inner), "value": sps3, "unit": "steps/s", "steps": steps3, "warmup": 5,
-                        "ms_per_step": 1e3 * el3 / steps3, "numiter_mean": float(np.mean(nit3)), "ice": w3.note}]
+    acc = {}
+    if a.precision == "fp64":
+        acc[f"{w.name}_fp64"] = vs_oracle(head, fx)
+    if not a.no_also and a.inner == "extrap" and a.workload in ("lmax512", "lmax1023"):
+        also, mixed = [], {}
+        legs = {}
+        full = w.nsteps - a.warmup
+        if a.precision == "fp64" and steps == full:
+            legs[(w.name, "fp64")] = head
+        elif a.precision == "fp64":
+            legs[(w.name, "fp64")] = Leg(w, ice_dev, dev, "fp64", a.inner, a.warmup, full, cps)
+            also.append(legs[(w.name, "fp64")].summary(w, "fp64", a.inner))
+            acc[f"{w.name}_fp64_full"] = vs_oracle(legs[(w.name, "fp64")], fx)
+        else:
+            legs[(w.name, "fp64")] = Leg(w, ice_dev, dev, "fp64", a.inner, a.warmup, full, cps)
+        legs[(w.name, "mixed")] = Leg(w, ice_dev, dev, "mixed", a.inner, a.warmup, full, cps)
+        del ice_dev
+        torch.cuda.empty_cache()
+        other = "lmax1023" if w.name == "lmax512" else "lmax512"
+        w2 = syn.workload(other)
+        fx2 = fixture(other)
+        cps2 = set(int(k) for k in fx2["check"]) if fx2 is not None else set()
+        ice2 = upload(w2, w2.nsteps, dev)
+        warm2 = 5 if other == "lmax1023" else 10
+        legs[(other, "fp64")] = Leg(w2, ice2, dev, "fp64", a.inner, warm2, w2.nsteps - warm2, cps2)
+        legs[(other, "mixed")] = Leg(w2, ice2, dev, "mixed", a.inner, warm2, w2.nsteps - warm2, cps2)
+        del ice2
+        also.append(legs[(other, "fp64")].summary(w2, "fp64", a.inner))
+        acc[f"{other}_fp64"] = vs_oracle(legs[(other, "fp64")], fx2)
+        for ww in (w, w2):
+            f64, mx = legs[(ww.name, "fp64")], legs[(ww.name, "mixed")]
+            s = mx.summary(ww, "mixed", a.inner)
+            s["ratio_to_fp64"] = mx.sps / f64.sps
+            s["accuracy_vs_fp64"] = vs_fp64(mx, f64)
+            mixed[ww.name] = s
+        out["also"] = also
+        out["mixed"] = mixed
+    out["accuracy"] = acc
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, w.ice, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     print(json.dumps(out))
 
 

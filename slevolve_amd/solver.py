@@ -19,7 +19,7 @@ the reference are not provided and raise.
 ``precision="mixed"`` (no counterpart in the reference) runs the early inner sweeps of every step
 with the FP32-product Legendre contractions; a sweep takes the FP64 contractions once the previous
 one changed no ocean-function cell and no grounded-ice flag, or once the convergence ratio is at or
-below ``eps_switch`` (default 100 epsilon1), and so do all later sweeps of the step; the step
+below ``eps_switch`` (default 1000 epsilon1), and so do all later sweeps of the step; the step
 converges on an FP64 sweep only.  Everything else stays FP64.
 """
 from __future__ import annotations
@@ -165,7 +165,7 @@ class SeaLevelSolver:
         self.tpw, self.checkmarine, self.calcRG = bool(tpw), bool(checkmarine), bool(calcRG)
         self.epsilon1 = float(epsilon1)
         self.precision = precision
-        self.eps_switch = float(100.0 * self.epsilon1 if eps_switch is None else eps_switch)
+        self.eps_switch = float(1000.0 * self.epsilon1 if eps_switch is None else eps_switch)
         if precision == "mixed" and not self.eps_switch > self.epsilon1:
             raise ValueError("eps_switch must exceed epsilon1")
         self.inner = inner

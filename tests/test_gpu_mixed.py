@@ -1,6 +1,6 @@
 """Mixed precision: the FP32-product Legendre contractions (FP64 recurrence, FP32 matrix-core
 products) against the oracle, and the mixed-precision solver (FP32-product sweeps until the masks
-stop changing or the convergence ratio reaches eps_switch = 100 epsilon1, then FP64 sweeps to
+stop changing or the convergence ratio reaches eps_switch = 1000 epsilon1, then FP64 sweeps to
 convergence) against the FP64 solver, field by field and iteration count by iteration count, and
 against the oracle's lmax-1023 fixture.
 
