@@ -81,21 +81,19 @@ def save(path, kind, a):
 
 
 def f4_exact(a):
-    """True if every value survives a round trip through float32."""
-    a = np.asarray(a, dtype=np.float64)
+    """True when float32 holds every value of ``a`` exactly, so that an '<f4' file loses nothing
+    (values outside the float32 range, and NaN, do not qualify)."""
+    x = np.asarray(a, dtype=np.float64)
     with np.errstate(over="ignore", invalid="ignore"):
-        back = a.astype(np.float32).astype(np.float64)
-    return bool(np.array_equal(back, a))
+        return bool((x.astype(np.float32) == x).all())
 
 
 def save_grid(path, a, dtype="f8"):
-    """Grid as '<f8' ('<f4' if asked and lossless); returns the dtype written."""
+    """Write a grid as '<f8', or as '<f4' when asked for and exact; returns the dtype written."""
     g = np.ascontiguousarray(_host(a), dtype=np.float64)
-    if dtype == "f4" and f4_exact(g):
-        np.save(path, g.astype("<f4"))
-        return "f4"
-    np.save(path, g.astype("<f8", copy=False))
-    return "f8"
+    kind = "f4" if dtype == "f4" and f4_exact(g) else "f8"
+    np.save(path, g.astype("<" + kind, copy=False))
+    return kind
 
 
 def save_mask(path, m, io_compat="fortran"):
@@ -138,47 +136,64 @@ def save_rates(out, tag, rates, fields=("dSLdt", "dRdt")):
 
 
 # ------------------------------------------------------------------------------ readers
+class GridFileError(ValueError):
+    """A grid or times file whose content does not fit what it is read for."""
+
+
+def _check_count(path, count, nlat, nlon, unit):
+    if count != nlat * nlon:
+        raise GridFileError(f"{path} holds {count} {unit}; a {nlat} x {nlon} grid needs {nlat * nlon}")
+
+
 def read_text_grid(path, nlat=NLAT, nlon=NLON):
-    """Whitespace-separated text grid (any values per line) -> (nlat, nlon), stream order."""
-    a = np.fromfile(path, sep=" ")
-    if a.size != nlat * nlon:
-        raise ValueError(f"{path}: got {a.size} values, expected {nlat * nlon}")
-    return a.reshape(nlat, nlon)
+    """A text grid: whitespace-separated values in stream order, any number per line."""
+    with open(path) as fh:
+        vals = np.array(fh.read().split(), dtype=np.float64)
+    _check_count(path, vals.size, nlat, nlon, "values")
+    return vals.reshape(nlat, nlon)
 
 
 def load_npy(path, mmap=False):
     return np.load(path, mmap_mode="r" if mmap else None, allow_pickle=False)
 
 
-def read_grid(path, nlat=NLAT, nlon=NLON):
-    """A grid from '.npy' (any real dtype, either order; a flat array of nlat*nlon values is taken
-    in stream order), a headerless little-endian f8 '.bin', or text.  A path without either
-    extension is looked up as <path>.npy, then <path>.bin, then the text file itself."""
-    if path.endswith(".npy"):
-        a = np.load(path, allow_pickle=False)
-        if a.shape != (nlat, nlon):
-            if a.size != nlat * nlon:
-                raise ValueError(f"{path}: shape {a.shape}, expected {(nlat, nlon)}")
-            a = a.reshape(nlat, nlon)
-        return np.ascontiguousarray(a, dtype=np.float64)
-    if path.endswith(".bin"):
-        nbytes = os.path.getsize(path)
-        if nbytes != 8 * nlat * nlon:
-            raise ValueError(f"{path}: {nbytes} bytes, expected {8 * nlat * nlon} (LE f8)")
-        return np.fromfile(path, dtype="<f8").reshape(nlat, nlon)
-    src = grid_source(os.path.dirname(path) or ".", os.path.basename(path))
-    if src is not None and src != path:
-        return read_grid(src, nlat, nlon)
-    return read_text_grid(path, nlat, nlon)
+def _read_npy_grid(path, nlat, nlon):
+    arr = np.load(path, allow_pickle=False)
+    _check_count(path, arr.size, nlat, nlon, "values")
+    if arr.shape != (nlat, nlon):  # a flat or otherwise shaped payload, taken in stream order
+        arr = arr.reshape(nlat, nlon)
+    return np.ascontiguousarray(arr, dtype=np.float64)
+
+
+def _read_bin_grid(path, nlat, nlon):
+    nbytes = os.path.getsize(path)
+    if nbytes % 8:
+        raise GridFileError(f"{path}: {nbytes} bytes is not a whole number of little-endian float64 values")
+    _check_count(path, nbytes // 8, nlat, nlon, "float64 values")
+    return np.fromfile(path, dtype="<f8").reshape(nlat, nlon)
+
+
+# the encodings a grid name may have on disk, in the order a reader prefers them
+_GRID_ENCODINGS = ((".npy", _read_npy_grid), (".bin", _read_bin_grid), ("", read_text_grid))
 
 
 def grid_source(dirname, name):
-    """The file a grid <dirname>/<name> is read from (.npy, then .bin, then text), or None."""
-    for ext in (".npy", ".bin", ""):
-        p = os.path.join(dirname, name + ext)
-        if os.path.isfile(p):
-            return p
-    return None
+    """Path of the file grid ``name`` in ``dirname`` is read from: <name>.npy, else <name>.bin, else
+    the text file <name>; None when there is none of them."""
+    base = os.path.join(dirname, name)
+    return next((base + suf for suf, _ in _GRID_ENCODINGS if os.path.isfile(base + suf)), None)
+
+
+def read_grid(path, nlat=NLAT, nlon=NLON):
+    """A float64 (nlat, nlon) grid from '.npy' (any real dtype and memory order), a headerless
+    little-endian '.bin' of float64 values, or text.  A path with neither suffix names a grid whose
+    preferred encoding is read (see grid_source)."""
+    for suf, reader in _GRID_ENCODINGS[:2]:
+        if path.endswith(suf):
+            return reader(path, nlat, nlon)
+    src = grid_source(os.path.dirname(path) or ".", os.path.basename(path)) or path
+    reader = next(r for suf, r in _GRID_ENCODINGS if src.endswith(suf))
+    return reader(src, nlat, nlon)
 
 
 def find_grid(dirname, name, nlat=NLAT, nlon=NLON):
@@ -186,111 +201,144 @@ def find_grid(dirname, name, nlat=NLAT, nlon=NLON):
 
 
 class IceDir:
-    """ice[k] read on demand from <dir>/<pfx><k> (.npy, .bin or text)."""
+    """The ice history of a directory as a lazily read sequence: ``ice[k]`` is grid <pfx><k>."""
 
     def __init__(self, d, pfx, nlat=NLAT, nlon=NLON):
-        self.d, self.pfx, self.nlat, self.nlon = d, pfx, nlat, nlon
+        self.directory, self.prefix, self.shape = d, pfx, (nlat, nlon)
 
     def __getitem__(self, k):
-        return find_grid(self.d, f"{self.pfx}{k}", self.nlat, self.nlon)
+        return find_grid(self.directory, f"{self.prefix}{int(k)}", *self.shape)
 
 
 def read_times(path, nsteps=None):
-    """Times t_0..t_N (years) of the ice files, strictly increasing; with nsteps, the first
-    nsteps + 1 (an error if fewer).  Step j lasts t_j - t_{j-1}."""
-    t = np.loadtxt(path, dtype=np.float64, ndmin=1)
-    if t.ndim != 1 or t.size < 2:
-        raise ValueError(f"{path}: need one time per line, at least 2 values")
+    """The times t_0, t_1, ... (years) of the ice files, one per line (the first number of a line;
+    blank lines are skipped).  They must be finite and strictly increasing, at least two of them;
+    with ``nsteps`` only t_0..t_nsteps are returned and fewer is an error.  Step j lasts
+    t_j - t_{j-1}."""
+    with open(path) as fh:
+        t = np.array([float(line.split()[0]) for line in fh if line.strip()], dtype=np.float64)
+    if t.size < 2:
+        raise GridFileError(f"{path}: a times file lists at least two times, found {t.size}")
     if nsteps is not None:
-        if t.size < nsteps + 1:
-            raise ValueError(f"{path}: {t.size} times, need {nsteps + 1} (files 0..{nsteps})")
+        if t.size <= nsteps:
+            raise GridFileError(f"{path}: {nsteps} steps need {nsteps + 1} times, the file lists {t.size}")
         t = t[: nsteps + 1]
-    if not (np.isfinite(t).all() and (np.diff(t) > 0).all()):
-        raise ValueError(f"{path}: times must be finite and strictly increasing")
+    if not np.isfinite(t).all():
+        raise GridFileError(f"{path}: non-finite time")
+    if (np.diff(t) <= 0).any():
+        raise GridFileError(f"{path}: times do not increase strictly")
     return t
 
 
 # --------------------------------------------------------------------------- ice cache
+# A cache stacks ice files 0..n-1 as one (n, nlat, nlon) '<f8' array; its JSON sidecar <cache>.json
+# records where the stack came from: {"icedir": absolute directory, "prefix": file prefix,
+# "nfiles": n, "files": [[file name, size in bytes, mtime in ns], ...] of the file a reader takes
+# for each k} (the reference's schema, so that caches are shared with it).
+def _file_identity(path):
+    st = os.stat(path)
+    return [os.path.basename(path), int(st.st_size), int(st.st_mtime_ns)]
+
+
 def ice_manifest(dirname, pfx, nfiles):
-    """Identity of an ice cache's sources: directory, prefix, count and, per file k < nfiles, the
-    [name, size, mtime_ns] of the file a reader would use."""
-    files = []
+    """The sidecar record of a cache built now from ice files 0..nfiles-1 of ``dirname``."""
+    ident = []
     for k in range(nfiles):
         src = grid_source(dirname, f"{pfx}{k}")
         if src is None:
-            raise FileNotFoundError(os.path.join(dirname, f"{pfx}{k}"))
-        st = os.stat(src)
-        files.append([os.path.basename(src), int(st.st_size), int(st.st_mtime_ns)])
-    return {"icedir": os.path.abspath(dirname), "prefix": pfx, "nfiles": int(nfiles), "files": files}
+            raise FileNotFoundError(f"ice file {pfx}{k} (.npy, .bin or text) not found in {dirname}")
+        ident.append(_file_identity(src))
+    return {"icedir": os.path.abspath(dirname), "prefix": pfx, "nfiles": int(nfiles), "files": ident}
 
 
 def manifest_path(cache_path):
-    return cache_path + ".json"
+    return f"{cache_path}.json"
 
 
 def write_manifest(cache_path, man):
     with open(manifest_path(cache_path), "w") as fh:
-        json.dump(man, fh, indent=0)
+        fh.write(json.dumps(man, indent=1))
+
+
+def _stale(cache_path, man):
+    """Why the cache at ``cache_path`` cannot stand for the sources ``man`` describes, or None.  A
+    cache of more files serves when its leading files are the same."""
+    if not os.path.isfile(cache_path):
+        return "cache file missing"
+    try:
+        with open(manifest_path(cache_path)) as fh:
+            rec = json.load(fh)
+    except FileNotFoundError:
+        return "sidecar missing"
+    except (OSError, ValueError):
+        return "sidecar not readable as JSON"
+    if rec.get("icedir") != man["icedir"]:
+        return "built from another ice directory"
+    if rec.get("prefix") != man["prefix"]:
+        return "built for another file prefix"
+    if int(rec.get("nfiles", -1)) < man["nfiles"]:
+        return "holds fewer ice files than needed"
+    have = [list(f) for f in rec.get("files", ())][: man["nfiles"]]
+    if have != man["files"]:
+        return "an ice file's size or modification time differs"
+    return None
 
 
 def cache_matches(cache_path, man):
-    """(ok, reason): whether the cache was built from the files of ``man``.  A cache of more files
-    matches when its leading entries do."""
-    if not os.path.exists(cache_path):
-        return False, "no cache"
-    mp = manifest_path(cache_path)
-    if not os.path.exists(mp):
-        return False, "no manifest"
-    try:
-        with open(mp) as fh:
-            old = json.load(fh)
-    except (OSError, ValueError):
-        return False, "unreadable manifest"
-    checks = (
-        (old.get("icedir") == man["icedir"], "ice directory changed"),
-        (old.get("prefix") == man["prefix"], "prefix changed"),
-        (int(old.get("nfiles", -1)) >= man["nfiles"], "file count changed"),
-        ([list(e) for e in old.get("files", [])[: man["nfiles"]]] == man["files"], "ice file size or mtime changed"),
-    )
-    for ok, why in checks:
-        if not ok:
-            return False, why
-    return True, ""
+    """(True, "") when the cache can stand for the sources ``man`` describes, else (False, why)."""
+    why = _stale(cache_path, man)
+    return (why is None, why or "")
+
+
+def _stacks(cache_dir, stem):
+    """[(n, path)] of the cached stacks <stem><n>.npy in cache_dir, largest n first."""
+    found = []
+    for f in os.listdir(cache_dir):
+        if f.startswith(stem) and f.endswith(".npy") and f[len(stem):-4].isdigit():
+            found.append((int(f[len(stem):-4]), os.path.join(cache_dir, f)))
+    return sorted(found, reverse=True)
 
 
 def build_ice_cache(icedir, pfx, nmax, nlat=NLAT, nlon=NLON, cache_dir="cache", rebuild=False, log=print):
-    """ice_<pfx>0_<n>.npy in cache_dir holding files 0..nmax: an existing cache of n >= nmax files
-    whose manifest matches is reused; otherwise one is written (to a temporary name, then renamed)
-    with its manifest.  Returns (path, seconds spent building; 0 when reused)."""
+    """The stack of ice files 0..nmax as cache_dir/ice_<pfx>0_<n>.npy (the reference's naming): a
+    cached stack of at least nmax + 1 files whose sidecar matches the current files is reused;
+    otherwise the stack is written to a temporary file that replaces ice_<pfx>0_<nmax>.npy once
+    complete, with its sidecar.  Returns (path, seconds spent writing; 0.0 when reused)."""
+    import tempfile
     import time
 
     os.makedirs(cache_dir, exist_ok=True)
     stem = f"ice_{pfx}0_"
-    path = os.path.join(cache_dir, f"{stem}{nmax}.npy")
     man = ice_manifest(icedir, pfx, nmax + 1)
     if not rebuild:
-        reason, found = "no cache", False
-        for f in sorted(os.listdir(cache_dir)):
-            m = re.fullmatch(re.escape(stem) + r"(\d+)\.npy", f)
-            if not m or int(m.group(1)) < nmax:
-                continue
-            found = True
-            ok, why = cache_matches(os.path.join(cache_dir, f), man)
-            if ok:
-                return os.path.join(cache_dir, f), 0.0
-            reason = why
-        if found:
-            log(f"ice cache out of date ({reason}): rebuilding {path}")
+        reasons = []
+        for n, cand in _stacks(cache_dir, stem):
+            if n < nmax:
+                break
+            why = _stale(cand, man)
+            if why is None:
+                return cand, 0.0
+            reasons.append(f"{os.path.basename(cand)}: {why}")
+        if reasons:
+            log(f"[slevolve_amd] cached ice stacks not usable ({'; '.join(reasons)}); writing a new one")
     t0 = time.perf_counter()
-    tmp = path + ".tmp"
-    stack = np.lib.format.open_memmap(tmp, mode="w+", dtype=np.float64, shape=(nmax + 1, nlat, nlon))
-    for k in range(nmax + 1):
-        stack[k] = find_grid(icedir, f"{pfx}{k}", nlat, nlon)
-    stack.flush()
-    del stack
-    os.replace(tmp, path)
-    write_manifest(path, man)
-    return path, time.perf_counter() - t0
+    target = os.path.join(cache_dir, f"{stem}{nmax}.npy")
+    fd, part = tempfile.mkstemp(suffix=".part", dir=cache_dir)
+    os.close(fd)
+    try:
+        out = np.lib.format.open_memmap(part, mode="w+", dtype="<f8", shape=(nmax + 1, nlat, nlon))
+        src = IceDir(icedir, pfx, nlat, nlon)
+        for k in range(nmax + 1):
+            out[k] = src[k]
+        out.flush()
+        del out
+        os.replace(part, target)
+    except BaseException:
+        if os.path.exists(part):
+            os.remove(part)
+        raise
+    write_manifest(target, man)
+    return target, time.perf_counter() - t0
 
 
 # ------------------------------------------------------------------------- converters

@@ -135,19 +135,19 @@ def test_ice_cache_manifest(tmp_path):
     man = npyio.ice_manifest(str(d), "ice", 4)
     assert [f[0] for f in man["files"]] == ["ice0", "ice1.npy", "ice2", "ice3.npy"]
     assert npyio.cache_matches(path, man) == (True, "")
-    assert npyio.cache_matches(path + "x", man) == (False, "no cache")
-    assert npyio.cache_matches(path, dict(man, icedir="/elsewhere"))[1] == "ice directory changed"
-    assert npyio.cache_matches(path, dict(man, prefix="other"))[1] == "prefix changed"
+    assert npyio.cache_matches(path + "x", man) == (False, "cache file missing")
+    assert npyio.cache_matches(path, dict(man, icedir="/elsewhere"))[1] == "built from another ice directory"
+    assert npyio.cache_matches(path, dict(man, prefix="other"))[1] == "built for another file prefix"
     assert npyio.cache_matches(path, npyio.ice_manifest(str(d), "ice", 3))[0]
     np.save(d / "ice4.npy", np.zeros((nlat, nlon)))
-    assert npyio.cache_matches(path, npyio.ice_manifest(str(d), "ice", 5))[1] == "file count changed"
+    assert npyio.cache_matches(path, npyio.ice_manifest(str(d), "ice", 5))[1] == "holds fewer ice files than needed"
     st = os.stat(d / "ice1.npy")
     os.utime(d / "ice1.npy", ns=(st.st_atime_ns, st.st_mtime_ns + 10**9))
-    assert npyio.cache_matches(path, npyio.ice_manifest(str(d), "ice", 4))[1] == "ice file size or mtime changed"
+    assert npyio.cache_matches(path, npyio.ice_manifest(str(d), "ice", 4))[1] == "an ice file's size or modification time differs"
     path2, secs2 = npyio.build_ice_cache(str(d), "ice", 3, nlat, nlon, cache, log=lambda *a: None)
     assert path2 == path and secs2 > 0.0
     os.remove(npyio.manifest_path(path))
-    assert npyio.cache_matches(path, man)[1] == "no manifest"
+    assert npyio.cache_matches(path, man)[1] == "sidecar missing"
 
 
 def test_tpw_rate_record(tmp_path):
