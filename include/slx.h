@@ -86,6 +86,21 @@ int slx_modal_advance_read(const slx_sht* h, int kmax, const int* nmod, const do
                            const double* omd, const double* a_sl, const double* a_rr, void* R,
                            const void* P, void* visc, void* rad, void* stream);
 
+/* ------------------------------------------------------------------- adjoint ---------------
+ * Building blocks of the hand-written adjoint of the time step (slevolve_amd/adjoint.py; the
+ * reference has no adjoint).  Transposes with respect to the real inner products of the stored
+ * arrays: <analysis(g), U> = <g, analysis_adjoint(U)>, <synthesis(U), g> = <U, synthesis_adjoint(g)>.
+ * analysis_adjoint: coef (device) nf x [ncoef] packed -> grid (device) nf x [nlat][nlon].
+ * synthesis_adjoint: grid -> coef.  Both FP64, any handle precision. */
+int slx_sht_analysis_adjoint(slx_sht* h, const void* coef, int nf, double* grid, void* stream);
+int slx_sht_synthesis_adjoint(slx_sht* h, const double* grid, int nf, void* coef, void* stream);
+/* Reverse of slx_modal_advance_read: Rbar_k <- decay (Rbar_k + a_sl visc_bar + a_rr rad_bar) and
+ * Pbar += sum_k omd (Rbar_k + a_sl visc_bar + a_rr rad_bar), per packed (l, m) and mode k < nmod[l]
+ * (device arrays as for the advance). */
+int slx_modal_reverse(const slx_sht* h, int kmax, const int* nmod, const double* decay, const double* omd,
+                      const double* a_sl, const double* a_rr, void* Rbar, const void* visc_bar,
+                      const void* rad_bar, void* Pbar, void* stream);
+
 /* ------------------------------------------------------------------- the step ------------
  * Replaces SeaLevelSolver (solver.py:521-1144) and OptSeaLevelSolver with inner = "off",
  * "extrap" or "anderson" (solver_opt.py:369-979), step or second-order ("ramp") load

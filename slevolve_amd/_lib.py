@@ -68,6 +68,10 @@ SIGNATURES = {
     "slx_sht_dense_to_tri": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "slx_modal_advance_read": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "slx_sht_analysis_adjoint": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "slx_sht_synthesis_adjoint": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "slx_modal_reverse": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p]),
     "slx_solver_create": (c_int, [c_void_p, C.POINTER(SlxSolverParams), C.POINTER(SlxOutputs), C.POINTER(c_void_p)]),
     "slx_solver_destroy": (c_int, [c_void_p]),
     "slx_solver_init": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -12,7 +12,8 @@ deterministic, so the CPU oracle and the GPU solver see identical arrays.
   (``python/slevolve/tools/write_lovenum.py:9-33``).
 * ``workload(name)``: the benchmark configurations "lmax63" (64 x 128, 100 x 1 yr, 6 modes),
   "lmax512" (512 x 1024, 550 x 1 yr, 30 modes, polar cap + retreating marine sector) and
-  "lmax1023" (1024 x 2048, 240 x 0.5 kyr, 30 modes, three caps over a glacial cycle).
+  "lmax1023" (1024 x 2048, 240 x 0.5 kyr, 30 modes, three caps over a glacial cycle) and
+  "lmax256" (256 x 512, 300 x 1 yr, 30 modes, the lmax512 ice geometry; the adjoint case).
 """
 from __future__ import annotations
 
@@ -196,6 +197,8 @@ def workload(name):
         cfg, nlat, lmax, nsteps, dt, K, lt = 2, 512, 512, 550, 1.0, 30, 256
     elif name == "lmax1023":
         cfg, nlat, lmax, nsteps, dt, K, lt = 3, 1024, 1023, 240, 500.0, 30, 256
+    elif name == "lmax256":  # the adjoint configuration: 256 x 512, lmax 256, 300 x 1 yr
+        cfg, nlat, lmax, nsteps, dt, K, lt = 5, 256, 256, 300, 1.0, 30, 128
     else:
         raise ValueError(f"unknown workload {name!r}")
     seqs = np.random.SeedSequence(20261203 + cfg).spawn(5)
@@ -212,7 +215,7 @@ def workload(name):
             f = 1.0 - k / nsteps
             return round_mm(sum(sh * H * f for sh, H in caps))
         note = "two caps (-80,0) R12 H3000 and (70,-60) R15 H2500, linear melt over 100 steps"
-    elif name == "lmax512":
+    elif name in ("lmax512", "lmax256"):
         # polar cap (R 20 deg, 3500 m) and a marine sector (lon 150-210, colat 20-32 deg) whose bed
         # is -500 +- 300 m; the sector's ice front retreats 2 km/yr towards the cap
         colat = 90.0 + LAT
@@ -228,6 +231,8 @@ def workload(name):
             return round_mm(np.maximum(cap, marine))
         note = ("polar cap R20 H3500 + marine sector lon 150-210 colat 20-32, bed -500+-300 m, "
                 "front retreating 2 km/yr")
+        if name == "lmax256":
+            note += "; adjoint misfit: 200 seeded ocean points and the J2 rate (seed stream 4)"
     else:
         caps = _caps(LAT, LON, [(-90.0, 0.0, 20.0, 3500.0), (60.0, -95.0, 12.0, 3000.0),
                                 (65.0, 20.0, 8.0, 2000.0)])
