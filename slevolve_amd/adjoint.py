@@ -165,6 +165,11 @@ class FrozenStepModel:
         self.cs00 = self.cs[:, 0].real.clone()
         self.m_hist = {}
 
+    def _converged(self, d, prev):
+        """Relative update d at or below the tolerance, or stalled at the round-off floor (within
+        100 x the tolerance and no longer shrinking)."""
+        return d <= self.tol or (d <= 100.0 * self.tol and d >= 0.5 * prev)
+
     # ------------------------------------------------------------------------- pieces
     def _rotation(self, DS2, DI2, sums):
         """DS2, DI2: [B, 3] complex (host) degree-2 coefficients m = 0, 1, 2; sums: host tensors.
@@ -231,6 +236,7 @@ class FrozenStepModel:
         base = self.w1 * (self.ri * DI) + self.w2 * visc
         cs, cs00, cstar, tO = self.cs[n], self.cs00[n], self.cstar[n], self.tO[n]
         x = st["x"].clone()
+        prev = math.inf
         for it in range(self.max_iter):
             DS = st["pO"] + x
             il, m, lam, rot_sl, rot_rr = self._rotation(self._deg2(DS), DI2, sums)
@@ -239,12 +245,13 @@ class FrozenStepModel:
             rt = ops.A(dsl * cstar - tO)
             cons = (-(self.ri / self.rw) * DI[:, 0].real - rt[:, 0].real) / cs00
             xn = -st["pO"] + rt + cs[None] * cons[:, None]
-            dx = (xn - x).abs().amax(dim=1) / xn.abs().amax(dim=1).clamp_min(1e-300)
+            dx = float(((xn - x).abs().amax(dim=1) / xn.abs().amax(dim=1).clamp_min(1e-300)).max())
             x = xn
-            if float(dx.max()) <= self.tol:
+            if self._converged(dx, prev):
                 break
+            prev = dx
         else:
-            raise RuntimeError(f"frozen-mask step {n}: fixed point not converged ({float(dx.max()):.2e})")
+            raise RuntimeError(f"frozen-mask step {n}: fixed point not converged ({dx:.2e})")
         DSx = st["pO"] + x  # committed state (at the fixed point)
         CSr = self._add_deg2(self.wh * (self.ri * DI + self.rw * (st["pO"] + x)) + self.w2 * rad, rot_rr)
         out = {"G20": (CS[:, self.i20] + CSr[:, self.i20]).real.clone(), "m": m}
@@ -398,13 +405,15 @@ class FrozenStepModel:
             o["rr"][:, 0] += g20b
         # adjoint fixed point mu = Phi_x^T mu + h_x^T o + xdir
         mu = xdir.clone()
+        prev = math.inf
         for it in range(self.max_iter):
             xin, _ = self._back_phi(n, mu, o)
             mun = xin + xdir
-            d = (mun - mu).abs().amax() / mun.abs().amax().clamp_min(1e-300)
+            d = float((mun - mu).abs().amax() / mun.abs().amax().clamp_min(1e-300))
             mu = mun
-            if float(d) <= self.tol:
+            if self._converged(d, prev):
                 break
+            prev = d
         else:
             raise RuntimeError(f"adjoint fixed point of step {n} not converged")
         _, th = self._back_phi(n, mu, o)

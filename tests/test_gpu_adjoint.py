@@ -92,18 +92,27 @@ def test_frozen_model_reproduces_the_solver():
 
 
 def test_gradient_vs_central_differences_lmax256():
-    """The adjoint configuration: 300 steps, 50 seeded directions (batched forward runs)."""
+    """The adjoint configuration: 300 steps, 50 seeded directions (batched forward runs).  The
+    hand-written FP64 gradient against central differences of the model with 1 m perturbations
+    (the model is affine in the ice up to the quadratic centrifugal terms, so the agreement is set
+    by those and by the fixed-point tolerance: ~1e-8 measured); the FP32 gradient of the
+    torch.autograd.Function (FP32 ice history in, FP32 gradient out) is the FP64 gradient of that
+    input, rounded."""
     from slevolve_amd.adjoint import IceHistoryMisfit, misfit_points
 
     N, ndir = 300, 50
     w, ice, model, _ = _nominal("lmax256", N)
     pts = misfit_points(model, 200)
     assert pts.numel() == 200
+    J64, g64 = model.gradient(ice, pts)
     ice32 = ice.float().requires_grad_(True)
     J = IceHistoryMisfit.apply(ice32, model, pts)
     J.backward()
-    grad = ice32.grad.double()
-    print(f"[adjoint] J = {float(J):.6e}, |grad| max {float(grad.abs().max()):.3e}", flush=True)
+    # the Function's FP32 gradient is the FP64 gradient of its (FP32-rounded) input, rounded
+    J32in, g32in = model.gradient(ice32.detach().double(), pts)
+    assert ice32.grad.dtype == torch.float32 and torch.equal(ice32.grad, g32in.float())
+    assert float(J) == float(torch.tensor(float(J32in[0]), dtype=torch.float32))
+    print(f"[adjoint] J = {float(J64[0]):.6e}, |grad| max {float(g64.abs().max()):.3e}", flush=True)
     seed = int(np.random.SeedSequence(20261203 + 5).spawn(5)[4].generate_state(1)[0])
     gen = torch.Generator(device="cuda").manual_seed(seed)
     eps = 1.0  # metres of ice
@@ -113,11 +122,10 @@ def test_gradient_vs_central_differences_lmax256():
     dslN, G20, _ = model.forward(batch, checkpoint=0)
     Jb = model.misfit(dslN, G20, pts)
     fd = (Jb[:ndir] - Jb[ndir:]) / (2 * eps)
-    ad = torch.tensor([float((grad * dirs[i]).sum()) for i in range(ndir)], dtype=torch.float64)
+    ad = torch.tensor([float((g64 * dirs[i]).sum()) for i in range(ndir)], dtype=torch.float64)
     rel = ((fd - ad).abs() / fd.abs().clamp_min(1e-300)).max()
     print(f"[adjoint] 50 directions: max |FD - adjoint| / |FD| = {float(rel):.2e}", flush=True)
-    # FP32 gradient storage (2^-24) and the fixed-point tolerance (1e-13) bound the agreement
-    assert float(rel) < 1e-5
+    assert float(rel) < 1e-7
 
 
 class _CpuOps:
