@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-also", action="store_true", help="only the headline leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble leg")
+    ap.add_argument("--ensemble-runs", type=int, default=64)
     ap.add_argument("--inner", default="extrap", choices=("off", "extrap", "anderson"),
                     help="inner start / acceleration (the metric is quoted with extrap)")
     ap.add_argument("--kernels-only", default=None,
@@ -213,6 +215,52 @@ def vs_fp64(mixed, fp64):
     return res
 
 
+def ensemble_leg(nruns, dev):
+    """The batched configuration: nruns independent runs (lmax 128/256/512, 50-1200 steps, dt 0.2 /
+    0.5 / 1.0 kyr, perturbed Love numbers, random caps), every member alone on one stream, then the
+    whole ensemble on the autotuned number of concurrent streams with per-lmax tuned kernel
+    variants; ensemble steps/s, the members' solo steps/s and the achieved concurrency (sum of the
+    members' solo times over the ensemble's wall time)."""
+    import numpy as np
+    import torch
+
+    from slevolve_amd import ensemble as E
+
+    t0 = time.perf_counter()
+    specs = E.make_ensemble(nruns)
+    ices = E.prepare(specs, dev)
+    var = E.tune_variants({s.lmax for s in specs}, dev)
+    prep = time.perf_counter() - t0
+    print(f"[bench] ensemble: {nruns} members prepared in {prep:.1f} s, variants {var}", file=sys.stderr, flush=True)
+    slot = E._Stream(dev, var)
+    solo = {}
+    for s in specs:
+        solo[s.id] = E.run_one(s, ices[s.id], slot)
+    t_solo = sum(v[0] for v in solo.values())
+    best, rates = E.autotune_streams(specs, ices, dev=dev, variants=var)
+    wall, out, _ = E.run_ensemble(specs, ices, nstreams=best, dev=dev, variants=var)
+    nst = sum(s.nsteps for s in specs)
+    same = all(torch.equal(solo[s.id][2], out[s.id][2]) for s in specs)
+    by_l = {}
+    for s in specs:
+        d = by_l.setdefault(str(s.lmax), {"members": 0, "steps": 0, "solo_s": 0.0})
+        d["members"] += 1
+        d["steps"] += s.nsteps
+        d["solo_s"] += solo[s.id][0]
+    for d in by_l.values():
+        d["solo_steps_per_s"] = d["steps"] / d["solo_s"]
+    del ices
+    torch.cuda.empty_cache()
+    print(f"[bench] ensemble: {nst} steps, alone {t_solo:.2f} s, ensemble {wall:.2f} s on {best} streams",
+          file=sys.stderr, flush=True)
+    return {"members": nruns, "steps_total": nst, "value": nst / wall, "unit": "steps/s",
+            "solo_steps_per_s": nst / t_solo, "speedup_vs_alone": t_solo / wall, "streams": best,
+            "stream_probe_steps_per_s": {str(k): v for k, v in rates.items()},
+            "kernel_variants": {str(k): v for k, v in var.items()}, "achieved_concurrency": t_solo / wall,
+            "members_equal_solo_runs": bool(same), "by_lmax": by_l, "prepare_s": prep,
+            "numiter_mean": float(np.mean([r["numiter"] for v in out.values() for r in v[1]]))}
+
+
 def describe(w, inner="extrap", precision="fp64"):
     prec = "FP64" if precision == "fp64" else "mixed (FP32-product contractions in early sweeps, FP64 to convergence)"
     return (f"{w.name}: {w.nlat}x{w.nlon} GL grid, lmax {w.lmax}, {w.nmodes} modes/degree, "
@@ -363,6 +411,8 @@ def main():
         out["also"] = also
         out["mixed"] = mixed
     out["accuracy"] = acc
+    if not a.no_ensemble and not a.no_also:
+        out["ensemble"] = ensemble_leg(a.ensemble_runs, dev)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     print(json.dumps(out))

@@ -1,6 +1,7 @@
 // Host-side launch helpers for the transform kernels (shared by the SHT and solver units).
 #pragma once
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "sht_kernels.h"
@@ -93,6 +94,8 @@ int launch_leg_ana_ck(const slx_sht* h, const double2* WS, const double2* WD, Ou
   const int nw = (h->nh + 16 * R - 1) / (16 * R);
   const size_t lds = (size_t)nw * 8 * 64 * sizeof(double);
   static int cached_nw = -1, cached_grid = 0;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
   if (nw != cached_nw) {
     int dev = 0, ncu = 0, per_cu = 0;
     SLX_HIP(hipGetDevice(&dev));
@@ -149,10 +152,16 @@ int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPt
 }  // namespace slx
 
 namespace slx {
-// Dynamic LDS above the default 64 KB is opted into once per kernel.
+// Dynamic LDS above the default 64 KB is opted into per kernel, raised monotonically under a
+// lock (host threads driving different handles may launch the same kernel with different sizes).
+inline std::mutex& lds_mutex() {
+  static std::mutex mu;
+  return mu;
+}
 template <auto KERN>
 int allow_lds(size_t lds) {
   static size_t allowed = 65536;
+  std::lock_guard<std::mutex> lock(lds_mutex());
   if (lds > allowed) {
     SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(KERN), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
@@ -204,12 +213,7 @@ int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long o
   const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
   const size_t lds = std::max((size_t)np2 * (sizeof(double) + NF * sizeof(double2)),
                               (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
-  static size_t lds_allowed = 65536;
-  if (lds > lds_allowed) {
-    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leg_syn_mf<NF, R>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    lds_allowed = lds;
-  }
+  if (int rc = allow_lds<&k_leg_syn_mf<NF, R>>(lds)) return rc;
   k_leg_syn_mf<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();
   return 0;
