@@ -57,9 +57,10 @@ def parse():
     ap.add_argument("--inner", default="extrap", choices=("off", "extrap", "anderson"),
                     help="inner start / acceleration (the metric is quoted with extrap)")
     ap.add_argument("--kernels-only", default=None,
-                    choices=("ana2", "ana2x32", "syn1", "syn1x32", "modal"),
+                    choices=("ana2", "ana2x32", "syn1", "syn1x32", "modal", "step"),
                     help="only launch one kernel 20x (the 2-field analysis contraction, FP64 or FP32-product; the "
-                         "1-field synthesis contraction; the modal advance) and a 256 MiB device copy, for PMC runs")
+                         "1-field synthesis contraction; the modal advance) or run 4 solver steps through the "
+                         "native loop (step), then a 256 MiB device copy, for PMC runs")
     return ap.parse_args()
 
 
@@ -279,8 +280,9 @@ def upload(w, nsteps, dev):
     return ice_dev
 
 
-def kernels_only(w, which, dev):
-    """Launch one contraction 20x (and a 256 MiB copy as the counter scale), for rocprofv3 --pmc."""
+def kernels_only(w, which, dev, precision="fp64"):
+    """Launch one contraction 20x, or run 4 steps of the workload through the native step loop
+    (every kernel of the time step), and a 256 MiB copy as the counter scale, for rocprofv3 --pmc."""
     import numpy as np
     import torch
 
@@ -296,6 +298,15 @@ def kernels_only(w, which, dev):
         mem._p_load.copy_(torch.randn_like(mem._p_load))
         for _ in range(20):
             mem.advance_and_read_packed()
+    elif which == "step":
+        from slevolve_amd import LoveNumbers, OptSeaLevelSolver
+
+        ice_dev = upload(w, 4, dev)
+        slv = OptSeaLevelSolver(SHT(w.nlat, w.nlon, w.lmax, device=dev), LoveNumbers(w.love_text, w.lmax, text=True),
+                                w.topo, w.ice(0), norder=w.lmax, dt=w.dt, tpw=True, checkmarine=True, calcRG=True,
+                                epsilon1=1e-5, inner="extrap", precision=precision)
+        rec = slv.run(ice_dev, 1, 5)
+        print(f"[bench] {w.name}: 4 steps, numiter {[r['numiter'] for r in rec]}", file=sys.stderr, flush=True)
     else:
         sht = SHT(w.nlat, w.nlon, w.lmax, device=dev, precision="mixed" if which.endswith("x32") else "fp64")
         grids = torch.from_numpy(np.stack([w.topo, w.ice(1)])).to(dev)
@@ -329,7 +340,7 @@ def main():
     w = syn.workload(a.workload)
     dev = torch.device("cuda", 0)
     if a.kernels_only:
-        kernels_only(w, a.kernels_only, dev)
+        kernels_only(w, a.kernels_only, dev, a.precision)
         return
     steps = a.steps if a.steps is not None else w.nsteps - a.warmup
     if a.warmup + steps > w.nsteps:

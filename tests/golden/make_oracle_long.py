@@ -16,10 +16,13 @@ incremental grounded-ice analyses):
 
 Inputs: the seeded workloads of slevolve_amd.synthetic, epsilon1 1e-5, inner "extrap", rotation
 and marine check on.  A file is rewritten after every checkpoint, so an interrupted run leaves the
-checkpoints it reached ("complete" says whether all steps ran).
-Run:  python tests/golden/make_oracle_long.py [lmax512|lmax1023 ...]
+checkpoints it reached ("complete" says whether all steps ran).  With --state DIR the whole
+generator state (oracle solver included) is pickled into DIR every 10 steps and a restarted run
+continues from there.
+Run:  python tests/golden/make_oracle_long.py [--state DIR] [lmax512|lmax1023 ...]
 """
 import os
+import pickle
 import sys
 import time
 
@@ -33,7 +36,7 @@ PLAN = {"lmax512": (550, (25, 100, 550), 8), "lmax1023": (40, (10, 20, 40), 16)}
 KEYS = ("gmslc_ocn", "gmslc_ocnBeta", "ocean_area", "oceanBeta_area", "ice_volume")
 
 
-def run(name):
+def run(name, state_dir=None):
     from oracle.love import LoveNumbers
     from oracle.sht import OracleSHT
     from oracle.solver import SeaLevelSolver
@@ -41,13 +44,19 @@ def run(name):
 
     steps, check, sub = PLAN[name]
     w = syn.workload(name)
-    slv = SeaLevelSolver(OracleSHT(w.nlat, w.nlon, w.lmax), LoveNumbers(w.love_text, w.lmax, text=True), w.topo,
-                         w.ice(0), norder=w.lmax, dt=w.dt, epsilon1=1e-5, inner="extrap")
-    per = {k: [] for k in ("numiter", "mm") + KEYS}
-    at = {k: [] for k in ("tgrid_sub", "R_sub", "tgrid_sum", "R_sum", "tgrid_ss", "R_ss")}
-    done = []
+    spath = os.path.join(state_dir, f"oracle_{name}_state.pkl") if state_dir else None
+    if spath and os.path.exists(spath):
+        with open(spath, "rb") as f:  # written by this script
+            slv, per, at, done, k_done = pickle.load(f)
+        print(f"{name}: resuming after step {k_done}", flush=True)
+    else:
+        slv = SeaLevelSolver(OracleSHT(w.nlat, w.nlon, w.lmax), LoveNumbers(w.love_text, w.lmax, text=True), w.topo,
+                             w.ice(0), norder=w.lmax, dt=w.dt, epsilon1=1e-5, inner="extrap")
+        per = {k: [] for k in ("numiter", "mm") + KEYS}
+        at = {k: [] for k in ("tgrid_sub", "R_sub", "tgrid_sum", "R_sum", "tgrid_ss", "R_ss")}
+        done, k_done = [], 0
     t0 = time.time()
-    for k in range(1, steps + 1):
+    for k in range(k_done + 1, steps + 1):
         r = slv.step(w.ice(k))
         per["numiter"].append(r["numiter"])
         per["mm"].append(np.asarray(r["mm"]).copy())
@@ -64,8 +73,17 @@ def run(name):
             np.savez_compressed(os.path.join(HERE, f"oracle_{name}_long.npz"), sub=sub, check=np.asarray(done),
                                 complete=(k == steps), **{a: np.asarray(v) for a, v in per.items()},
                                 **{a: np.asarray(v) for a, v in at.items()})
+        if spath and (k % 10 == 0 or k == steps):
+            with open(spath + ".tmp", "wb") as f:
+                pickle.dump((slv, per, at, done, k), f, protocol=pickle.HIGHEST_PROTOCOL)
+            os.replace(spath + ".tmp", spath)
 
 
 if __name__ == "__main__":
-    for n in sys.argv[1:] or list(PLAN):
-        run(n)
+    args = sys.argv[1:]
+    state = None
+    if args[:1] == ["--state"]:
+        state, args = args[1], args[2:]
+        os.makedirs(state, exist_ok=True)
+    for n in args or list(PLAN):
+        run(n, state)

@@ -108,6 +108,54 @@ def test_modal_state_bit_exact(prem_text):
         b.commit(load, dil, dm, dl)
 
 
+def test_modal_state_ramp_and_readouts(prem_text):
+    """The second-order placement (ramp commit R_k += (1 - phi_k) L before P += L) and the
+    horizontal and rate read-outs of the device ExactMemory against the oracle's, over steps of
+    two lengths: states and horizontal sums bit for bit (the kernel pass with unit decay rewrites
+    the states unchanged and sums k in order), the rate read-out to 1e-14 of its largest value."""
+    from oracle.love import LoveNumbers as OLove
+    from slevolve_amd import ExactMemory, LoveNumbers
+
+    L = 63
+    rng = np.random.default_rng(12)
+    love_a, love_b = LoveNumbers(prem_text, L, text=True), OLove(prem_text, L, text=True)
+    a = ExactMemory(love_a, L, 500.0, True)
+    b = OMemory(love_b, L, 500.0)
+    a.ramp = b.ramp = True
+    a.horizontal_weights(love_a)
+    b.horizontal_weights(love_b)
+    assert np.array_equal(a.omphi, b.omphi) and np.array_equal(a.wsphi_sl, b.wsphi_sl)
+    assert (a.c_il, a.c_m, a.c_tsl, a.c_trr) == (b.c_il, b.c_m, b.c_tsl, b.c_trr)
+    for step in range(6):
+        if step == 3:
+            a.set_dt(100.0)
+            b.set_dt(100.0)
+        va, ra, *rot_a = a.advance_and_read()
+        vb, rb, *rot_b = b.advance_and_read()
+        assert np.array_equal(va.cpu().numpy(), vb) and np.array_equal(ra.cpu().numpy(), rb)
+        assert all(np.array_equal(x, y) for x, y in zip(rot_a, rot_b))
+        hza, thla = a.read_horizontal()
+        hzb, thlb = b.read_horizontal()
+        assert np.array_equal(hza.cpu().numpy(), hzb) and np.array_equal(thla, thlb)
+        load = np.tril(rng.standard_normal((L + 1, L + 1)) + 1j * rng.standard_normal((L + 1, L + 1)))
+        load[0] = 0
+        load[:, 0] = load[:, 0].real
+        dil, dm, dl = rng.standard_normal((3, 3)), rng.standard_normal(3), rng.standard_normal(3) + 0j
+        rsa, rra = a.read_rate(load)
+        rsb, rrb = b.read_rate(load)
+        for x, y in ((rsa, rsb), (rra, rrb)):
+            assert np.abs(x.cpu().numpy() - y).max() <= 1e-14 * np.abs(y).max()
+        for x, y in zip(a.read_rate_rotation(dil, dm, dl), b.read_rate_rotation(dil, dm, dl)):
+            assert np.array_equal(x, y)
+        a.commit(load, dil, dm, dl)
+        b.commit(load, dil, dm, dl)
+        # states of existing modes (k < nmod[l]; the oracle's loop also carries the weightless others)
+        Ra = torch.stack([a._sht.to_dense(a.R[k]) for k in range(a.kmax)]).cpu().numpy()
+        live = np.arange(a.kmax)[:, None] < a.nmod[None, :]
+        assert np.array_equal(Ra[live], b.R[live]) and np.array_equal(a.p_load.cpu().numpy(), b.p_load)
+        assert np.array_equal(a.r_lambda, b.r_lambda) and np.array_equal(a.r_il, b.r_il)
+
+
 @pytest.mark.parametrize("nlat,nlon,L", [(64, 4096, 63), (16, 8, 3)])
 def test_ring_passes_at_the_row_length_limits(nlat, nlon, L):
     """The ring passes run nlon / 4 threads per ring pair up to nlon = 2048 and nlon / 8 at 4096;
