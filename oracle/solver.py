@@ -65,12 +65,54 @@ def convergence_ratio(sa, so):
     return abs((sa - so) / so)
 
 
+REUSE_MODES = ("off", "warm", "warm+ice", "warm+corr", "warm+corr2")
+
+
+def reuse_guess(store, it, mode):
+    """Warm-start load increment of step it (solver_opt.py:515-528), or None without a stored entry:
+    the stored converged increment of the previous pass, dS_n^(k-1), plus for "warm+corr" the
+    previous step's pass-to-pass change D_{n-1} = dS_{n-1}^(k) - dS_{n-1}^(k-1) and for
+    "warm+corr2" 2 D_{n-1} - D_{n-2} (D_{n-1} alone when step n-2 has no previous-pass value).
+    Returns (guess, corrected)."""
+    rs = store.get(it) if mode != "off" else None
+    if rs is None:
+        return None, False
+    g = rs["dS"].copy()
+    if mode in ("warm+corr", "warm+corr2"):
+        p1 = store.get(it - 1)
+        if p1 is not None and "dS_old" in p1:
+            d1 = p1["dS"] - p1["dS_old"]
+            p2 = store.get(it - 2) if mode == "warm+corr2" else None
+            if p2 is not None and "dS_old" in p2:
+                g += 2.0 * d1 - (p2["dS"] - p2["dS_old"])
+            else:
+                g += d1
+            return g, True
+    return g, False
+
+
+def reuse_store_step(store, it, mode, dS, cxy):
+    """Keeps step it's converged increment and ocean function (solver_opt.py:932-941); the corr
+    modes keep the previous pass's increment as dS_old."""
+    ent = store.get(it)
+    if ent is None:
+        store[it] = {"dS": dS.copy(), "cxy": cxy != 0.0}
+        return
+    if mode in ("warm+corr", "warm+corr2"):
+        ent["dS_old"] = ent["dS"].copy()
+    ent["dS"] = dS.copy()
+    ent["cxy"] = cxy != 0.0
+
+
 class SeaLevelSolver:
     """The step of solver.py:521-1144 with the inner starts of solver_opt.py ("off", "extrap") and
-    the "warm" cross-pass reuse of solver_opt.py:272-286: with reuse="warm" and a caller-owned
+    the cross-pass reuse of solver_opt.py:272-286: with reuse="warm" and a caller-owned
     reuse_store dict (keyed by step number), step n starts from the ocean function and the
     converged load increment that step n stored in the previous outer pass
-    (solver_opt.py:400-402,517-528) and stores its own at the end (solver_opt.py:932-941)."""
+    (solver_opt.py:400-402,517-528) and stores its own at the end (solver_opt.py:932-941);
+    "warm+corr" / "warm+corr2" add the previous steps' pass-to-pass changes to that start
+    (reuse_guess); "warm+ice" starts as "warm" (its stored ice analysis equals the recomputed one
+    whenever the reference would reuse it: same grounded support, same ice)."""
 
     def __init__(self, sht, love, topo0, ice0, *, norder, dt=1.0, tpw=True, checkmarine=True,
                  calcRG=True, epsilon1=1.0e-5, inner="off", planet=None, reuse="off", reuse_store=None,
@@ -83,8 +125,8 @@ class SeaLevelSolver:
         self.anderson_keep = bool(anderson_keep)
         self.inner_stats = {"mixed": 0, "plain": 0, "fallback": 0, "singular": 0, "mask_refresh": 0,
                             "frozen_iters": 0}
-        if reuse not in ("off", "warm"):
-            raise ValueError("reuse must be 'off' or 'warm'")
+        if reuse not in REUSE_MODES:
+            raise ValueError(f"reuse must be one of {REUSE_MODES}")
         self.reuse = reuse
         self.reuse_store = reuse_store if reuse_store is not None else {}
         self.sht, self.love = sht, love
@@ -279,7 +321,7 @@ class SeaLevelSolver:
             load_inc = extrap_guess(h1, h2, self.dt, dt1, dt2)
             dS_cur = dS_prev + load_inc
         if rs is not None:  # warm start: the stored converged increment replaces the guess
-            load_inc = rs["dS"].copy()
+            load_inc, _ = reuse_guess(self.reuse_store, self.iter, self.reuse)
             dS_cur = dS_prev + load_inc
 
         kTE2, hTE2, kE2 = self.love.kTE[2], self.love.hTE[2], self.love.ke[2]
@@ -449,7 +491,7 @@ class SeaLevelSolver:
         self.cxy = (topo < 0.0).astype(np.float64)
         self.topo_prev = topo.copy()
         if self.reuse != "off":
-            self.reuse_store[self.iter] = {"dS": load_inc.copy(), "cxy": self.cxy != 0.0}
+            reuse_store_step(self.reuse_store, self.iter, self.reuse, load_inc, self.cxy)
         area = 4 * PI * p.radius**2
         slc = deltasl * self.cxy
         gmslc_ocn = (sht.analysis(slc)[0, 0] / sht.analysis(self.cxy)[0, 0]).real

@@ -85,6 +85,8 @@ SIGNATURES = {
     "slx_run_steps": (c_int, [c_void_p, c_void_p, C.c_long, c_int, P_double, P_double, P_int,
                               C.POINTER(SlxStatus), c_void_p]),
     "slx_solver_fields": (c_int, [c_void_p, C.POINTER(SlxFields)]),
+    "slx_solver_profile": (c_int, [c_void_p, c_int]),
+    "slx_solver_stage_times": (c_int, [c_void_p, C.POINTER(c_double)]),
 }
 
 

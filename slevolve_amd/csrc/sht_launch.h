@@ -231,6 +231,7 @@ int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field 
   } else {
     k_coef_fill<NF, Coef><<<(h->ncoef + 255) / 256, 256, 0, s>>>(dev_of(h), cf, h->CS, h->ncoef, h->ncoef, pr);
     SLX_LAUNCH_CHECK();
+    slx_mark(h, SLX_STAGE_SPEC, s);
     src = h->CS;
     cstride = h->ncoef;
   }
@@ -307,6 +308,7 @@ int launch_leg_syn32(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int fiel
   } else {
     k_coef_fill<NF, Coef><<<(h->ncoef + 255) / 256, 256, 0, s>>>(dev_of(h), cf, h->CS, h->ncoef, h->ncoef, pr);
     SLX_LAUNCH_CHECK();
+    slx_mark(h, SLX_STAGE_SPEC, s);
     src = h->CS;
     cstride = h->ncoef;
   }

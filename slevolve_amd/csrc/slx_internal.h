@@ -86,6 +86,37 @@ struct slx_sht {
   double2* FSo = nullptr; // [max_fields][nm][nh] synthesis: southern Fourier coefficients
   std::vector<double> h_x, h_w;  // host copies: all nlat nodes / weights (north -> south)
   std::vector<int> h_moff;
+  struct slx_stage_marks* marks = nullptr;  // stage timing of a profiled solver step (else null)
 };
+
+// ------------------------------------------------------------------ stage timing
+// Stages of a profiled step (the keys of the reference's step(profile=True), solver.py:829-831):
+// events recorded on the stream after each group of launches; the time between two consecutive
+// events belongs to the later event's stage (stage -1: not attributed, e.g. the host's gap before
+// a step starts).
+enum { SLX_STAGE_MEMORY = 0, SLX_STAGE_SHT, SLX_STAGE_SPEC, SLX_STAGE_GRID, SLX_STAGE_ROT, SLX_NSTAGE };
+struct slx_stage_marks {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> stage;
+  int n = 0;
+  hipError_t err = hipSuccess;
+};
+static inline void slx_mark(const slx_sht* h, int stage, hipStream_t s) {
+  slx_stage_marks* m = h->marks;
+  if (!m || m->err != hipSuccess) return;
+  if (m->n >= 16384) {  // read out with slx_solver_stage_times at least every few steps
+    m->err = hipErrorOutOfMemory;
+    return;
+  }
+  if (m->n == (int)m->ev.size()) {
+    hipEvent_t e;
+    m->err = hipEventCreate(&e);
+    if (m->err != hipSuccess) return;
+    m->ev.push_back(e);
+    m->stage.push_back(-1);
+  }
+  m->err = hipEventRecord(m->ev[m->n], s);
+  if (m->err == hipSuccess) m->stage[m->n++] = stage;
+}
 
 static inline int slx_moff(int m, int L) { return m * (L + 1) - m * (m - 1) / 2; }

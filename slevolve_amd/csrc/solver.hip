@@ -75,6 +75,7 @@ struct Consts {
 
 struct slx_solver {
   slx_sht* h = nullptr;
+  slx_stage_marks marks;  // stage timing while profiling (slx_solver_profile)
   Consts c;
   int extrap = 0;
   int nhist = 0;         // converged increments stored for the extrapolated start
@@ -1701,6 +1702,8 @@ extern "C" int slx_solver_destroy(slx_solver* s) {
   if (s->hflag) (void)hipHostFree(s->hflag);
   if (s->run_stat) (void)hipHostFree(s->run_stat);
   if (s->ring_tok) (void)hipFree(s->ring_tok);
+  if (s->h && s->h->marks == &s->marks) s->h->marks = nullptr;
+  for (hipEvent_t e : s->marks.ev) (void)hipEventDestroy(e);
   delete s;
   return 0;
 }
@@ -1848,13 +1851,18 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
   const double2* oldice = (s->step == 1) ? s->oldice_lm : s->prev_ice;
   Pts p = make_pts(s, ice_cur, oldice);
   const Consts& c = s->c;
+  slx_mark(s->h, -1, st);
   k_begin<<<19, 64, 0, st>>>(p, c);
   SLX_LAUNCH_CHECK();
+  slx_mark(s->h, SLX_STAGE_ROT, st);
   int rc = launch_modal(s->h, c.kmax, s->nmod, s->decay, s->omd, s->a_sl, s->a_rr, s->R, s->p_load, s->visc,
                         s->rad, st);
   if (rc) return rc;
-  if (warm_ocean)
+  slx_mark(s->h, SLX_STAGE_MEMORY, st);
+  if (warm_ocean) {
     SLX_HIP(hipMemcpyAsync(s->cxy, warm_ocean, (size_t)c.npts * sizeof(double), hipMemcpyDeviceToDevice, st));
+    slx_mark(s->h, SLX_STAGE_GRID, st);
+  }
   const int extrap_ready = (s->extrap && s->nhist >= 2) ? 1 : 0;
   const LoadIceGrid lg{ice_cur, s->topo_prev, s->cxy, s->icestar, s->beta, s->cstar, c.rhow, c.rhoi, c.checkmarine};
   // the eustatic first guess reads all of C*_lm, so only the extrapolated start defers C*
@@ -1870,6 +1878,7 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
     rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->icestar_lm, s->cstar_lm}}, Pred{}, st);
     if (rc) return rc;
   }
+  slx_mark(s->h, SLX_STAGE_SHT, st);
   GuessArgs g;
   g.extrap_ready = extrap_ready;
   g.uniform = (dt0 == dt1 && dt1 == dt2) ? 1 : 0;
@@ -1879,6 +1888,7 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
   g.warm = static_cast<const double2*>(warm_dS);
   k_guess<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, g);
   SLX_LAUNCH_CHECK();
+  slx_mark(s->h, SLX_STAGE_SPEC, st);
   return 0;
 }
 
@@ -1902,6 +1912,7 @@ static int launch_reguess_incremental(slx_solver* s, const LoadReguess& ld, Pred
   int rc = ring_points(h) == 4 ? launch_reguess_ring_p<4>(s, ld, token, pr_ring, st)
                                : launch_reguess_ring_p<8>(s, ld, token, pr_ring, st);
   if (rc) return rc;
+  slx_mark(h, SLX_STAGE_GRID, st);
   return launch_ice_sparse(s, token, pr_ana, st);
 }
 
@@ -1941,18 +1952,22 @@ static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
     rc = launch_leg_ana<2>(s->h, s->h->WS, s->h->WD, OutPtrs<2>{{s->rt_lm, s->cstar_lm}}, Pred{done, 0, cst, 1}, st);
     if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->rt_lm}}, Pred{done, 0, cst, 0}, st);
     if (rc) return rc;
+    slx_mark(s->h, SLX_STAGE_SHT, st);
     k_update<<<nrb, 256, 0, st>>>(p, c, it_step, s->gnew);
     SLX_LAUNCH_CHECK();
+    slx_mark(s->h, SLX_STAGE_SPEC, st);
     k_refresh_aa<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, nrb, nrb);
     SLX_LAUNCH_CHECK();
     k_decide_aa<<<1, 64, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
     k_refresh2_aa<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
+    slx_mark(s->h, SLX_STAGE_GRID, st);
     if (c.checkmarine) {
       rc = launch_fft_ana(s->h, LoadPtrs<1>{{s->icestar}}, 1, Pred{done, 0, changed, 1}, st);
       if (!rc) rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
       if (rc) return rc;
+      slx_mark(s->h, SLX_STAGE_SHT, st);
     }
     k_aa_push<<<nrb, 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
@@ -1962,6 +1977,7 @@ static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
     SLX_LAUNCH_CHECK();
     k_aa_plain<<<grid_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step);
     SLX_LAUNCH_CHECK();
+    slx_mark(s->h, SLX_STAGE_SPEC, st);
   }
   return 0;
 }
@@ -1975,7 +1991,11 @@ static int iter_head(slx_solver* s, hipStream_t st) {
   CoefB18 cb{s->icestar_lm, oldice, s->p_ice, s->p_ocean, s->load_inc, s->visc, s->dS_cur, s->ttlekhl, s->ttl, s->deg,
              s->st, s->c, 1};
   const int* done = &s->st->st.done;
-  if (!s->c.mixed) return launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
+  if (!s->c.mixed) {
+    const int rc = launch_leg_syn<1>(s->h, cb, Pred{done, 0}, st);
+    slx_mark(s->h, SLX_STAGE_SHT, st);
+    return rc;
+  }
   cb.it = s->iters_enqueued;
   // mixed sweeps: the coefficients once, then the FP64 or the FP32-product contraction as the
   // device decided for this iteration (head of iteration iters_enqueued)
@@ -1983,9 +2003,11 @@ static int iter_head(slx_solver* s, hipStream_t st) {
   k_coef_fill<1, CoefB18><<<(h->ncoef + 255) / 256, 256, 0, st>>>(dev_of(h), cb, h->CS, h->ncoef, h->ncoef,
                                                                    Pred{done, 0});
   SLX_LAUNCH_CHECK();
+  slx_mark(h, SLX_STAGE_SPEC, st);
   const int* sw = &s->st->sweep64[s->iters_enqueued & 1];
   int rc = launch_leg_syn<1>(h, CoefPlain{h->CS, h->ncoef}, Pred{done, 0, sw, 1}, st);
   if (!rc) rc = launch_leg_syn32<1>(h, CoefPlain{h->CS, h->ncoef}, Pred{done, 0, sw, 0}, st);
+  slx_mark(h, SLX_STAGE_SHT, st);
   return rc;
 }
 
@@ -2019,20 +2041,25 @@ static int iter_tail(slx_solver* s, hipStream_t st) {
     }
   }
   if (rc) return rc;
+  slx_mark(s->h, SLX_STAGE_SHT, st);
   k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step, s->load_inc);
   SLX_LAUNCH_CHECK();
+  slx_mark(s->h, SLX_STAGE_SPEC, st);
   k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
   SLX_LAUNCH_CHECK();
+  slx_mark(s->h, SLX_STAGE_GRID, st);
   if (c.checkmarine) {
     const LoadReguess ld{s->ice_cur, s->topo, s->icestar, s->beta, c.rhow, c.rhoi, s->st, it_step, c.checkmarine};
     if (s->ice_incr) {  // the re-guess changes I* at a few points: analyse the change only
       rc = launch_reguess_incremental(s, ld, Pred{done, 0}, Pred{done, 0, changed, 1}, st);
     } else {
       rc = launch_fft_ana(s->h, ld, 1, Pred{done, 0}, st);
+      slx_mark(s->h, SLX_STAGE_GRID, st);
       if (!rc)
         rc = launch_leg_ana<1>(s->h, s->h->WS, s->h->WD, OutPtrs<1>{{s->icestar_lm}}, Pred{done, 0, changed, 1}, st);
     }
     if (rc) return rc;
+    slx_mark(s->h, SLX_STAGE_SHT, st);
   }
   return 0;
 }
@@ -2082,12 +2109,14 @@ extern "C" int slx_step_finish_readouts(slx_solver* s, const slx_readouts* ro, v
     if (rc) return rc;
     rc = launch_fft_syn(s->h, StorePlain{s->rr, 0}, 1, fin, st);
     if (rc) return rc;
+    slx_mark(s->h, SLX_STAGE_SHT, st);
   }
   double2* dsl_user = rates ? static_cast<double2*>(ro->dSLdt_lm) : nullptr;
   if (rates || hor) {
     // read-outs on the states before the commit (solver.py:1074-1093)
     k_rot_readout<<<1, 64, 0, st>>>(p, c, rates ? 1 : 0, hor ? 1 : 0, rates ? ro->rot_rates : nullptr);
     SLX_LAUNCH_CHECK();
+    slx_mark(s->h, SLX_STAGE_ROT, st);
     ReadoutSpec a{};
     if (rates) {
       a.dsl = s->scratch_lm;
@@ -2098,25 +2127,68 @@ extern "C" int slx_step_finish_readouts(slx_solver* s, const slx_readouts* ro, v
     if (hor) a.hz = static_cast<double2*>(ro->horiz_lm);
     k_readout_spec<<<(c.ncoef + 255) / 256, 256, 0, st>>>(p, c, a);
     SLX_LAUNCH_CHECK();
+    slx_mark(s->h, SLX_STAGE_SPEC, st);
     if (rates) {  // rate grids without the degree-0 term (solver.py:797-798)
       int rc = launch_leg_syn<2>(s->h, CoefPlain{s->scratch_lm, c.ncoef}, fin, st);
       if (rc) return rc;
       rc = launch_fft_syn(s->h, StorePair{ro->dSLdt, ro->dRdt}, 2, fin, st);
       if (rc) return rc;
+      slx_mark(s->h, SLX_STAGE_SHT, st);
     }
   }
   k_commit<<<(c.ncoef + 255) / 256, 256, 0, st>>>(p, c, s->extrap);
   SLX_LAUNCH_CHECK();
+  slx_mark(s->h, SLX_STAGE_MEMORY, st);
   k_post_grid<<<c.nlat, 256, 0, st>>>(p, c, rates ? ro->dSLdt : nullptr);
   SLX_LAUNCH_CHECK();
+  slx_mark(s->h, SLX_STAGE_GRID, st);
   k_finish<<<1, 256, 0, st>>>(p, c, rates ? 1 : 0, dsl_user, rates ? 0 : 1);
   SLX_LAUNCH_CHECK();
+  slx_mark(s->h, SLX_STAGE_ROT, st);
   if (rates) {
     k_rate_grid<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, ro->dSLdt);
     SLX_LAUNCH_CHECK();
+    slx_mark(s->h, SLX_STAGE_GRID, st);
     k_close<<<1, 64, 0, st>>>(p);
     SLX_LAUNCH_CHECK();
+    slx_mark(s->h, SLX_STAGE_ROT, st);
   }
+  return 0;
+}
+
+extern "C" int slx_solver_profile(slx_solver* s, int on) {
+  SLX_CHECK(s != nullptr, "null solver");
+  SLX_CHECK(!on || s->h->marks == nullptr || s->h->marks == &s->marks,
+            "slx_solver_profile: another solver is profiling on this transform handle");
+  if (on) {
+    s->marks.n = 0;
+    s->marks.err = hipSuccess;
+    s->h->marks = &s->marks;
+  } else if (s->h->marks == &s->marks) {
+    s->h->marks = nullptr;
+  }
+  return 0;
+}
+
+extern "C" int slx_solver_stage_times(slx_solver* s, double* ms) {
+  SLX_CHECK(s && ms, "slx_solver_stage_times: null argument");
+  slx_stage_marks& m = s->marks;
+  for (int i = 0; i < SLX_NSTAGE; ++i) ms[i] = 0.0;
+  if (m.err != hipSuccess) {
+    const hipError_t e = m.err;
+    m.err = hipSuccess;
+    m.n = 0;
+    slx_set_error(std::string("stage timing: ") + hipGetErrorString(e));
+    return (int)e;
+  }
+  if (m.n > 0) SLX_HIP(hipEventSynchronize(m.ev[m.n - 1]));
+  for (int i = 1; i < m.n; ++i) {
+    if (m.stage[i] < 0) continue;
+    float t = 0.0f;
+    SLX_HIP(hipEventElapsedTime(&t, m.ev[i - 1], m.ev[i]));
+    ms[m.stage[i]] += t;
+  }
+  m.n = 0;
   return 0;
 }
 
@@ -2190,6 +2262,12 @@ extern "C" int slx_run_steps(slx_solver* s, const double* ice, long ice_stride, 
   SLX_CHECK(nsteps >= 0 && ice_stride >= s->c.npts, "slx_run_steps: bad nsteps or ice_stride");
   if (nsteps == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
+  struct NoMarks {  // the native loop is not stage-profiled
+    slx_sht* h;
+    slx_stage_marks* saved;
+    ~NoMarks() { h->marks = saved; }
+  } nomarks{s->h, s->h->marks};
+  s->h->marks = nullptr;
   if (nsteps > s->run_stat_cap) {  // grown on demand; later calls of at most this length allocate nothing
     SLX_HIP(hipStreamSynchronize(st));
     if (s->run_stat) SLX_HIP(hipHostFree(s->run_stat));

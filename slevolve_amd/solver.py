@@ -11,10 +11,11 @@ its reused buffers).
 Supported options: tpw, checkmarine, calcRG, epsilon1, dt per step, inner "off" | "extrap" |
 "anderson" (Anderson mixing with the frozen ocean mask, OptSeaLevelSolver), the
 second-order load placement (ramp), the analytic rates (step(rates=True)), the horizontal read-out
-(solver.horizontal = True, result key "horiz"), the "warm" cross-pass reuse of OptSeaLevelSolver
-(per-step converged states kept on the device) and its raw-mode spectra (opt "raw" with a raw-mode
-SHT: dense spectral outputs in V = (-1)^m U).  Axial-rotation variants and the other reuse modes of
-the reference are not provided and raise.
+(solver.horizontal = True, result key "horiz"), the cross-pass reuse modes of OptSeaLevelSolver
+("warm", "warm+corr", "warm+corr2", "warm+ice"; per-step converged states kept on the device), its
+raw-mode spectra (opt "raw" with a raw-mode SHT: dense spectral outputs in V = (-1)^m U) and the
+per-stage device times of step(profile=True).  The axial-rotation variants are not provided and
+raise.
 
 ``precision="mixed"`` (no counterpart in the reference) runs the early inner sweeps of every step
 with the FP32-product Legendre contractions; a sweep takes the FP64 contractions once the previous
@@ -41,6 +42,11 @@ EPS0 = float(np.finfo(np.float64).eps)
 GR_MAXACC = 3
 ALL_OPTS = ("raw", "diag", "dsl", "numba", "sums")
 INNER_MODES = ("off", "extrap", "anderson")
+# cross-pass reuse (solver_opt.py:272-286).  "warm+ice" starts as "warm": the reference reuses the
+# stored ice analysis only where the grounded support is unchanged, i.e. where it equals the
+# recomputed one; here the step-start ice pass is incremental anyway (it transforms the change of
+# I* only), so it is always recomputed (reuse_stats["ice_recomputed"]).
+REUSE_MODES = ("off", "warm", "warm+ice", "warm+corr", "warm+corr2")
 
 
 class Planet:
@@ -305,11 +311,16 @@ class SeaLevelSolver:
             ice.copy_(D.as_device(icexy_cur, torch.float64, self.dev, (self.nlat, self.nlon)))
         h = self._s_handle
         sp = D.stream_ptr()
+        if profile:
+            _lib.call("slx_solver_profile", h, 1)
         rs = self._reuse_store.get(self.iter) if self._reuse != "off" else None
-        if rs is not None:  # warm start from the previous outer pass (solver_opt.py:400-402,517-528)
+        if rs is not None:  # warm start from the previous outer pass (solver_opt.py:400-402,515-528)
+            guess = self._warm_guess(rs)
             _lib.call("slx_step_begin_warm", h, ice.data_ptr(), self.dt, self._dt_hist[0], self._dt_hist[1],
-                      rs["dS"].data_ptr(), rs["cxy"].data_ptr(), sp)
+                      guess.data_ptr(), rs["cxy"].data_ptr(), sp)
             self.reuse_stats["warm_steps"] += 1
+            if self._reuse == "warm+ice":
+                self.reuse_stats["ice_recomputed"] += 1
         else:
             _lib.call("slx_step_begin", h, ice.data_ptr(), self.dt, self._dt_hist[0], self._dt_hist[1], sp)
         n = max([self._guess_iters] + self._recent)  # first batch: the largest count of the last four steps
@@ -323,6 +334,8 @@ class SeaLevelSolver:
                 _lib.call("slx_step_finish_readouts", h, C.byref(ro), sp)
             st = self._status()
             if st.failed:  # the step is not committed (its finish kernels skip a failed step)
+                if profile:
+                    _lib.call("slx_solver_profile", h, 0)
                 raise InnerLoopError(f"inner loop failed to converge after {st.numiter} iterations, xi={st.xi}",
                                      status=_status_record(st))
             if st.finished:
@@ -345,6 +358,10 @@ class SeaLevelSolver:
                 ent = self._reuse_store[self.iter] = {
                     "dS": torch.empty(self.sht.ncoef, dtype=torch.complex128, device=self.dev),
                     "cxy": torch.empty_like(self.cxy)}
+            elif self._reuse in ("warm+corr", "warm+corr2"):  # keep the previous pass's increment
+                if "dS_old" not in ent:
+                    ent["dS_old"] = torch.empty_like(ent["dS"])
+                ent["dS_old"].copy_(ent["dS"])
             _copy_dev(ent["dS"], self._fields.load_inc, self.sht.ncoef * 16)
             ent["cxy"].copy_(self.cxy)
         rate_out = hz = None
@@ -360,7 +377,8 @@ class SeaLevelSolver:
             hz = self.sht.to_dense(self._ro_bufs["horiz"])
         wall = time.perf_counter() - t_all
         if profile:
-            self.prof = {"total": wall}
+            self.prof = self._stage_times()
+            self.prof["total"] = wall
         return {
             "iter": self.iter,
             "numiter": st.numiter,
@@ -383,6 +401,39 @@ class SeaLevelSolver:
             "wall": wall,
             "prof": self.prof if profile else None,
         }
+
+    def _warm_guess(self, rs):
+        """Start increment of a warm step (solver_opt.py:515-528): the stored converged increment of
+        the previous pass, plus for "warm+corr" the previous step's pass-to-pass change
+        D_{n-1} = dS_{n-1}^(k) - dS_{n-1}^(k-1) and for "warm+corr2" 2 D_{n-1} - D_{n-2} (D_{n-1}
+        alone while step n-2 has no previous-pass value); packed device arrays, elementwise."""
+        if self._reuse not in ("warm+corr", "warm+corr2"):
+            return rs["dS"]
+        p1 = self._reuse_store.get(self.iter - 1)
+        if p1 is None or "dS_old" not in p1:
+            return rs["dS"]
+        g = getattr(self, "_guess_buf", None)
+        if g is None:
+            g = self._guess_buf = torch.empty_like(rs["dS"])
+        g.copy_(rs["dS"])
+        d1 = p1["dS"] - p1["dS_old"]
+        p2 = self._reuse_store.get(self.iter - 2) if self._reuse == "warm+corr2" else None
+        if p2 is not None and "dS_old" in p2:
+            g += 2.0 * d1 - (p2["dS"] - p2["dS_old"])
+        else:
+            g += d1
+        self.reuse_stats["corr_steps"] = self.reuse_stats.get("corr_steps", 0) + 1
+        return g
+
+    def _stage_times(self):
+        """Device time per stage of the step just profiled, in seconds, under the reference's keys
+        (solver.py:829-831): memory, sht, spec, grid, rot; profiling is switched off again."""
+        ms = (C.c_double * 5)()
+        try:
+            _lib.call("slx_solver_stage_times", self._s_handle, ms)
+        finally:
+            _lib.call("slx_solver_profile", self._s_handle, 0)
+        return {k: ms[i] / 1e3 for i, k in enumerate(("memory", "sht", "spec", "grid", "rot"))}
 
     def run(self, ice, k0=1, k1=None, dts=None):
         """Steps k0..k1-1 with the ice grids ice[k] (a device float64 tensor [n][nlat][nlon], e.g.
@@ -449,8 +500,8 @@ class OptSeaLevelSolver(SeaLevelSolver):
                  anderson_keep=False, reuse="off", reuse_store=None, **kw):
         if inner not in INNER_MODES:
             raise ValueError(f"inner must be one of {INNER_MODES}, got {inner!r}")
-        if reuse not in ("off", "warm"):
-            raise NotImplementedError(f"reuse={reuse!r} is not provided (off | warm)")
+        if reuse not in REUSE_MODES:
+            raise ValueError(f"reuse must be off|warm|warm+ice|warm+corr|warm+corr2, got {reuse!r}")
         self.opt = parse_opts(opt) if isinstance(opt, str) else set(opt)
         unknown = self.opt - set(ALL_OPTS)
         if unknown:
@@ -468,6 +519,7 @@ class OptSeaLevelSolver(SeaLevelSolver):
         self._reuse = reuse
         self._reuse_store = reuse_store if reuse_store is not None else {}
         self.reuse_store = self._reuse_store
+        self.reuse_stats = {"warm_steps": 0, "ice_reused": 0, "ice_recomputed": 0}
 
     def step(self, icexy_cur, profile=False, dt=None, rates=False):
         r = super().step(icexy_cur, profile=profile, dt=dt, rates=rates)

@@ -1,5 +1,6 @@
 """GPU outer (initial-topography) iteration against the oracle's (run_outer.py:174-279), with and
-without the warm start across passes (solver_opt.py:272-286), through the C-ABI."""
+without the warm starts across passes (solver_opt.py:272-286,515-528: warm, warm+corr, warm+corr2,
+warm+ice), through the C-ABI."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,8 @@ from slevolve_amd import synthetic as syn
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("outer,reuse", [("picard", "off"), ("secant", "warm")])
+@pytest.mark.parametrize("outer,reuse", [("picard", "off"), ("secant", "warm"), ("picard", "warm+corr"),
+                                         ("secant", "warm+corr2"), ("picard", "warm+ice")])
 def test_outer_iteration_matches_oracle(outer, reuse):
     from slevolve_amd import SHT, LoveNumbers
     from slevolve_amd.outer import outer_iteration

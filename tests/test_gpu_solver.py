@@ -153,3 +153,35 @@ def test_lmax1023_workload_vs_oracle_fixture():
         for key in ("gmslc_ocn", "gmslc_ocnBeta"):
             assert abs(r[key] - f[key][k]) < 1e-9, key
         print(f"[lmax1023] step {k + 1} ok", flush=True)
+
+
+@pytest.mark.parametrize("inner", ["extrap", "anderson"])
+def test_profile_stage_keys(prem_text, topo_seed, inner):
+    """step(profile=True) returns the reference's per-stage keys (solver.py:829-831, seconds of
+    device time per stage, from timing events between the step's launch groups) and the total
+    wall time; profiling changes no result bit and switches itself off after the step."""
+    lat, lon, topo, ice = syn.synthetic_earth(128, topo_seed)
+    kw = dict(dt=1000.0, tpw=True, checkmarine=True, calcRG=True, epsilon1=1e-5, inner=inner)
+    a = gpu_solver(128, 63, prem_text, topo, ice[0], **kw)
+    b = gpu_solver(128, 63, prem_text, topo, ice[0], **kw)
+    for k in range(1, 5):
+        print(f"[profile {inner}] step {k}", flush=True)
+        ra, rb = a.step(ice[k], profile=True), b.step(ice[k])
+        assert rb["prof"] is None
+        p = ra["prof"]
+        assert set(p) == {"memory", "sht", "spec", "grid", "rot", "total"}
+        assert all(v >= 0.0 for v in p.values())
+        assert p["sht"] > 0 and p["memory"] > 0 and p["spec"] > 0 and p["grid"] > 0 and p["rot"] > 0
+        dev = sum(v for key, v in p.items() if key != "total")
+        assert dev <= p["total"], p
+        assert ra["numiter"] == rb["numiter"]
+        for key in ("tgrid", "R", "ocean", "beta", "dS_converged"):
+            assert (ra[key] == rb[key]).all(), key
+    # profiling is off again: the native loop runs unprofiled and agrees with the stepped solver
+    import torch
+
+    grids = torch.from_numpy(np.stack([ice[k] for k in range(5, 7)])).cuda()
+    ia = a.run(grids, 0, 2)
+    ib = b.run(grids, 0, 2)
+    assert [r["numiter"] for r in ia] == [r["numiter"] for r in ib]
+    assert (a.topoxy == b.topoxy).all()
