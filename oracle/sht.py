@@ -54,6 +54,32 @@ SCALE_K = 600  # one scale unit: 2^600
 SCALE_T = 300  # a scaled value is renormalised once it exceeds 2^300
 
 
+def sectoral_starts(x: np.ndarray, lmax: int):
+    """Extended-range sectoral start values of ``legendre_block``: for m = 0..lmax the arrays
+    (p0[m], sc[m]) with P~_mm(x_j) = p0 2^(-600 sc) (sc = 0: the value itself)."""
+    x = np.asarray(x, dtype=np.float64)
+    s2 = 1.0 - x * x
+    prod = np.ones_like(x)
+    E = np.zeros(x.size, dtype=np.int64)
+    a, b = 1.0, 0.0
+    p0s, scs = [], []
+    for m in range(lmax + 1):
+        if m:
+            a += 2.0
+            b += 2.0
+            prod = prod * s2 * a / b
+            low = prod < 2.0**-500
+            prod[low] *= 2.0**500
+            E[low] -= 500
+        base = np.sqrt(0.5 * prod)
+        e2 = E // 2
+        lg = np.frexp(base)[1] - 1 + e2  # floor(log2 |P~_mm|)
+        sc = np.maximum(0, np.ceil((-SCALE_T - lg) / SCALE_K)).astype(np.int64)
+        p0s.append(np.ldexp(base, e2 + SCALE_K * sc))
+        scs.append(sc)
+    return p0s, scs
+
+
 def legendre_block(x: np.ndarray, lmax: int):
     """List over m = 0..lmax of arrays ``P[m][l - m, j] = P~_lm(x_j)``.
 
@@ -70,26 +96,12 @@ def legendre_block(x: np.ndarray, lmax: int):
     start values lose precision and its high-degree functions are wrong, these are not.
     """
     x = np.asarray(x, dtype=np.float64)
-    s2 = 1.0 - x * x
-    prod = np.ones_like(x)
-    E = np.zeros(x.size, dtype=np.int64)
-    a, b = 1.0, 0.0
+    p0s, scs = sectoral_starts(x, lmax)
     out = []
     for m in range(lmax + 1):
-        if m:
-            a += 2.0
-            b += 2.0
-            prod = prod * s2 * a / b
-            low = prod < 2.0**-500
-            prod[low] *= 2.0**500
-            E[low] -= 500
         nl = lmax + 1 - m
         P = np.zeros((nl, x.size))
-        base = np.sqrt(0.5 * prod)
-        e2 = E // 2
-        lg = np.frexp(base)[1] - 1 + e2  # floor(log2 |P~_mm|)
-        sc = np.maximum(0, np.ceil((-SCALE_T - lg) / SCALE_K)).astype(np.int64)
-        p0 = np.ldexp(base, e2 + SCALE_K * sc)
+        p0, sc = p0s[m], scs[m].copy()
         P[0] = np.where(sc == 0, p0, 0.0)
         if nl >= 2:
             p1 = np.sqrt(float(2 * m + 3)) * x * p0
