@@ -532,6 +532,13 @@ struct GrpDev {
 };
 
 constexpr int MF_S = 32;  // degrees per segment for the matrix-core analysis
+// LDS row stride (doubles) of a segment's staged recurrence tables.  The lanes of one instruction
+// read the same degree of 4 segments at once; with a stride of 32 doubles those 4 addresses fall in
+// the same bank, with 34 they start 4 banks apart (conflict-free, and 16-byte aligned for b128 reads).
+constexpr int MF_QS = MF_S + 2;
+// LDS stride (double2) of a segment's staged synthesis coefficients: 33 entries put the 4 segments'
+// same-degree entries 4 banks apart.
+constexpr int MF_CS = MF_S + 1;
 
 // acc[v] summed over the lanes that differ in bits 2 and 3 (the 4 MFMA blocks), reduce-scatter:
 // afterwards acc[v] (v < 8) holds the sum of value 16 ((lane >> 3) & 1) + 8 ((lane >> 2) & 1) + v.
@@ -618,7 +625,7 @@ __device__ __forceinline__ void ana_mf_steps(double (&acc)[MF_S], LegState (&st)
   for (int t = 0; t < S; t += 2) {
     if ((t & 7) == 0) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) cc[u] = qcs[slot * S + t + u];
+      for (int u = 0; u < 8; ++u) cc[u] = qcs[slot * MF_QS + t + u];
     }
     double pv[2][R];
 #pragma unroll
@@ -660,8 +667,8 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   if (!pr.active()) return;
   constexpr int S = MF_S;
   extern __shared__ double red[];  // [NW][8][64] wave partials
-  __shared__ double qcs[4 * S];    // scaled recurrence coefficients of the 4 slots
-  __shared__ double qgs[4 * S];    // and the factors P~ = qg Q
+  __shared__ double qcs[4 * MF_QS];  // scaled recurrence coefficients of the 4 slots
+  __shared__ double qgs[4 * MF_QS];  // and the factors P~ = qg Q
   const int per = (gr.ngrp + 7) >> 3;
   const int gi = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (gi >= gr.ngrp) return;
@@ -673,8 +680,8 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
   for (int i = threadIdx.x; i < 4 * S; i += blockDim.x) {
     const int off = (s0 + i / S) * S + i % S;  // l - m
     const bool use = off <= L - m;
-    qcs[i] = use ? d.qc[mo + off] : 0.0;
-    qgs[i] = use ? d.qg[mo + off] : 0.0;
+    qcs[(i / S) * MF_QS + i % S] = use ? d.qc[mo + off] : 0.0;
+    qgs[(i / S) * MF_QS + i % S] = use ? d.qg[mo + off] : 0.0;
   }
   __syncthreads();
   const int blk = (lane >> 2) & 3, slot = lane & 3, kk = lane >> 4;
@@ -716,7 +723,7 @@ __global__ void __launch_bounds__(512) k_leg_ana_mf(ShtDev d, SegDev g, GrpDev g
     const int off = (s0 + row) * S + t;
     if ((s0 + row) < nsm && off <= L - m && (cj >> 1) < NF)
       reinterpret_cast<double*>(out.p[cj >> 1])[2 * (long)(mo + off) + (cj & 1)] =
-          (val * qgs[row * S + t]) / SLX_SQRT2_LEGACY;
+          (val * qgs[row * MF_QS + t]) / SLX_SQRT2_LEGACY;
   };
   if (NW == 1) {
 #pragma unroll
@@ -756,7 +763,7 @@ __device__ __forceinline__ void ana_ck_steps(LegState (&st)[R], const double (&x
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int tt = 8 * c + u + h;
-        const double cq = qcs[slot * S + tt];
+        const double cq = qcs[slot * MF_QS + tt];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           double p;
@@ -819,8 +826,8 @@ k_leg_ana_ck(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, cons
   if (!pr.active()) return;
   constexpr int S = MF_S;
   extern __shared__ double red[];  // [NW][8][64] wave partials
-  __shared__ double qcs[4 * S];    // scaled recurrence coefficients of the 4 slots
-  __shared__ double qgs[4 * S];    // and the factors P~ = qg Q
+  __shared__ double qcs[4 * MF_QS];  // scaled recurrence coefficients of the 4 slots
+  __shared__ double qgs[4 * MF_QS];  // and the factors P~ = qg Q
   const int per = (gr.ngrp + 7) >> 3;
   const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
   const int gend = min(gr.ngrp, (x + 1) * per);
@@ -838,8 +845,8 @@ k_leg_ana_ck(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, cons
     for (int i = threadIdx.x; i < 4 * S; i += blockDim.x) {
       const int off = (s0 + i / S) * S + i % S;  // l - m
       const bool use = off <= L - m;
-      qcs[i] = use ? d.qc[mo + off] : 0.0;
-      qgs[i] = use ? d.qg[mo + off] : 0.0;
+      qcs[(i / S) * MF_QS + i % S] = use ? d.qc[mo + off] : 0.0;
+      qgs[(i / S) * MF_QS + i % S] = use ? d.qg[mo + off] : 0.0;
     }
     const int segi = s0 + slot;
     const bool sval = segi < nsm;
@@ -876,7 +883,7 @@ k_leg_ana_ck(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, cons
       const int off = (s0 + row) * S + t;
       if ((s0 + row) < nsm && off <= L - m && (cj >> 1) < NF)
         reinterpret_cast<double*>(out.p[cj >> 1])[2 * (long)(mo + off) + (cj & 1)] =
-            (sum * qgs[row * S + t]) / SLX_SQRT2_LEGACY;
+            (sum * qgs[row * MF_QS + t]) / SLX_SQRT2_LEGACY;
     }
     __syncthreads();  // the tables and partials are refilled by the next group
   }
@@ -895,8 +902,8 @@ __global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr
   if (!pr.active()) return;
   constexpr int S = MF_S;
   __shared__ double redw[8 * 64];  // the wave's partials of one ring chunk
-  __shared__ double qcs[4 * S];
-  __shared__ double qgs[4 * S];
+  __shared__ double qcs[4 * MF_QS];
+  __shared__ double qgs[4 * MF_QS];
   __shared__ int list[512];  // nh <= 512 on the matrix-core path
   const int L = d.lmax, nh = d.nh, lane = threadIdx.x;
   int tk[8];  // every ring's token loaded first (one round trip), then listed
@@ -925,8 +932,8 @@ __global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr
     for (int i = lane; i < 4 * S; i += 64) {
       const int off = (s0 + i / S) * S + i % S;
       const bool use = off <= L - m;
-      qcs[i] = use ? d.qc[mo + off] : 0.0;
-      qgs[i] = use ? d.qg[mo + off] : 0.0;
+      qcs[(i / S) * MF_QS + i % S] = use ? d.qc[mo + off] : 0.0;
+      qgs[(i / S) * MF_QS + i % S] = use ? d.qg[mo + off] : 0.0;
     }
     const int segi = s0 + slot;
     const bool sval = segi < nsm;
@@ -967,7 +974,7 @@ __global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr
       const int off = (s0 + row) * S + t;
       if ((s0 + row) < nsm && off <= L - m && cj < 2) {
         double* o = reinterpret_cast<double*>(out) + 2 * (long)(mo + off) + cj;
-        *o = *o + (accv[v] * qgs[row * S + t]) / SLX_SQRT2_LEGACY;
+        *o = *o + (accv[v] * qgs[row * MF_QS + t]) / SLX_SQRT2_LEGACY;
       }
     }
     __syncthreads();  // the tables are refilled by the next group
@@ -1239,8 +1246,9 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   // staging, packed: order p's ng0 groups of 4 S degrees, then order L - p's ng1 groups, zero past
   // each order's last degree; np2 = 4 S (ng0 + ng1) <= 4 S (ceil((L + 2) / (4 S)) + 1)
   const int np2 = 4 * S * (ng0 + ng1), base1 = 4 * S * ng0;
-  double* qcl = reinterpret_cast<double*>(lsy);  // [np2] scaled recurrence coefficients
-  double2* cl = lsy + np2 / 2;                   // [NF][np2] coefficients times qg
+  const int nsg = np2 / S;  // staged segments, each padded to MF_QS / MF_CS entries
+  double* qcl = reinterpret_cast<double*>(lsy);  // [nsg][MF_QS] scaled recurrence coefficients
+  double2* cl = lsy + nsg * (MF_QS / 2);         // [NF][nsg][MF_CS] coefficients times qg
   double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
   {
     const int mo0 = d.moff[mord[0]], mo1 = d.moff[mord[1]];
@@ -1250,12 +1258,13 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
       const int k = second ? i - base1 : i;
       const bool v = k < (second ? cnt1 : cnt0);
       const int src = (second ? mo1 : mo0) + k;
-      qcl[i] = v ? d.qc[src] : 0.0;
+      const int sgi = i / S, tq = i % S;
+      qcl[sgi * MF_QS + tq] = v ? d.qc[src] : 0.0;
       const double gq = v ? d.qg[src] : 0.0;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const double2 c = v ? cf[f * cstride + src] : make_double2(0.0, 0.0);
-        cl[f * np2 + i] = make_double2(c.x * gq, c.y * gq);
+        cl[(f * nsg + sgi) * MF_CS + tq] = make_double2(c.x * gq, c.y * gq);
       }
     }
   }
@@ -1302,9 +1311,9 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
       aE[r] = aO[r] = 0.0;
     }
     const double s23 = o ? s23o1 : s23o0;
-    const int ob = (o ? base1 : 0) + sg * S;
-    const double* qco = qcl + ob;
-    const double* clo = reinterpret_cast<const double*>(cl + (colok ? fb : 0) * np2 + ob) + comp;
+    const int ob = (o ? base1 : 0) / S + sg;  // staged segment
+    const double* qco = qcl + ob * MF_QS;
+    const double* clo = reinterpret_cast<const double*>(cl + ((colok ? fb : 0) * nsg + ob) * MF_CS) + comp;
     if (gq == 0)
       syn_mf_steps<R, true>(aE, aO, st, xr, qco, clo, colok, head, s23);
     else
