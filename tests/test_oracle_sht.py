@@ -78,3 +78,33 @@ def test_mean00_weights():
     sht = OracleSHT(64, 128, 63)
     g = np.random.default_rng(1).standard_normal((64, 128))
     assert abs(sht.mean00_weights() @ g.sum(axis=1) - sht.analysis(g)[0, 0].real) < 1e-15
+
+
+@pytest.mark.parametrize("m", [0, 1, 5, 20])
+def test_aliased_top_degree_row(m):
+    """lmax = nlat (the lmax-512 workload's shape): the GL rule of nlat points does not integrate
+    P~_{nlat,m}^2 exactly, so the analysis of the pure mode P~_{nlat,m}(mu) cos(m phi) is not 1 in its
+    own coefficient but sum_j w_j P~^2 / 2 (m > 0; m = 0: sum_j w_j P~^2) / 1.41421356, with P~ from
+    scipy (an implementation independent of the oracle's recursion), and the aliased mass shows in
+    the degree nlat - 2 of the same order as sum_j w_j P~_{nlat} P~_{nlat-2}."""
+    from scipy.special import gammaln
+
+    nlat, nlon = 64, 128
+    L = nlat
+    sht = OracleSHT(nlat, nlon, L)
+    x, w = sht.gaulats, sht.weights
+
+    def pt(l):
+        lognorm = 0.5 * (np.log((2 * l + 1) / 2) + gammaln(l - m + 1) - gammaln(l + m + 1))
+        return (-1) ** m * lpmv(m, l, x) * np.exp(lognorm)
+
+    p_top, p_low = pt(L), pt(L - 2)
+    phi = 2 * np.pi * np.arange(nlon) / nlon
+    g = p_top[:, None] * np.cos(m * phi)[None, :]
+    U = sht.analysis(g)
+    fac = 1.0 if m == 0 else 0.5
+    top = fac * float((w * p_top * p_top).sum()) / SQRT2_LEGACY
+    low = fac * float((w * p_top * p_low).sum()) / SQRT2_LEGACY
+    assert abs(top - fac / SQRT2_LEGACY) > 1e-4  # the row is aliased, not exact
+    assert abs(U[L, m].real - top) < 1e-12 and abs(U[L, m].imag) < 1e-12
+    assert abs(U[L - 2, m].real - low) < 1e-12
