@@ -1,11 +1,13 @@
-// Device-resident sea-level time step (FP64).  Restates python/slevolve/solver.py:521-1144 (step
+// Device-resident sea-level time step (FP64; optionally FP32-product contractions in the early
+// inner sweeps).  Restates python/slevolve/solver.py:521-1144 (step
 // or second-order load placement; the optional rate and horizontal read-outs of solver.py:687-806)
 // and the extrapolated start and Anderson mixing of solver_opt.py:67-165,504-535,705-775 with the
 // grid and spectral algebra fused into the transform passes:
 //   * rO - tO is analysed as one field (its analysis is all the update uses, solver.py:1002-1015);
 //   * the ocean function C* is analysed in the same sweep as rO - tO of the next iteration;
 //   * deltasl = dslxy + conserv K00 (one synthesis less per iteration, solver_opt.py:28-32);
-//   * eq. B18 is assembled while the synthesis stages its coefficients.
+//   * eq. B18 is assembled by one coefficient pass (k_coef_fill with the CoefB18 loader, which also
+//     evaluates the rotation) straight into the synthesis contraction's input.
 // The inner iterations are launched ahead and predicated on a device convergence flag.
 #include <cmath>
 #include <cstring>
