@@ -111,7 +111,7 @@ def test_gradient_vs_central_differences_lmax256():
     # the Function's FP32 gradient is the FP64 gradient of its (FP32-rounded) input, rounded
     J32in, g32in = model.gradient(ice32.detach().double(), pts)
     assert ice32.grad.dtype == torch.float32 and torch.equal(ice32.grad, g32in.float())
-    assert float(J) == float(torch.tensor(float(J32in[0]), dtype=torch.float32))
+    assert float(J.detach()) == float(torch.tensor(float(J32in[0]), dtype=torch.float32))
     print(f"[adjoint] J = {float(J64[0]):.6e}, |grad| max {float(g64.abs().max()):.3e}", flush=True)
     seed = int(np.random.SeedSequence(20261203 + 5).spawn(5)[4].generate_state(1)[0])
     gen = torch.Generator(device="cuda").manual_seed(seed)
