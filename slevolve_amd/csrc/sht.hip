@@ -373,6 +373,9 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
     SLX_HIP(hipEventDestroy(e1));
     SLX_HIP(hipFree(tmp));
   }
+  // tables, seeds and work buffers were built on the null stream, which does not order itself with
+  // non-blocking streams: they are complete before any stream may use the handle
+  SLX_HIP(hipStreamSynchronize(nullptr));
   *out = h;
   return 0;
 }

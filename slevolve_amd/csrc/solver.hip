@@ -122,6 +122,10 @@ struct slx_solver {
   int* hflag = nullptr;
   int* hflag_dev = nullptr;
   int pipelined = 0;
+  // the stream of the latest enqueued work (the step tables that slx_solver_set_dt rewrites are
+  // read by it until it drains)
+  hipStream_t last_stream = nullptr;
+  int used = 0;
   // token of the step in progress in the posted decisions: a per-handle count of started steps that
   // no re-initialisation resets, so that a decision posted before slx_solver_init never matches
   // a later step
@@ -1685,6 +1689,9 @@ extern "C" int slx_solver_create(slx_sht* h, const slx_solver_params* p, const s
   }
   SLX_HIP(hipMalloc((void**)&s->ring_tok, (size_t)h->nh * sizeof(int)));
   SLX_HIP(hipMemset(s->ring_tok, 0, (size_t)h->nh * sizeof(int)));
+  // the set-up's memsets and copies went to the null stream, which does not order itself with
+  // non-blocking streams: they complete before any stream may use the solver
+  SLX_HIP(hipStreamSynchronize(nullptr));
   *out = s;
   return 0;
 }
@@ -1724,6 +1731,8 @@ static void ramp_factor(double x, double& phi, double& omp) {
 extern "C" int slx_solver_set_dt(slx_solver* s, double dt) {
   SLX_CHECK(s != nullptr, "null solver");
   if (dt == s->dt) return 0;
+  // enqueued steps still read the decay, ramp and degree tables rewritten below
+  if (s->used) SLX_HIP(hipStreamSynchronize(s->last_stream));
   const int K = s->c.kmax, nl = s->c.nl;
   const std::vector<int>& nmod = s->h_nmod;
   std::vector<double> d((size_t)K * nl, 0.0), o((size_t)K * nl, 0.0);
@@ -1780,6 +1789,7 @@ extern "C" int slx_solver_set_dt(slx_solver* s, double dt) {
     SLX_HIP(hipMemcpy(s->omphi, omphi.data(), omphi.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   SLX_HIP(hipMemcpy(s->deg, deg.data(), deg.size() * sizeof(double), hipMemcpyHostToDevice));
+  SLX_HIP(hipStreamSynchronize(nullptr));  // the copies land before work on any stream reads them
   s->dt = dt;
   return 0;
 }
@@ -1787,6 +1797,8 @@ extern "C" int slx_solver_set_dt(slx_solver* s, double dt) {
 extern "C" int slx_solver_init(slx_solver* s, const double* topo0, const double* ice0, void* stream) {
   SLX_CHECK(s && topo0 && ice0, "slx_solver_init: null argument");
   hipStream_t st = (hipStream_t)stream;
+  s->last_stream = st;
+  s->used = 1;
   Pts p = make_pts(s, nullptr, s->oldice_lm);
   k_init_grid<<<grid_blocks(s->c.npts), 256, 0, st>>>(p, s->c, topo0, ice0);
   SLX_LAUNCH_CHECK();
@@ -1845,6 +1857,8 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
   SLX_CHECK(s->dt > 0.0, "slx_solver_set_dt must be called before the first step");
   hipStream_t st = (hipStream_t)stream;
   SLX_CHECK(s->closed, "slx_step_begin: the previous step has not finished (check slx_solver_status)");
+  s->last_stream = st;
+  s->used = 1;
   s->step += 1;
   s->epoch += 1;
   s->closed = 0;
