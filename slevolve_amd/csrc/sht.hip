@@ -252,6 +252,137 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
     for (size_t i = 0; i < gm.size(); ++i) gd[i] = make_int4(gm[i], gs[i], first[gm[i]] + gs[i], h->h_moff[gm[i]]);
     ALLOC_COPY(h->grp_desc, gd);
   }
+  // walk analysis layout (k_leg_ana_qw): order m is cut into 4 W chains of Q steps, W the least
+  // that keeps Q <= T; a unit = 4 chains (one walk) with its recurrence tables (restart at
+  // t = 32 k, chain 0 at t = 2 after its head), start states and reload rows; units are dealt
+  // longest first to the least loaded of the blocks (at most 3 units per block).  By default T is
+  // the longest of (unbounded, 256, 128, 96, 64, 48, 32) that still gives >= 768 units (3 per CU:
+  // longer chains load fewer start states, more units keep more waves resident) and every unit
+  // has a block of its own; SLX_QW_T / SLX_QW_BLOCKS override both.
+  if (h->ana_mfma) {
+    auto units_for = [&](int T) {
+      long n_u = 0;
+      for (int m = 0; m < nm; ++m) n_u += std::max(1, (L - m + 1 + 4 * T - 1) / (4 * T));
+      return n_u;
+    };
+    int T = 32;
+    for (int cand : {1 << 20, 256, 128, 96, 64, 48}) {
+      if (units_for(cand) >= 768) {
+        T = cand;
+        break;
+      }
+    }
+    if (const char* e = std::getenv("SLX_QW_T")) T = std::max(8, std::atoi(e));
+    std::vector<int4> ord(nm), ua, ub;
+    std::vector<int> u0(nm);
+    std::vector<double> tqc, tqg;
+    int roff = 0, npts = 0, qsmax = 0;
+    for (int m = 0; m < nm; ++m) {
+      const int n = L - m + 1;
+      const int W = std::max(1, (n + 4 * T - 1) / (4 * T));
+      int Q = (n + 4 * W - 1) / (4 * W);
+      Q += Q & 1;
+      const int Qp = 8 * ((Q + 7) / 8), nk = (Q + 31) / 32;
+      SLX_CHECK(nk - 1 <= 8, "walk analysis: a chain longer than 288 degrees");
+      ord[m] = make_int4(Q, 4 * W, nk, npts);
+      npts += 4 * W * nk;
+      u0[m] = (int)ua.size();
+      qsmax = std::max(qsmax, Qp + 2);
+      for (int w = 0; w < W; ++w) {
+        ua.push_back(make_int4(m, Q, Qp, (int)tqc.size()));
+        ub.push_back(make_int4(nk, roff, 0, 4 * w));
+        roff += 4 * (nk - 1);
+        for (int i = 0; i < 4; ++i) {
+          const int c = 4 * w + i;
+          long double g = 1.0L;
+          for (int t = 0; t < Qp; ++t) {
+            const int l = m + c * Q + t;
+            double cq = 0.0, gg = 0.0;
+            if (t < Q && l <= L) {
+              if (c == 0 && t < 2) {
+                gg = 1.0;  // head: P~_m, P~_{m+1} unscaled
+              } else {
+                const int rs = (c == 0 && t < 32) ? 2 : (t & ~31);
+                if (t == rs) g = 1.0L;
+                const long double lm1 = l - 1, mm = m, ll = l;
+                const long double ep = std::sqrt((lm1 * lm1 - mm * mm) / (4.0L * lm1 * lm1 - 1.0L));
+                const long double e = std::sqrt((ll * ll - mm * mm) / (4.0L * ll * ll - 1.0L));
+                cq = (double)((t == rs) ? ep : ep * ep);
+                g /= e;
+                gg = (double)g;
+              }
+            }
+            tqc.push_back(cq);
+            tqg.push_back(gg);
+          }
+        }
+      }
+    }
+    const int nunit = (int)ua.size();
+    h->qw_qsmax = qsmax;
+    int nblk = nunit;
+    if (const char* e = std::getenv("SLX_QW_BLOCKS")) nblk = std::atoi(e);
+    nblk = std::max(nblk, (nunit + 2) / 3);
+    {
+      std::vector<int> idx(nunit);
+      for (int u = 0; u < nunit; ++u) idx[u] = u;
+      std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ua[a].z > ua[b].z; });
+      std::vector<long> load(nblk, 0);
+      std::vector<int4> blk(nblk, make_int4(-1, -1, -1, -1));
+      std::vector<int> cnt(nblk, 0);
+      for (int u : idx) {
+        int best = -1;
+        for (int b = 0; b < nblk; ++b)
+          if (cnt[b] < 3 && (best < 0 || load[b] < load[best])) best = b;
+        int* slotp = cnt[best] == 0 ? &blk[best].x : cnt[best] == 1 ? &blk[best].y : &blk[best].z;
+        *slotp = u;
+        ++cnt[best];
+        load[best] += ua[u].z;
+      }
+      int used = 0;  // drop empty blocks
+      for (int b = 0; b < nblk; ++b)
+        if (cnt[b] > 0) blk[used++] = blk[b];
+      blk.resize(used);
+      h->qw_nblk = used;
+      ALLOC_COPY(h->qw_blk, blk);
+    }
+    int *d_u0 = nullptr, *d_umask = nullptr;
+    int4* d_ord = nullptr;
+    ALLOC_COPY(h->qw_ua, ua);
+    ALLOC_COPY(h->qw_ub, ub);
+    ALLOC_COPY(h->qw_qc, tqc);
+    ALLOC_COPY(h->qw_qg, tqg);
+    ALLOC_COPY(d_ord, ord);
+    ALLOC_COPY(d_u0, u0);
+    SLX_HIP(hipMalloc(&h->qw_seed, (size_t)nunit * 4 * nh * sizeof(double2)));
+    SLX_HIP(hipMalloc(&h->qw_rl, (size_t)std::max(roff, 1) * nh * sizeof(double2)));
+    SLX_HIP(hipMalloc(&d_umask, (size_t)nunit * sizeof(int)));
+    SLX_HIP(hipMemset(d_umask, 0, (size_t)nunit * sizeof(int)));
+    {
+      std::vector<int2> band((size_t)nunit * 32, make_int2(nh, 0));  // empty bands [nh, 0)
+      ALLOC_COPY(h->qw_band, band);
+    }
+    double2* pab = nullptr;
+    int* psc = nullptr;
+    SLX_HIP(hipMalloc(&pab, (size_t)npts * nh * sizeof(double2)));
+    SLX_HIP(hipMalloc(&psc, (size_t)npts * nh * sizeof(int)));
+    dim3 grid(nm, (nh + 63) / 64);
+    const QwOrd qo{d_ord, d_u0};
+    k_qw_seed_walk<<<grid, 64>>>(dev_of(h), qo, pab, psc);
+    SLX_LAUNCH_CHECK();
+    k_qw_seed_unscale<<<grid, 64>>>(dev_of(h), qw_of(h), qo, pab, psc, h->qw_seed, h->qw_rl, d_umask, h->qw_band);
+    SLX_LAUNCH_CHECK();
+    SLX_HIP(hipDeviceSynchronize());
+    std::vector<int> umask(nunit);
+    SLX_HIP(hipMemcpy(umask.data(), d_umask, (size_t)nunit * sizeof(int), hipMemcpyDeviceToHost));
+    for (int u = 0; u < nunit; ++u) ub[u].z = umask[u];
+    SLX_HIP(hipMemcpy(h->qw_ub, ub.data(), ub.size() * sizeof(int4), hipMemcpyHostToDevice));
+    SLX_HIP(hipFree(pab));
+    SLX_HIP(hipFree(psc));
+    SLX_HIP(hipFree(d_ord));
+    SLX_HIP(hipFree(d_u0));
+    SLX_HIP(hipFree(d_umask));
+  }
 #undef ALLOC_COPY
   SLX_HIP(hipMalloc(&h->WS, wsz));
   SLX_HIP(hipMalloc(&h->WD, wsz));
@@ -262,9 +393,10 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
   SLX_HIP(hipMemset(h->WD, 0, wsz));
   // choice of the analysis contraction.  By default a fixed rule per shape, so that every handle
   // of a shape sums in the same order and results repeat bit for bit across handles and runs:
-  // the chunked-accumulator persistent kernel with 2 rings per lane where it fits (nh <= 256),
-  // else one group per block with 4 rings per lane (matrix-core path); 1 wave per segment where
-  // it fits, else 2 (VALU path).  SLX_ANA_VARIANT = mf2 | mf4 | ck2 | ck4 fixes another variant;
+  // the walk kernel with 4 rings per lane from 256 ring pairs on (both benchmark grids), the
+  // chunked-accumulator persistent kernel with 2 rings per lane below (matrix-core path); 1 wave
+  // per segment where it fits, else 2 (VALU path).  SLX_ANA_VARIANT = mf2 | mf4 | ck2 | ck4 | qw2 |
+  // qw4 fixes another variant;
   // SLX_ANA_VARIANT = auto times the candidates (2 fields, or 1 for a 1-field handle) on the
   // device with the handle's own shapes and keeps the fastest.
   {
@@ -275,21 +407,23 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
     SLX_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int best_v = -1;
-    // matrix-core candidates v = 1..4: rings per lane 2 or 4, one group per block (v = 1, 2) or
-    // the chunked-accumulator persistent kernel (v = 3, 4)
     const char* av = std::getenv("SLX_ANA_VARIANT");
     std::string awant = av ? av : "";
-    if (awant.empty()) awant = h->ana_mfma ? ((nh + 31) / 32 <= 8 ? "ck2" : "mf4") : "";
+    if (awant.empty()) awant = h->ana_mfma ? (nh >= 256 ? "qw4" : "ck2") : "";
     const bool timed = awant == "auto";
-    const int vmax = h->ana_mfma ? 4 : 2;
+    // v = 1, 2: one group per block; 3, 4: persistent chunked; 5, 6: quarter walk (odd v: 2 rings
+    // per lane, even: 4)
+    const int vmax = h->ana_mfma ? 6 : 2;
+    static const char* const kind[] = {"mf", "ck", "qw"};
     for (int v = 1; v <= vmax; ++v) {
       if (h->ana_mfma) {
-        const int r = (v & 1) ? 2 : 4, ck = v > 2 ? 1 : 0;
+        const int r = (v & 1) ? 2 : 4, kv = (v - 1) / 2;
         if ((nh + 16 * r - 1) / (16 * r) > 8) continue;
-        if (!timed && awant != std::string(ck ? "ck" : "mf") + std::to_string(r)) continue;
-        if (timed && ck && r == 4) continue;  // spills at its occupancy: only when asked for
+        if (!timed && awant != std::string(kind[kv]) + std::to_string(r)) continue;
+        if (timed && kv == 1 && r == 4) continue;  // spills at its occupancy: only when asked for
         h->ana_r = r;
-        h->ana_ck = ck;
+        h->ana_ck = kv == 1 ? 1 : 0;
+        h->ana_qw = kv == 2 ? 1 : 0;
       } else {
         if ((nh + 64 * v - 1) / (64 * v) > 4) continue;
         if (!timed && best_v > 0) continue;  // the first that fits
@@ -319,13 +453,15 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
     if (best_v > 0) {
       if (h->ana_mfma) {
         h->ana_r = (best_v & 1) ? 2 : 4;
-        h->ana_ck = best_v > 2 ? 1 : 0;
+        h->ana_ck = (best_v - 1) / 2 == 1 ? 1 : 0;
+        h->ana_qw = (best_v - 1) / 2 == 2 ? 1 : 0;
       } else {
         h->ana_nw = best_v;
       }
     } else if (h->ana_mfma) {  // the requested variant does not fit this handle: 4 rings per lane
       h->ana_r = 4;
       h->ana_ck = 0;
+      h->ana_qw = 0;
     }
     // synthesis kernel: scalar-fed VALU, or matrix cores with 2 or 4 rings per lane (the latter
     // only with the matrix-core segments).  By default a fixed rule (matrix cores with 2 rings per
@@ -384,7 +520,8 @@ extern "C" int slx_sht_destroy(slx_sht* h) {
   if (!h) return 0;
   void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->stw,
                   h->WS, h->WD, h->FN, h->FSo, h->seg_m, h->seg_l0, h->seg_first, h->seed_a, h->seed_b,
-                  h->seed_s, h->seed_ab, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg, h->grp_desc};
+                  h->seed_s, h->seed_ab, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg, h->grp_desc,
+                  h->qw_blk, h->qw_ua, h->qw_ub, h->qw_qc, h->qw_qg, h->qw_seed, h->qw_rl, h->qw_band};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -399,9 +536,9 @@ extern "C" int slx_sht_info(const slx_sht* h, int* dims, double* gaulats, double
     dims[2] = h->lmax;
     dims[3] = h->ncoef;
     dims[4] = h->max_fields;
-    // kernels chosen at creation: analysis (0 one group per block, 1 persistent chunked; rings per
-    // lane), synthesis (0 scalar-fed VALU, 1 matrix cores; rings per lane)
-    dims[5] = h->ana_mfma ? h->ana_ck : -1;
+    // kernels chosen at creation: analysis (0 one group per block, 1 persistent chunked, 2 quarter
+    // walk; rings per lane), synthesis (0 scalar-fed VALU, 1 matrix cores; rings per lane)
+    dims[5] = h->ana_mfma ? (h->ana_qw ? 2 : h->ana_ck) : -1;
     dims[6] = h->ana_mfma ? h->ana_r : 0;
     dims[7] = h->syn_mfma;
     dims[8] = h->syn_r;

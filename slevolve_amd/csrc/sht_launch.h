@@ -5,6 +5,7 @@
 #include <type_traits>
 
 #include "sht_kernels.h"
+#include "sht_kernels_qw.h"
 #include "sht_kernels_x32.h"
 
 namespace slx {
@@ -68,6 +69,24 @@ inline SegDev seg_of(const slx_sht* h) {
   return g;
 }
 
+// Dynamic LDS above the default 64 KB is opted into per kernel, raised monotonically under a
+// lock (host threads driving different handles may launch the same kernel with different sizes).
+inline std::mutex& lds_mutex() {
+  static std::mutex mu;
+  return mu;
+}
+template <auto KERN>
+int allow_lds(size_t lds) {
+  static size_t allowed = 65536;
+  std::lock_guard<std::mutex> lock(lds_mutex());
+  if (lds > allowed) {
+    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(KERN), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    allowed = lds;
+  }
+  return 0;
+}
+
 inline int ana_rings_per_lane(const slx_sht* h) {
   const int r = (h->nh + 64 * h->ana_nw - 1) / (64 * h->ana_nw);
   return r <= 1 ? 1 : r <= 2 ? 2 : 4;
@@ -112,6 +131,32 @@ int launch_leg_ana_ck(const slx_sht* h, const double2* WS, const double2* WD, Ou
   return 0;
 }
 
+inline QwDev qw_of(const slx_sht* h) {
+  return QwDev{h->qw_blk, h->qw_ua, h->qw_ub, h->qw_qc, h->qw_qg, h->qw_seed, h->qw_rl, h->qw_band};
+}
+// Walk analysis: one block per entry of the unit-block list, 16 R rings per wave; CHB = 4
+// chunks between reductions when two blocks still fit a CU's LDS, else 2.
+inline size_t qw_lds(const slx_sht* h, int nw, int chb) {
+  return (size_t)(2 * nw * 2 * chb * 64 + 2 * 4 * h->qw_qsmax) * sizeof(double) + 32 * sizeof(int2);
+}
+template <int NF, int R, int CHB>
+int launch_leg_ana_qw_c(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
+                        hipStream_t s) {
+  const int nw = (h->nh + 16 * R - 1) / (16 * R);
+  const size_t lds = qw_lds(h, nw, CHB);
+  if (int rc = allow_lds<&k_leg_ana_qw<NF, R, CHB>>(lds)) return rc;
+  k_leg_ana_qw<NF, R, CHB><<<h->qw_nblk, 64 * nw, lds, s>>>(dev_of(h), qw_of(h), WS, WD, out, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
+template <int NF, int R>
+int launch_leg_ana_qw(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
+                      hipStream_t s) {
+  const int nw = (h->nh + 16 * R - 1) / (16 * R);
+  if (qw_lds(h, nw, 4) <= 80 * 1024) return launch_leg_ana_qw_c<NF, R, 4>(h, WS, WD, out, pr, s);
+  return launch_leg_ana_qw_c<NF, R, 2>(h, WS, WD, out, pr, s);
+}
+
 template <int NF>
 int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
                    hipStream_t s) {
@@ -121,6 +166,9 @@ int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPt
       int rc = launch_leg_ana<2>(h, WS, WD, OutPtrs<2>{{out.p[0], out.p[1]}}, pr, s);
       if (rc) return rc;
       return launch_leg_ana<1>(h, WS + 2 * fs, WD + 2 * fs, OutPtrs<1>{{out.p[2]}}, pr, s);
+    } else if (h->ana_qw) {
+      return h->ana_r == 4 ? launch_leg_ana_qw<NF, 4>(h, WS, WD, out, pr, s)
+                           : launch_leg_ana_qw<NF, 2>(h, WS, WD, out, pr, s);
     } else if (h->ana_ck) {
       return h->ana_r == 4 ? launch_leg_ana_ck<NF, 4>(h, WS, WD, out, pr, s)
                            : launch_leg_ana_ck<NF, 2>(h, WS, WD, out, pr, s);
@@ -152,24 +200,6 @@ int launch_leg_ana(const slx_sht* h, const double2* WS, const double2* WD, OutPt
 }  // namespace slx
 
 namespace slx {
-// Dynamic LDS above the default 64 KB is opted into per kernel, raised monotonically under a
-// lock (host threads driving different handles may launch the same kernel with different sizes).
-inline std::mutex& lds_mutex() {
-  static std::mutex mu;
-  return mu;
-}
-template <auto KERN>
-int allow_lds(size_t lds) {
-  static size_t allowed = 65536;
-  std::lock_guard<std::mutex> lock(lds_mutex());
-  if (lds > allowed) {
-    SLX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(KERN), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
-    allowed = lds;
-  }
-  return 0;
-}
-
 // Ring passes: points per thread (4 up to nlon = 2048, else 8), LDS = two rows.
 inline int ring_points(const slx_sht* h) { return h->nlon <= 4 * RING_MAXNT ? 4 : 8; }
 inline size_t ring_lds(const slx_sht* h) { return 2 * (size_t)h->nlon * sizeof(double2); }

@@ -76,6 +76,16 @@ struct slx_sht {
   int ana_mfma = 0, ana_r = 2, ngrp = 0;
   int ana_ck = 0;         // chunked-accumulator persistent analysis kernel (else one group per block)
   int4* grp_desc = nullptr;  // [ngrp] (m, first segment s0, seed row first[m] + s0, moff[m])
+  // walk analysis (k_leg_ana_qw): orders cut into chains of at most a target length, 4 chains per
+  // unit, units packed onto blocks
+  int ana_qw = 0;         // the analysis runs the walk kernel (ana_r rings per lane)
+  int qw_nblk = 0, qw_qsmax = 0;
+  int4* qw_blk = nullptr;   // [qw_nblk] units of a block, -1 padded
+  int4* qw_ua = nullptr;    // [nunit] (m, Q, Qpad, table offset)
+  int4* qw_ub = nullptr;    // [nunit] (sub-segments per chain, reload row offset, reload mask, first chain)
+  double *qw_qc = nullptr, *qw_qg = nullptr;
+  double2 *qw_seed = nullptr, *qw_rl = nullptr;
+  int2* qw_band = nullptr;  // [nunit][4][8] ring band [j0, j1) reloading at each boundary
   int syn_mfma = 0;       // synthesis on the matrix cores (else wave-uniform scalar-fed VALU)
   int syn_r = 4;          // rings per lane of the matrix-core synthesis (2 or 4)
   int *grp_m = nullptr, *grp_s0 = nullptr;

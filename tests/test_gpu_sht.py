@@ -45,15 +45,21 @@ def test_synthesis_and_analysis_match_oracle(nlat, L, variant, monkeypatch):
         assert np.abs(back - c).max() / np.abs(c).max() < 1e-12 * max(1.0, L / 511)
 
 
-@pytest.mark.parametrize("variant", ["mf2", "mf4", "ck2", "ck4"])
+@pytest.mark.parametrize("variant", ["mf2", "mf4", "ck2", "ck4", "qw2", "qw4", "qw4:48:257", "qw2:1048576:300"])
 @pytest.mark.parametrize("nlat,L", [(64, 63), (512, 512), (1024, 1023)])
 def test_analysis_variants_match_oracle(nlat, L, variant, monkeypatch):
-    """Every matrix-core analysis kernel (one group per block or persistent with chunked
-    accumulators, 2 or 4 rings per lane; SLX_ANA_VARIANT is read when a handle is created, and a
-    shape that does not fit falls back to 4 rings) against the oracle, 1 and 2 fields."""
+    """Every matrix-core analysis kernel (one group per block, persistent with chunked
+    accumulators, or walking long chains, 2 or 4 rings per lane; SLX_ANA_VARIANT is read when a
+    handle is created, and a shape that does not fit falls back to 4 rings) against the oracle, 1
+    and 2 fields.  "qw4:T:B" sets the walk kernel's chain length and block count (SLX_QW_T,
+    SLX_QW_BLOCKS) so that blocks run several units one after another."""
     from slevolve_amd import SHT
 
-    monkeypatch.setenv("SLX_ANA_VARIANT", variant)
+    name, *walk = variant.split(":")
+    monkeypatch.setenv("SLX_ANA_VARIANT", name)
+    if walk:
+        monkeypatch.setenv("SLX_QW_T", walk[0])
+        monkeypatch.setenv("SLX_QW_BLOCKS", walk[1])
     print(f"[sht] nlat {nlat} lmax {L} analysis {variant}", flush=True)
     nlon = 2 * nlat
     rng = np.random.default_rng(nlat + L + 7)
