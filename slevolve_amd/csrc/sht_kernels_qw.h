@@ -49,6 +49,32 @@ struct QwDev {
   const int2* band;      // [nunit][4][8] rings [j0, j1) that reload at boundary k (entry k - 1)
 };
 
+// 64-bit DPP moves within 16-lane rows.  dpp_merge<CA, CB, BANKS>(a, b): lanes in the 4-lane
+// banks of BANKS take a moved by CA, the other lanes b moved by CB.  bank_merge<BANKS>(a, b): a in
+// BANKS, b elsewhere (identity quad permutation).
+template <int CTRL, int BANKS>
+__device__ __forceinline__ int dpp32(int old, int src) {
+  return __builtin_amdgcn_update_dpp(old, src, CTRL, 0xF, BANKS, false);
+}
+template <int CA, int CB, int BANKS>
+__device__ __forceinline__ double dpp_merge(double a, double b) {
+  const int2 ia = __builtin_bit_cast(int2, a), ib = __builtin_bit_cast(int2, b);
+  int2 r;
+  r.x = dpp32<CB, 0xF & ~BANKS>(0, ib.x);
+  r.y = dpp32<CB, 0xF & ~BANKS>(0, ib.y);
+  r.x = dpp32<CA, BANKS>(r.x, ia.x);
+  r.y = dpp32<CA, BANKS>(r.y, ia.y);
+  return __builtin_bit_cast(double, r);
+}
+template <int BANKS>
+__device__ __forceinline__ double bank_merge(double a, double b) {
+  const int2 ia = __builtin_bit_cast(int2, a), ib = __builtin_bit_cast(int2, b);
+  int2 r;
+  r.x = dpp32<0xE4, BANKS>(ib.x, ia.x);
+  r.y = dpp32<0xE4, BANKS>(ib.y, ia.y);
+  return __builtin_bit_cast(double, r);
+}
+
 template <int R, bool HEAD>
 __device__ __forceinline__ void qw_chunk(LegState (&st)[R], const double (&xr)[R], const double (&wv)[2][R],
                                          const double* qcs, int t0, bool head, double s23, double* redw,
@@ -89,23 +115,22 @@ __device__ __forceinline__ void qw_chunk(LegState (&st)[R], const double (&xr)[R
       acc[u + 1] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv[1][r], wv[1][r], acc[u + 1], 0, 0, 0);
     }
   }
-  {
-    const bool up = (lane & 8) != 0;
+  // reduce-scatter over the 4 MFMA blocks (lane bits 3, 2) with row rotations of 16-lane rows:
+  // level 1 pairs lane i with i ^ 8 (row_ror 8), level 2 with i ^ 4 (row_ror 12 for lanes with
+  // bit 2 clear, row_ror 4 for the others); bank masks (4-lane banks of a row) merge the two halves
+  // of a level, so the lanes keep the same sums in the same order as the xor shuffles of
+  // k_leg_ana_ck.
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const double send = up ? acc[v] : acc[v + 4];
-      const double keep = up ? acc[v + 4] : acc[v];
-      acc[v] = keep + __shfl_xor(send, 8, 64);
-    }
+  for (int v = 0; v < 4; ++v) {
+    const double recv = dpp_merge<0x128, 0x128, 0x3>(acc[v], acc[v + 4]);  // partner's acc[v] / acc[v + 4]
+    const double keep = bank_merge<0x3>(acc[v], acc[v + 4]);
+    acc[v] = keep + recv;
   }
-  {
-    const bool up = (lane & 4) != 0;
 #pragma unroll
-    for (int v = 0; v < 2; ++v) {
-      const double send = up ? acc[v] : acc[v + 2];
-      const double keep = up ? acc[v + 2] : acc[v];
-      acc[v] = keep + __shfl_xor(send, 4, 64);
-    }
+  for (int v = 0; v < 2; ++v) {
+    const double recv = dpp_merge<0x12C, 0x124, 0x5>(acc[v], acc[v + 2]);
+    const double keep = bank_merge<0x5>(acc[v], acc[v + 2]);
+    acc[v] = keep + recv;
   }
   redw[lane] = acc[0];
   redw[64 + lane] = acc[1];
