@@ -51,6 +51,111 @@ static int gauss_legendre(int nlat, std::vector<double>& z_out, std::vector<doub
   return 0;
 }
 
+// Walk layout for chain length T (see slx_walk): tables on the host, start states, reload rows,
+// bands and reload masks from the device seeding walk.
+static int build_walk(slx_sht* h, int T, slx_walk& w) {
+  const int nm = h->nm, nh = h->nh, L = h->lmax;
+#define ALLOC_COPY(ptr, vec)                                                                   \
+  SLX_HIP(hipMalloc(&ptr, vec.size() * sizeof(vec[0])));                                       \
+  SLX_HIP(hipMemcpy(ptr, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice));
+  w.T = T;
+  std::vector<int4> ord(nm), ua, ub;
+  std::vector<double> tqc, tqg;
+  std::vector<int> u0(nm, 0);
+  int roff = 0, npts = 0, qsmax = 0;
+  for (int m = 0; m < nm; ++m) {
+    const int n = L - m + 1;
+    const int W = std::max(1, (n + 4 * T - 1) / (4 * T));
+    int Q = (n + 4 * W - 1) / (4 * W);
+    Q += Q & 1;
+    const int Qp = 8 * ((Q + 7) / 8), nk = (Q + 31) / 32;
+    SLX_CHECK(nk - 1 <= 8, "walk contraction: a chain longer than 288 degrees");
+    ord[m] = make_int4(Q, 4 * W, nk, npts);
+    npts += 4 * W * nk;
+    u0[m] = (int)ua.size();
+    qsmax = std::max(qsmax, Qp + 2);
+    for (int wk = 0; wk < W; ++wk) {
+      ua.push_back(make_int4(m, Q, Qp, (int)tqc.size()));
+      ub.push_back(make_int4(nk, roff, 0, 4 * wk));
+      roff += 4 * (nk - 1);
+      for (int i = 0; i < 4; ++i) {
+        const int c = 4 * wk + i;
+        long double g = 1.0L;
+        for (int t = 0; t < Qp; ++t) {
+          const int l = m + c * Q + t;
+          double cq = 0.0, gg = 0.0;
+          if (t < Q && l <= L) {
+            if (c == 0 && t < 2) {
+              gg = 1.0;  // head: P~_m, P~_{m+1} unscaled
+            } else {
+              const int rs = (c == 0 && t < 32) ? 2 : (t & ~31);
+              if (t == rs) g = 1.0L;
+              const long double lm1 = l - 1, mm = m, ll = l;
+              const long double ep = std::sqrt((lm1 * lm1 - mm * mm) / (4.0L * lm1 * lm1 - 1.0L));
+              const long double e = std::sqrt((ll * ll - mm * mm) / (4.0L * ll * ll - 1.0L));
+              cq = (double)((t == rs) ? ep : ep * ep);
+              g /= e;
+              gg = (double)g;
+            }
+          }
+          tqc.push_back(cq);
+          tqg.push_back(gg);
+        }
+      }
+    }
+  }
+  const int nunit = (int)ua.size();
+  w.nunit = nunit;
+  w.qsmax = qsmax;
+  int *d_u0 = nullptr, *d_umask = nullptr;
+  int4* d_ord = nullptr;
+  ALLOC_COPY(w.ua, ua);
+  ALLOC_COPY(w.ub, ub);
+  ALLOC_COPY(w.qc, tqc);
+  ALLOC_COPY(w.qg, tqg);
+  ALLOC_COPY(d_ord, ord);
+  ALLOC_COPY(d_u0, u0);
+  SLX_HIP(hipMalloc(&w.seed, (size_t)nunit * 4 * nh * sizeof(double2)));
+  SLX_HIP(hipMalloc(&w.rl, (size_t)std::max(roff, 1) * nh * sizeof(double2)));
+  SLX_HIP(hipMalloc(&d_umask, (size_t)nunit * sizeof(int)));
+  SLX_HIP(hipMemset(d_umask, 0, (size_t)nunit * sizeof(int)));
+  {
+    std::vector<int2> band((size_t)nunit * 32, make_int2(nh, 0));  // empty bands [nh, 0)
+    ALLOC_COPY(w.band, band);
+  }
+  double2* pab = nullptr;
+  int* psc = nullptr;
+  SLX_HIP(hipMalloc(&pab, (size_t)npts * nh * sizeof(double2)));
+  SLX_HIP(hipMalloc(&psc, (size_t)npts * nh * sizeof(int)));
+  dim3 grid(nm, (nh + 63) / 64);
+  const QwOrd qo{d_ord, d_u0};
+  k_qw_seed_walk<<<grid, 64>>>(dev_of(h), qo, pab, psc);
+  SLX_LAUNCH_CHECK();
+  k_qw_seed_unscale<<<grid, 64>>>(dev_of(h), qw_of(w, nullptr), qo, pab, psc, w.seed, w.rl, d_umask, w.band);
+  SLX_LAUNCH_CHECK();
+  SLX_HIP(hipDeviceSynchronize());
+  std::vector<int> umask(nunit);
+  SLX_HIP(hipMemcpy(umask.data(), d_umask, (size_t)nunit * sizeof(int), hipMemcpyDeviceToHost));
+  for (int u = 0; u < nunit; ++u) ub[u].z = umask[u];
+  SLX_HIP(hipMemcpy(w.ub, ub.data(), ub.size() * sizeof(int4), hipMemcpyHostToDevice));
+  w.h_ua = ua;
+  SLX_HIP(hipFree(pab));
+  SLX_HIP(hipFree(psc));
+  SLX_HIP(hipFree(d_ord));
+  SLX_HIP(hipFree(d_u0));
+  SLX_HIP(hipFree(d_umask));
+#undef ALLOC_COPY
+  return 0;
+}
+
+static void free_walk(slx_walk* w) {
+  if (!w) return;
+  void* ptrs[] = {w->ua, w->ub, w->qc, w->qg, w->seed, w->rl, w->band};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete w;
+}
+
 extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int max_fields, slx_sht** out) {
   SLX_CHECK(out != nullptr, "slx_sht_create: null handle pointer");
   SLX_CHECK(precision == SLX_PREC_FP64 || precision == SLX_PREC_FP32, "precision must be 0 (FP64) or 1 (FP32 products)");
@@ -252,13 +357,10 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
     for (size_t i = 0; i < gm.size(); ++i) gd[i] = make_int4(gm[i], gs[i], first[gm[i]] + gs[i], h->h_moff[gm[i]]);
     ALLOC_COPY(h->grp_desc, gd);
   }
-  // walk analysis layout (k_leg_ana_qw): order m is cut into 4 W chains of Q steps, W the least
-  // that keeps Q <= T; a unit = 4 chains (one walk) with its recurrence tables (restart at
-  // t = 32 k, chain 0 at t = 2 after its head), start states and reload rows; units are dealt
-  // longest first to the least loaded of the blocks (at most 3 units per block).  By default T is
-  // the longest of (unbounded, 256, 128, 96, 64, 48, 32) that still gives >= 768 units (3 per CU:
-  // longer chains load fewer start states, more units keep more waves resident) and every unit
-  // has a block of its own; SLX_QW_T / SLX_QW_BLOCKS override both.
+  // walk analysis: layout with T the longest of (unbounded, 256, 128, 96, 64, 48, 32) that still
+  // gives >= 768 units (3 per CU: longer chains load fewer start states, more units keep more waves
+  // resident), every unit a block of its own (units are dealt longest first to the least loaded
+  // block, at most 3 per block, when SLX_QW_BLOCKS asks for fewer blocks; SLX_QW_T sets T).
   if (h->ana_mfma) {
     auto units_for = [&](int T) {
       long n_u = 0;
@@ -273,57 +375,14 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
       }
     }
     if (const char* e = std::getenv("SLX_QW_T")) T = std::max(8, std::atoi(e));
-    std::vector<int4> ord(nm), ua, ub;
-    std::vector<int> u0(nm);
-    std::vector<double> tqc, tqg;
-    int roff = 0, npts = 0, qsmax = 0;
-    for (int m = 0; m < nm; ++m) {
-      const int n = L - m + 1;
-      const int W = std::max(1, (n + 4 * T - 1) / (4 * T));
-      int Q = (n + 4 * W - 1) / (4 * W);
-      Q += Q & 1;
-      const int Qp = 8 * ((Q + 7) / 8), nk = (Q + 31) / 32;
-      SLX_CHECK(nk - 1 <= 8, "walk analysis: a chain longer than 288 degrees");
-      ord[m] = make_int4(Q, 4 * W, nk, npts);
-      npts += 4 * W * nk;
-      u0[m] = (int)ua.size();
-      qsmax = std::max(qsmax, Qp + 2);
-      for (int w = 0; w < W; ++w) {
-        ua.push_back(make_int4(m, Q, Qp, (int)tqc.size()));
-        ub.push_back(make_int4(nk, roff, 0, 4 * w));
-        roff += 4 * (nk - 1);
-        for (int i = 0; i < 4; ++i) {
-          const int c = 4 * w + i;
-          long double g = 1.0L;
-          for (int t = 0; t < Qp; ++t) {
-            const int l = m + c * Q + t;
-            double cq = 0.0, gg = 0.0;
-            if (t < Q && l <= L) {
-              if (c == 0 && t < 2) {
-                gg = 1.0;  // head: P~_m, P~_{m+1} unscaled
-              } else {
-                const int rs = (c == 0 && t < 32) ? 2 : (t & ~31);
-                if (t == rs) g = 1.0L;
-                const long double lm1 = l - 1, mm = m, ll = l;
-                const long double ep = std::sqrt((lm1 * lm1 - mm * mm) / (4.0L * lm1 * lm1 - 1.0L));
-                const long double e = std::sqrt((ll * ll - mm * mm) / (4.0L * ll * ll - 1.0L));
-                cq = (double)((t == rs) ? ep : ep * ep);
-                g /= e;
-                gg = (double)g;
-              }
-            }
-            tqc.push_back(cq);
-            tqg.push_back(gg);
-          }
-        }
-      }
-    }
-    const int nunit = (int)ua.size();
-    h->qw_qsmax = qsmax;
+    h->wa = new slx_walk();
+    if (int rc = build_walk(h, T, *h->wa)) return rc;
+    const int nunit = h->wa->nunit;
     int nblk = nunit;
     if (const char* e = std::getenv("SLX_QW_BLOCKS")) nblk = std::atoi(e);
     nblk = std::max(nblk, (nunit + 2) / 3);
     {
+      const std::vector<int4>& ua = h->wa->h_ua;
       std::vector<int> idx(nunit);
       for (int u = 0; u < nunit; ++u) idx[u] = u;
       std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ua[a].z > ua[b].z; });
@@ -346,42 +405,6 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
       h->qw_nblk = used;
       ALLOC_COPY(h->qw_blk, blk);
     }
-    int *d_u0 = nullptr, *d_umask = nullptr;
-    int4* d_ord = nullptr;
-    ALLOC_COPY(h->qw_ua, ua);
-    ALLOC_COPY(h->qw_ub, ub);
-    ALLOC_COPY(h->qw_qc, tqc);
-    ALLOC_COPY(h->qw_qg, tqg);
-    ALLOC_COPY(d_ord, ord);
-    ALLOC_COPY(d_u0, u0);
-    SLX_HIP(hipMalloc(&h->qw_seed, (size_t)nunit * 4 * nh * sizeof(double2)));
-    SLX_HIP(hipMalloc(&h->qw_rl, (size_t)std::max(roff, 1) * nh * sizeof(double2)));
-    SLX_HIP(hipMalloc(&d_umask, (size_t)nunit * sizeof(int)));
-    SLX_HIP(hipMemset(d_umask, 0, (size_t)nunit * sizeof(int)));
-    {
-      std::vector<int2> band((size_t)nunit * 32, make_int2(nh, 0));  // empty bands [nh, 0)
-      ALLOC_COPY(h->qw_band, band);
-    }
-    double2* pab = nullptr;
-    int* psc = nullptr;
-    SLX_HIP(hipMalloc(&pab, (size_t)npts * nh * sizeof(double2)));
-    SLX_HIP(hipMalloc(&psc, (size_t)npts * nh * sizeof(int)));
-    dim3 grid(nm, (nh + 63) / 64);
-    const QwOrd qo{d_ord, d_u0};
-    k_qw_seed_walk<<<grid, 64>>>(dev_of(h), qo, pab, psc);
-    SLX_LAUNCH_CHECK();
-    k_qw_seed_unscale<<<grid, 64>>>(dev_of(h), qw_of(h), qo, pab, psc, h->qw_seed, h->qw_rl, d_umask, h->qw_band);
-    SLX_LAUNCH_CHECK();
-    SLX_HIP(hipDeviceSynchronize());
-    std::vector<int> umask(nunit);
-    SLX_HIP(hipMemcpy(umask.data(), d_umask, (size_t)nunit * sizeof(int), hipMemcpyDeviceToHost));
-    for (int u = 0; u < nunit; ++u) ub[u].z = umask[u];
-    SLX_HIP(hipMemcpy(h->qw_ub, ub.data(), ub.size() * sizeof(int4), hipMemcpyHostToDevice));
-    SLX_HIP(hipFree(pab));
-    SLX_HIP(hipFree(psc));
-    SLX_HIP(hipFree(d_ord));
-    SLX_HIP(hipFree(d_u0));
-    SLX_HIP(hipFree(d_umask));
   }
 #undef ALLOC_COPY
   SLX_HIP(hipMalloc(&h->WS, wsz));
@@ -521,9 +544,10 @@ extern "C" int slx_sht_destroy(slx_sht* h) {
   void* ptrs[] = {h->x, h->wn, h->ws, h->pmm, h->pms, h->rec, h->s2m3, h->moff, h->l_of, h->jb, h->stw,
                   h->WS, h->WD, h->FN, h->FSo, h->seg_m, h->seg_l0, h->seg_first, h->seed_a, h->seed_b,
                   h->seed_s, h->seed_ab, h->m_of, h->grp_m, h->grp_s0, h->CS, h->qc, h->qg, h->grp_desc,
-                  h->qw_blk, h->qw_ua, h->qw_ub, h->qw_qc, h->qw_qg, h->qw_seed, h->qw_rl, h->qw_band};
+                  h->qw_blk};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  free_walk(h->wa);
   delete h;
   return 0;
 }

@@ -131,13 +131,13 @@ int launch_leg_ana_ck(const slx_sht* h, const double2* WS, const double2* WD, Ou
   return 0;
 }
 
-inline QwDev qw_of(const slx_sht* h) {
-  return QwDev{h->qw_blk, h->qw_ua, h->qw_ub, h->qw_qc, h->qw_qg, h->qw_seed, h->qw_rl, h->qw_band};
+inline QwDev qw_of(const slx_walk& w, const int4* blk) {
+  return QwDev{blk, w.ua, w.ub, w.qc, w.qg, w.seed, w.rl, w.band};
 }
 // Walk analysis: one block per entry of the unit-block list, 16 R rings per wave; CHB = 4
 // chunks between reductions when two blocks still fit a CU's LDS, else 2.
 inline size_t qw_lds(const slx_sht* h, int nw, int chb) {
-  return (size_t)(2 * nw * 2 * chb * 64 + 2 * 4 * h->qw_qsmax) * sizeof(double) + 32 * sizeof(int2);
+  return (size_t)(2 * nw * 2 * chb * 64 + 2 * 4 * h->wa->qsmax) * sizeof(double) + 32 * sizeof(int2);
 }
 template <int NF, int R, int CHB>
 int launch_leg_ana_qw_c(const slx_sht* h, const double2* WS, const double2* WD, OutPtrs<NF> out, Pred pr,
@@ -145,7 +145,7 @@ int launch_leg_ana_qw_c(const slx_sht* h, const double2* WS, const double2* WD, 
   const int nw = (h->nh + 16 * R - 1) / (16 * R);
   const size_t lds = qw_lds(h, nw, CHB);
   if (int rc = allow_lds<&k_leg_ana_qw<NF, R, CHB>>(lds)) return rc;
-  k_leg_ana_qw<NF, R, CHB><<<h->qw_nblk, 64 * nw, lds, s>>>(dev_of(h), qw_of(h), WS, WD, out, pr);
+  k_leg_ana_qw<NF, R, CHB><<<h->qw_nblk, 64 * nw, lds, s>>>(dev_of(h), qw_of(*h->wa, h->qw_blk), WS, WD, out, pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }

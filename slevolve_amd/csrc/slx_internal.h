@@ -79,13 +79,9 @@ struct slx_sht {
   // walk analysis (k_leg_ana_qw): orders cut into chains of at most a target length, 4 chains per
   // unit, units packed onto blocks
   int ana_qw = 0;         // the analysis runs the walk kernel (ana_r rings per lane)
-  int qw_nblk = 0, qw_qsmax = 0;
+  struct slx_walk* wa = nullptr;  // its layout
+  int qw_nblk = 0;
   int4* qw_blk = nullptr;   // [qw_nblk] units of a block, -1 padded
-  int4* qw_ua = nullptr;    // [nunit] (m, Q, Qpad, table offset)
-  int4* qw_ub = nullptr;    // [nunit] (sub-segments per chain, reload row offset, reload mask, first chain)
-  double *qw_qc = nullptr, *qw_qg = nullptr;
-  double2 *qw_seed = nullptr, *qw_rl = nullptr;
-  int2* qw_band = nullptr;  // [nunit][4][8] ring band [j0, j1) reloading at each boundary
   int syn_mfma = 0;       // synthesis on the matrix cores (else wave-uniform scalar-fed VALU)
   int syn_r = 4;          // rings per lane of the matrix-core synthesis (2 or 4)
   int *grp_m = nullptr, *grp_s0 = nullptr;
@@ -97,6 +93,19 @@ struct slx_sht {
   std::vector<double> h_x, h_w;  // host copies: all nlat nodes / weights (north -> south)
   std::vector<int> h_moff;
   struct slx_stage_marks* marks = nullptr;  // stage timing of a profiled solver step (else null)
+};
+
+// Walk layout of the chain analysis: order m cut into 4 W chains of Q steps (Q even, W the least
+// with Q <= T); unit = 4 chains of one order with its scaled-recurrence tables (restarted every 32
+// steps), start states, reload rows and ring bands for the extended range.
+struct slx_walk {
+  int T = 0, nunit = 0, qsmax = 0;
+  int4* ua = nullptr;     // [nunit] (m, Q, Qpad, table offset)
+  int4* ub = nullptr;     // [nunit] (sub-segments per chain nk, reload row offset, reload mask, first chain)
+  double *qc = nullptr, *qg = nullptr;  // [table] per (unit, chain, t < Qpad)
+  double2 *seed = nullptr, *rl = nullptr;
+  int2* band = nullptr;   // [nunit][4][8] rings [j0, j1) reloading at boundary k (entry k - 1)
+  std::vector<int4> h_ua;
 };
 
 // ------------------------------------------------------------------ stage timing

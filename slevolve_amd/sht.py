@@ -57,9 +57,10 @@ class SHT:
         self.gaulats, self.weights = x, w
         # kernels the handle chose at creation (a fixed rule per shape unless SLX_ANA_VARIANT /
         # SLX_SYN_VARIANT ask otherwise)
-        self.kernels = {"analysis": {-1: "k_leg_ana", 0: "k_leg_ana_mf", 1: "k_leg_ana_ck", 2: "k_leg_ana_qw"}[int(dims[5])],
-                        "analysis_rings_per_lane": int(dims[6]),
-                        "synthesis": "k_leg_syn_mf" if dims[7] else "k_leg_syn_s", "synthesis_rings_per_lane": int(dims[8])}
+        ana = {-1: "k_leg_ana", 0: "k_leg_ana_mf", 1: "k_leg_ana_ck", 2: "k_leg_ana_qw"}[int(dims[5])]
+        self.kernels = {"analysis": ana, "analysis_rings_per_lane": int(dims[6]),
+                        "synthesis": "k_leg_syn_mf" if dims[7] else "k_leg_syn_s",
+                        "synthesis_rings_per_lane": int(dims[8])}
         self._tri = torch.zeros((self.max_fields, self.ncoef), dtype=torch.complex128, device=self.dev)
         self._coef_out = torch.zeros((self.nm, self.nm), dtype=torch.complex128, device=self.dev)
         self._grid_out = torch.zeros((self.nlat, self.nlon), dtype=torch.float64, device=self.dev)
