@@ -34,16 +34,65 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
 // at N = 4096 (two rows and their ping-pong buffer fill the LDS at that size).
 constexpr int RING_MAXNT = 512;
 
+// FFT twiddles.  stw (ShtDev::stw) holds, for every radix-4 stage with Ns > 1 in stage order,
+// W1[0..Ns), W2[0..Ns), W3[0..Ns) consecutively (fewer than N entries); a butterfly j = tid + r nt
+// of a stage uses W1[k], W2[k], W3[k] with k = j & (Ns - 1), at offset off of its stage.
+//
+// TwReg: the thread's twiddles of every stage held in registers.  A ring pass with 4 points per
+// thread loads them once, together with its other global reads, and every FFT of the pass
+// (forward and inverse) uses them, so no stage waits on a memory round trip.
+template <int P>
+struct TwReg {
+  static constexpr int MAXST = P == 4 ? 5 : 6;  // radix-4 stages of N <= 4 P RING_MAXNT
+  static constexpr int NB = P / 4;
+  double2 w[MAXST][3][NB];
+  __device__ __forceinline__ void load(const double2* __restrict__ stw, int N, int log2N) {
+    const int tid = threadIdx.x, nt = N / P;
+    int Ns = (log2N & 1) ? 2 : 1;
+    const double2* w0 = stw;
+#pragma unroll
+    for (int st = 0; st < MAXST; ++st) {
+      const bool on = Ns > 1 && Ns < N;
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int k = (tid + r * nt) & (Ns - 1);
+        w[st][0][r] = on ? w0[k] : make_double2(1.0, 0.0);
+        w[st][1][r] = on ? w0[Ns + k] : make_double2(1.0, 0.0);
+        w[st][2][r] = on ? w0[2 * Ns + k] : make_double2(1.0, 0.0);
+      }
+      if (on) w0 += 3 * Ns;
+      if (Ns < N) Ns *= 4;
+    }
+  }
+  __device__ __forceinline__ void get(int st, int r, int, int, int, double2& w1, double2& w2, double2& w3) const {
+    w1 = w[st][0][r];
+    w2 = w[st][1][r];
+    w3 = w[st][2][r];
+  }
+};
+// TwMem: the twiddles read from stw at each stage (8 points per thread, where a register copy of
+// every stage would take 144 registers).
+struct TwMem {
+  const double2* stw;
+  __device__ __forceinline__ void load(const double2* __restrict__ t, int, int) { stw = t; }
+  __device__ __forceinline__ void get(int, int, int off, int Ns, int k, double2& w1, double2& w2,
+                                      double2& w3) const {
+    w1 = stw[off + k];
+    w2 = stw[off + Ns + k];
+    w3 = stw[off + 2 * Ns + k];
+  }
+};
+// the twiddle holder of a ring pass with P points per thread (REG = false: per-stage reads)
+template <int P, bool REG = true>
+using TwOf = typename std::conditional<P == 4 && REG, TwReg<4>, TwMem>::type;
+
 // Complex FFT of length N = 2^log2N held in LDS (a), Stockham autosort with ping-pong buffer b:
 // one radix-2 stage first when log2N is odd, then radix-4 stages.  Forward uses
-// exp(-2 pi i k n / N), inverse exp(+...), both unnormalised.  stw is the per-stage twiddle table
-// (ShtDev::stw): for every radix-4 stage with Ns > 1, in stage order, W1[k], W2[k], W3[k] =
-// exp(-2 pi i r k N / (4 Ns) / N) for r = 1, 2, 3 and k < Ns, consecutive over k, so that a wave's
-// twiddle loads are contiguous (the table has fewer than N entries and stays in cache).  Returns
-// the buffer holding the result.  Every thread of the block (nt = N / P of them) must call it.
-template <int P>
-__device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ stw, int N, int log2N,
-                            bool inverse) {
+// exp(-2 pi i k n / N), inverse exp(+...), both unnormalised; tw supplies each stage's twiddles.
+// Returns the buffer holding the result.  Every thread of the block (nt = N / P of them) must
+// call it.
+template <int P, class TW>
+__device__ double2* fft_lds(double2* a, double2* b, const TW& tw, int N, int log2N, bool inverse) {
   const int tid = threadIdx.x, nt = N / P;
   int Ns = 1;
   if (log2N & 1) {
@@ -59,15 +108,18 @@ __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ 
     Ns = 2;
   }
   const int q = N / 4;
-  const double2* w = stw;  // this stage's W1 (W2, W3 follow, Ns apart)
-  for (; Ns < N; Ns *= 4) {
+  int off = 0;  // this stage's W1 in stw
+#pragma unroll
+  for (int st = 0; st < 6; ++st) {  // N <= 4096
+    if (Ns >= N) break;
 #pragma unroll
     for (int r = 0; r < P / 4; ++r) {
       const int j = tid + r * nt;
       const int k = j & (Ns - 1);
       double2 v0 = a[j], v1 = a[j + q], v2 = a[j + 2 * q], v3 = a[j + 3 * q];
       if (Ns > 1) {
-        const double2 w1 = w[k], w2 = w[Ns + k], w3 = w[2 * Ns + k];
+        double2 w1, w2, w3;
+        tw.get(st, r, off, Ns, k, w1, w2, w3);
         if (!inverse) {
           v1 = cmul(v1, w1);
           v2 = cmul(v2, w2);
@@ -94,9 +146,10 @@ __device__ double2* fft_lds(double2* a, double2* b, const double2* __restrict__ 
       b[o + 2 * Ns] = csub(s02, s13);
       b[o + 3 * Ns] = y3;
     }
-    if (Ns > 1) w += 3 * Ns;
+    if (Ns > 1) off += 3 * Ns;
     __syncthreads();
     double2* t = a; a = b; b = t;
+    Ns *= 4;
   }
   return a;
 }
@@ -239,6 +292,8 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_ana(ShtDev d, Loader ld, dou
   double2* b = smem + N;
   const int j = xcd_ring(blockIdx.x, d.nh), f = blockIdx.y, js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
+  TwOf<P> tw;
+  tw.load(d.stw, N, d.log2n);
   double vn[P], vs[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -255,7 +310,7 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_ana(ShtDev d, Loader ld, dou
     }
   }
   __syncthreads();
-  double2* Z = fft_lds<P>(a, b, d.stw, N, d.log2n, false);
+  double2* Z = fft_lds<P>(a, b, tw, N, d.log2n, false);
   ring_fold(Z, WS, WD, (long)f * d.nm * d.nh, d.nh, j, d.lmax, N, d.wn[j], d.ws[j], nt);
 }
 
@@ -494,11 +549,13 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_syn(ShtDev d, const double2*
   double2* a = smem;
   double2* b = smem + N;
   const int j = xcd_ring(blockIdx.x, d.nh), f = blockIdx.y, js = d.nlat - 1 - j;
+  TwOf<P> tw;
+  tw.load(d.stw, N, d.log2n);
   RingCoef<P> rc;
   rc.load(FN, FS, (long)f * d.nm * d.nh, d.nh, j, L, nt);
   rc.scatter(a, N, L, nt);
   __syncthreads();
-  double2* z = fft_lds<P>(a, b, d.stw, N, d.log2n, true);
+  double2* z = fft_lds<P>(a, b, tw, N, d.log2n, true);
   const long rn = (long)j * N, rs = (long)js * N;
 #pragma unroll
   for (int k = 0; k < P; ++k) {

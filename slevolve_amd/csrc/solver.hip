@@ -544,7 +544,10 @@ struct OceanFn {  // C* = C beta and tO = tinit (C* - C*0) of a grid point (solv
   __device__ __forceinline__ double tO(long i, double cs) const { return tinit[i] * (cs - cstar0[i]); }
 };
 
-template <int P>
+// REG: the FFT twiddles in registers (TwReg), where one block per CU is what the shape has anyway
+// (nlon <= 1024); at nlon 2048 their registers would halve the resident blocks (two blocks of 8
+// waves per CU need <= 128 registers), so that shape reads them per stage.
+template <int P, bool REG>
 __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const double2* __restrict__ FN,
                                                            const double2* __restrict__ FS, double* __restrict__ dsl,
                                                            OceanFn oc, double2* __restrict__ WS,
@@ -558,8 +561,10 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const doubl
   double2* b = smem + N;
   const int j = xcd_ring(blockIdx.x, d.nh), js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
-  // every global read of the pass first: the ring pair's Fourier coefficients, then C* and tO of
-  // the thread's points of both rows
+  // every global read of the pass first: the FFT twiddles, the ring pair's Fourier coefficients,
+  // then C* and tO of the thread's points of both rows
+  TwOf<P, REG> tw;
+  tw.load(d.stw, N, d.log2n);
   RingCoef<P> rc;
   rc.load(FN, FS, 0, d.nh, j, L, nt);
   double cn[P], cs[P], tn[P], ts[P];
@@ -573,7 +578,7 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const doubl
   }
   rc.scatter(a, N, L, nt);
   __syncthreads();
-  double2* z = fft_lds<P>(a, b, d.stw, N, d.log2n, true);
+  double2* z = fft_lds<P>(a, b, tw, N, d.log2n, true);
   double2* w = (z == a) ? b : a;
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -592,7 +597,7 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const doubl
       for (int k = 0; k < P; ++k) z[tid + k * nt] = make_double2(cn[k], cs[k]);
       __syncthreads();
     }
-    double2* Z = fft_lds<P>(z, w, d.stw, N, d.log2n, false);
+    double2* Z = fft_lds<P>(z, w, tw, N, d.log2n, false);
     ring_fold(Z, WS, WD, f * fstride, d.nh, j, L, N, wn, ws, nt);
     __syncthreads();
     if (Z != z) {  // keep z as the buffer the next field is loaded into
@@ -773,10 +778,12 @@ __global__ void __launch_bounds__(RING_MAXNT) k_reguess_ring(ShtDev d, LoadRegue
     any |= (dn[k] != 0.0 || dv[k] != 0.0) ? 1 : 0;
   }
   if (!__syncthreads_or(any)) return;
+  TwOf<P> tw;  // only the blocks whose rows changed transform
+  tw.load(d.stw, N, d.log2n);
 #pragma unroll
   for (int k = 0; k < P; ++k) a[tid + k * nt] = make_double2(dn[k], dv[k]);
   __syncthreads();
-  double2* Z = fft_lds<P>(a, b, d.stw, N, d.log2n, false);
+  double2* Z = fft_lds<P>(a, b, tw, N, d.log2n, false);
   ring_fold(Z, WS, WD, 0, d.nh, j, d.lmax, N, d.wn[j], d.ws[j], nt);
   if (tid == 0) ring_tok[j] = token;
 }
@@ -814,10 +821,12 @@ __global__ void __launch_bounds__(RING_MAXNT) k_begin_ring(ShtDev d, LoadIceGrid
     any |= (dn[k] != 0.0 || dv[k] != 0.0) ? 1 : 0;
   }
   if (!__syncthreads_or(any)) return;
+  TwOf<P> tw;  // only the blocks whose rows changed transform
+  tw.load(d.stw, N, d.log2n);
 #pragma unroll
   for (int k = 0; k < P; ++k) a[tid + k * nt] = make_double2(dn[k], dv[k]);
   __syncthreads();
-  double2* Z = fft_lds<P>(a, b, d.stw, N, d.log2n, false);
+  double2* Z = fft_lds<P>(a, b, tw, N, d.log2n, false);
   ring_fold(Z, WS, WD, 0, d.nh, j, d.lmax, N, d.wn[j], d.ws[j], nt);
   if (tid == 0) ring_tok[j] = token;
 }
@@ -1933,19 +1942,21 @@ static int launch_reguess_incremental(slx_solver* s, const LoadReguess& ld, Pred
 }
 
 // ring pass of an inner iteration (k_fft_synana) on the solver's grids
-template <int P>
+template <int P, bool REG>
 static int launch_fft_synana_p(slx_solver* s, int nf, const int* cst, Pred pr, hipStream_t st) {
   const slx_sht* h = s->h;
   const size_t lds = ring_lds(h);
-  if (int rc = allow_lds<&k_fft_synana<P>>(lds)) return rc;
-  k_fft_synana<P><<<h->nh, h->nlon / P, lds, st>>>(dev_of(h), h->FN, h->FSo, s->dsl,
-                                                    OceanFn{s->cxy, s->beta, s->tinit, s->cstar0}, h->WS, h->WD, nf, cst,
-                                                    pr);
+  if (int rc = allow_lds<&k_fft_synana<P, REG>>(lds)) return rc;
+  k_fft_synana<P, REG><<<h->nh, h->nlon / P, lds, st>>>(dev_of(h), h->FN, h->FSo, s->dsl,
+                                                         OceanFn{s->cxy, s->beta, s->tinit, s->cstar0}, h->WS, h->WD,
+                                                         nf, cst, pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }
 static int launch_fft_synana(slx_solver* s, int nf, const int* cst, Pred pr, hipStream_t st) {
-  return ring_points(s->h) == 4 ? launch_fft_synana_p<4>(s, nf, cst, pr, st) : launch_fft_synana_p<8>(s, nf, cst, pr, st);
+  if (ring_points(s->h) != 4) return launch_fft_synana_p<8, false>(s, nf, cst, pr, st);
+  return s->h->nlon <= 1024 ? launch_fft_synana_p<4, true>(s, nf, cst, pr, st)
+                            : launch_fft_synana_p<4, false>(s, nf, cst, pr, st);
 }
 
 static int iterate_anderson(slx_solver* s, int n, hipStream_t st) {
