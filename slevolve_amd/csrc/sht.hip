@@ -487,11 +487,14 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
       h->ana_qw = 0;
     }
     // synthesis kernel: scalar-fed VALU, or matrix cores with 2 or 4 rings per lane (the latter
-    // only with the matrix-core segments).  By default a fixed rule (matrix cores with 2 rings per
-    // lane: its 16 KB of partial sums leave the packed staging area the LDS limit, 5 blocks per CU
-    // up to lmax 1023; scalar without the matrix-core segments);
-    // SLX_SYN_VARIANT = scalar | mfma2 | mfma4 fixes the choice, mfma times the two ring counts and
-    // auto all three.
+    // only with the matrix-core segments).  By default a fixed rule: matrix cores with 2 rings per
+    // lane, the waves of a block over the order pair's segment groups up to 256 ring pairs (its
+    // 16 KB of partial sums leave the packed staging area the LDS limit, 5 blocks per CU) and over
+    // rings above (k_leg_syn_mw: every wave walks all groups, so a pair's 8-10 groups do not leave
+    // waves idle: 108-114 -> 96 us with the ring pass at lmax 1023); scalar without the matrix-core
+    // segments;
+    // SLX_SYN_VARIANT = scalar | mfma2 | mfma4 | mw1 | mw2 (matrix cores with the waves over rings)
+    // fixes the choice, mfma times the two segmented ring counts and auto all five.
     {
       const char* v = std::getenv("SLX_SYN_VARIANT");
       const std::string want = v ? v : "";
@@ -500,9 +503,11 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
       if (want == "scalar" || !h->ana_mfma) cand = {{0, 4}};
       else if (want == "mfma2") cand = {{1, 2}};
       else if (want == "mfma4") cand = {{1, 4}};
+      else if (want == "mw1") cand = {{2, 1}};
+      else if (want == "mw2") cand = {{2, 2}};
       else if (want == "mfma") cand = {{1, 2}, {1, 4}};
-      else if (want == "auto") cand = {{0, 4}, {1, 2}, {1, 4}};
-      else cand = {{1, 2}};
+      else if (want == "auto") cand = {{0, 4}, {1, 2}, {1, 4}, {2, 1}, {2, 2}};
+      else cand = {{nh > 256 ? 2 : 1, 2}};  // the shape rule: waves over rings above 256 ring pairs
       float bs = 1e30f;
       Cand bc = cand[0];
       for (const Cand& c : cand) {
@@ -561,7 +566,8 @@ extern "C" int slx_sht_info(const slx_sht* h, int* dims, double* gaulats, double
     dims[3] = h->ncoef;
     dims[4] = h->max_fields;
     // kernels chosen at creation: analysis (0 one group per block, 1 persistent chunked, 2 quarter
-    // walk; rings per lane), synthesis (0 scalar-fed VALU, 1 matrix cores; rings per lane)
+    // walk; rings per lane), synthesis (0 scalar-fed VALU, 1 matrix cores with the waves over
+    // segment groups, 2 matrix cores with the waves over rings; rings per lane)
     dims[5] = h->ana_mfma ? (h->ana_qw ? 2 : h->ana_ck) : -1;
     dims[6] = h->ana_mfma ? h->ana_r : 0;
     dims[7] = h->syn_mfma;

@@ -1420,4 +1420,118 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   }
 }
 
+
+// Synthesis contraction with the waves over rings: block = (order pair, chunk of 64 R rings), wave w
+// owns rings 16 R w .. 16 R w + 16 R - 1 of the chunk and walks every segment group of the pair
+// (the same degree steps and staged coefficients as k_leg_syn_mf), so the waves carry equal work
+// whatever the pair's group count, and each writes its own rings' FN, FS with no cross-wave sum.
+template <int NF, int R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R <= 2 ? 4 : 1, 8)))
+k_leg_syn_mw(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, double2* __restrict__ FN,
+             double2* __restrict__ FS, Pred pr) {
+  static_assert(NF == 1 || NF == 2, "one or two fields per launch");
+  if (!pr.active()) return;
+  constexpr int S = MF_S;
+  extern __shared__ double2 lsy[];
+  const int L = d.lmax, nl = L + 1, nh = d.nh;
+  // XCD-aware block order: block b runs on XCD b % 8; the ny ring chunks of one order pair go to
+  // the same XCD, so that its L2 fetches the pair's staged coefficients once
+  const int ny = (nh + 64 * R - 1) / (64 * R), bt = blockIdx.x >> 3;
+  const int yb = bt % ny, p = (blockIdx.x & 7) + 8 * (bt / ny);
+  if (p >= (L + 2) / 2) return;
+  const int mord[2] = {p, L - p};
+  const int nord = (L - p == p) ? 1 : 2;
+  const int ng0 = ((nl - mord[0] + S - 1) / S + 3) / 4;
+  const int ng1 = nord == 2 ? ((nl - mord[1] + S - 1) / S + 3) / 4 : 0;
+  const int np2 = 4 * S * (ng0 + ng1), base1 = 4 * S * ng0;
+  const int nsg = np2 / S;
+  double* qcl = reinterpret_cast<double*>(lsy);  // [nsg][MF_QS] scaled recurrence coefficients
+  double2* cl = lsy + nsg * (MF_QS / 2);         // [NF][nsg][MF_CS] coefficients times qg
+  {
+    const int mo0 = d.moff[mord[0]], mo1 = d.moff[mord[1]];
+    const int cnt0 = nl - mord[0], cnt1 = nl - mord[1];
+    for (int i = threadIdx.x; i < np2; i += 256) {
+      const bool second = i >= base1;
+      const int k = second ? i - base1 : i;
+      const bool v = k < (second ? cnt1 : cnt0);
+      const int src = (second ? mo1 : mo0) + k;
+      const int sgi = i / S, tq = i % S;
+      qcl[sgi * MF_QS + tq] = v ? d.qc[src] : 0.0;
+      const double gq = v ? d.qg[src] : 0.0;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double2 c = v ? cf[f * cstride + src] : make_double2(0.0, 0.0);
+        cl[(f * nsg + sgi) * MF_CS + tq] = make_double2(c.x * gq, c.y * gq);
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int blk = (lane >> 2) & 3, ri = lane & 3, slot = lane >> 4;
+  const int col = lane & 3, fb = col >> 1, comp = col & 1;
+  const bool colok = fb < NF;
+  const int jw = yb * 64 * R + wave * 16 * R;  // this wave's first ring
+  int jr[R];
+  bool okr[R];
+  double xr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    jr[r] = jw + r * 16 + 4 * blk + ri;
+    okr[r] = jr[r] < nh;
+    xr[r] = d.x[okr[r] ? jr[r] : 0];
+  }
+  if (jw >= nh) return;  // a wave past the last ring (no barrier follows)
+  const int first0 = g.first[mord[0]], first1 = g.first[mord[1]];
+  const double s23o0 = d.s2m3[mord[0]], s23o1 = d.s2m3[mord[1]];
+  const long fstride = (long)d.nm * nh;
+  const int rowr = lane >> 4;  // ring row of the D element this lane holds
+  for (int o = 0; o < nord; ++o) {
+    const int m = mord[o], ng = o ? ng1 : ng0;
+    const int nsm = (nl - m + S - 1) / S;
+    const double s23 = o ? s23o1 : s23o0;
+    double E[R], O[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) E[r] = O[r] = 0.0;
+    for (int gq = 0; gq < ng; ++gq) {
+      const int sg = 4 * gq + slot;
+      const bool sval = sg < nsm;
+      const bool head = sval && sg == 0;
+      const long sx = (long)((o ? first1 : first0) + (sval ? sg : 0)) * nh;
+      LegState st[R];
+      double aE[R], aO[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int jj = okr[r] ? jr[r] : 0;
+        const double2 ab = (okr[r] && sval) ? g.sab[sx + jj] : make_double2(0.0, 0.0);
+        st[r] = LegState{ab.x, ab.y, 0};
+        aE[r] = aO[r] = 0.0;
+      }
+      const int ob = (o ? base1 : 0) / S + sg;  // staged segment
+      const double* qco = qcl + ob * MF_QS;
+      const double* clo = reinterpret_cast<const double*>(cl + ((colok ? fb : 0) * nsg + ob) * MF_CS) + comp;
+      if (gq == 0)
+        syn_mf_steps<R, true>(aE, aO, st, xr, qco, clo, colok, head, s23);
+      else
+        syn_mf_steps<R, false>(aE, aO, st, xr, qco, clo, colok, head, s23);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        E[r] += aE[r];
+        O[r] += aO[r];
+      }
+    }
+    if (colok) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = jw + r * 16 + 4 * blk + rowr;
+        if (j < nh) {
+          const long idx = fb * fstride + (long)m * nh + j;
+          reinterpret_cast<double*>(FN)[2 * idx + comp] = SLX_SQRT2_LEGACY * (E[r] + O[r]);
+          reinterpret_cast<double*>(FS)[2 * idx + comp] = SLX_SQRT2_LEGACY * (E[r] - O[r]);
+        }
+      }
+    }
+  }
+}
+
 }  // namespace slx

@@ -233,6 +233,19 @@ template <class Store>
 int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
   return ring_points(h) == 4 ? launch_fft_syn_p<4>(h, st, nf, pr, s) : launch_fft_syn_p<8>(h, st, nf, pr, s);
 }
+// Synthesis with the waves over rings: 8 XCD lanes x chunks of 64 R rings x ceil(order pairs / 8)
+// blocks; LDS = the pair's staged tables and coefficients.
+template <int NF, int R>
+int launch_leg_syn_mw(const slx_sht* h, const double2* src, long cstride, long off, Pred pr, hipStream_t s) {
+  const int ny = (h->nh + 64 * R - 1) / (64 * R), np = (h->lmax + 2) / 2;
+  const dim3 grid(8 * ny * ((np + 7) / 8));
+  const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
+  const size_t lds = (size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2));
+  if (int rc = allow_lds<&k_leg_syn_mw<NF, R>>(lds)) return rc;
+  k_leg_syn_mw<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
+  SLX_LAUNCH_CHECK();
+  return 0;
+}
 template <int NF, int R>
 int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long off, Pred pr, hipStream_t s) {
   // 1-D grid: 8 XCD lanes x ring chunks x ceil(order pairs / 8) (blocks past the last pair exit)
@@ -267,6 +280,8 @@ int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field 
   }
   const long off = (long)field * h->nm * h->nh;
   if constexpr (NF <= 2) {
+    if (h->syn_mfma == 2) return h->syn_r == 1 ? launch_leg_syn_mw<NF, 1>(h, src, cstride, off, pr, s)
+                                               : launch_leg_syn_mw<NF, 2>(h, src, cstride, off, pr, s);
     if (h->syn_mfma) return h->syn_r == 2 ? launch_leg_syn_mf<NF, 2>(h, src, cstride, off, pr, s)
                                           : launch_leg_syn_mf<NF, 4>(h, src, cstride, off, pr, s);
   }

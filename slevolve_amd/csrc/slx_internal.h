@@ -82,7 +82,8 @@ struct slx_sht {
   struct slx_walk* wa = nullptr;  // its layout
   int qw_nblk = 0;
   int4* qw_blk = nullptr;   // [qw_nblk] units of a block, -1 padded
-  int syn_mfma = 0;       // synthesis on the matrix cores (else wave-uniform scalar-fed VALU)
+  int syn_mfma = 0;       // synthesis: 0 wave-uniform scalar-fed VALU, 1 matrix cores with the waves over
+                          // segment groups (k_leg_syn_mf), 2 with the waves over rings (k_leg_syn_mw)
   int syn_r = 4;          // rings per lane of the matrix-core synthesis (2 or 4)
   int *grp_m = nullptr, *grp_s0 = nullptr;
   double2* CS = nullptr;  // [max_fields][ncoef] synthesis coefficients assembled by a loader

@@ -115,7 +115,9 @@ def _variant_names(sht):
     """The SLX_ANA_VARIANT / SLX_SYN_VARIANT names of the kernels a handle runs."""
     k = sht.kernels
     ana = {"k_leg_ana_mf": "mf", "k_leg_ana_ck": "ck", "k_leg_ana_qw": "qw"}.get(k["analysis"])
-    syn_v = "scalar" if k["synthesis"] == "k_leg_syn_s" else f"mfma{k['synthesis_rings_per_lane']}"
+    syn_v = {"k_leg_syn_s": "scalar", "k_leg_syn_mf": "mfma", "k_leg_syn_mw": "mw"}[k["synthesis"]]
+    if syn_v != "scalar":
+        syn_v += str(k["synthesis_rings_per_lane"])
     return (f"{ana}{k['analysis_rings_per_lane']}" if ana else None), syn_v
 
 

@@ -59,7 +59,7 @@ class SHT:
         # SLX_SYN_VARIANT ask otherwise)
         ana = {-1: "k_leg_ana", 0: "k_leg_ana_mf", 1: "k_leg_ana_ck", 2: "k_leg_ana_qw"}[int(dims[5])]
         self.kernels = {"analysis": ana, "analysis_rings_per_lane": int(dims[6]),
-                        "synthesis": "k_leg_syn_mf" if dims[7] else "k_leg_syn_s",
+                        "synthesis": {0: "k_leg_syn_s", 1: "k_leg_syn_mf", 2: "k_leg_syn_mw"}[int(dims[7])],
                         "synthesis_rings_per_lane": int(dims[8])}
         self._tri = torch.zeros((self.max_fields, self.ncoef), dtype=torch.complex128, device=self.dev)
         self._coef_out = torch.zeros((self.nm, self.nm), dtype=torch.complex128, device=self.dev)

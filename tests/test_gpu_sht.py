@@ -15,11 +15,12 @@ def rand_coeffs(L, rng):
     return c
 
 
-@pytest.mark.parametrize("variant", ["mfma2", "mfma4", "scalar"])
+@pytest.mark.parametrize("variant", ["mfma2", "mfma4", "mw1", "mw2", "scalar"])
 @pytest.mark.parametrize("nlat,L", [(64, 63), (128, 63), (64, 31), (512, 511), (512, 512), (1024, 1023)])
 def test_synthesis_and_analysis_match_oracle(nlat, L, variant, monkeypatch):
-    """Both synthesis kernels (matrix-core and scalar-fed VALU; SLX_SYN_VARIANT is read when a
-    transform handle is created) and the analysis against the oracle."""
+    """Every synthesis kernel (matrix cores with the waves over segment groups or over rings, and
+    scalar-fed VALU; SLX_SYN_VARIANT is read when a transform handle is created) and the analysis
+    against the oracle."""
     from slevolve_amd import SHT
 
     monkeypatch.setenv("SLX_SYN_VARIANT", variant)
@@ -181,8 +182,7 @@ def test_ring_passes_at_the_row_length_limits(nlat, nlon, L):
 
 def test_kernel_choice_is_a_rule_per_shape(monkeypatch):
     """Without SLX_ANA_VARIANT / SLX_SYN_VARIANT every handle of a shape takes the same kernels
-    (2 rings per lane for the synthesis; for the analysis up to nh = 256), so two handles give
-    bit-identical transforms."""
+    (a fixed rule per shape), so two handles give bit-identical transforms."""
     from slevolve_amd import SHT
 
     monkeypatch.delenv("SLX_ANA_VARIANT", raising=False)
@@ -195,3 +195,9 @@ def test_kernel_choice_is_a_rule_per_shape(monkeypatch):
     assert torch.equal(a.analysis(field), b.analysis(field))
     c = torch.from_numpy(rand_coeffs(127, rng)).cuda()
     assert torch.equal(a.synthesis(c), b.synthesis(c))
+    # the benchmark grids: the walk analysis, and the synthesis with the waves over rings above 256
+    # ring pairs
+    assert SHT(512, 1024, 512).kernels == {"analysis": "k_leg_ana_qw", "analysis_rings_per_lane": 4,
+                                           "synthesis": "k_leg_syn_mf", "synthesis_rings_per_lane": 2}
+    assert SHT(1024, 2048, 1023).kernels == {"analysis": "k_leg_ana_qw", "analysis_rings_per_lane": 4,
+                                             "synthesis": "k_leg_syn_mw", "synthesis_rings_per_lane": 2}
