@@ -488,11 +488,11 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
     }
     // synthesis kernel: scalar-fed VALU, or matrix cores with 2 or 4 rings per lane (the latter
     // only with the matrix-core segments).  By default a fixed rule: matrix cores with 2 rings per
-    // lane, the waves of a block over the order pair's segment groups up to 256 ring pairs (its
-    // 16 KB of partial sums leave the packed staging area the LDS limit, 5 blocks per CU) and over
-    // rings above (k_leg_syn_mw: every wave walks all groups, so a pair's 8-10 groups do not leave
-    // waves idle: 108-114 -> 96 us with the ring pass at lmax 1023); scalar without the matrix-core
-    // segments;
+    // lane, the waves of a block over rings from 256 ring pairs on (k_leg_syn_mw: every wave walks
+    // all of the pair's segment groups, so 5-10 groups per pair leave no wave idle; with the ring
+    // pass 26.4 -> 24.7 us at lmax 512, 111 -> 89-93 us at lmax 1023) and over the pair's segment
+    // groups below (its 16 KB of partial sums leave the packed staging area the LDS limit, 5 blocks
+    // per CU); scalar without the matrix-core segments;
     // SLX_SYN_VARIANT = scalar | mfma2 | mfma4 | mw1 | mw2 (matrix cores with the waves over rings)
     // fixes the choice, mfma times the two segmented ring counts and auto all five.
     {
@@ -507,7 +507,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
       else if (want == "mw2") cand = {{2, 2}};
       else if (want == "mfma") cand = {{1, 2}, {1, 4}};
       else if (want == "auto") cand = {{0, 4}, {1, 2}, {1, 4}, {2, 1}, {2, 2}};
-      else cand = {{nh > 256 ? 2 : 1, 2}};  // the shape rule: waves over rings above 256 ring pairs
+      else cand = {{nh >= 256 ? 2 : 1, 2}};  // the shape rule: waves over rings from 256 ring pairs on
       float bs = 1e30f;
       Cand bc = cand[0];
       for (const Cand& c : cand) {

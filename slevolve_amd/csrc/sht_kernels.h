@@ -1235,7 +1235,7 @@ __global__ void __launch_bounds__(256) k_coef_fill(ShtDev d, Coef cf, double2* _
 template <int R, bool HEADG>
 __device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], LegState (&st)[R],
                                              const double (&xr)[R], const double* qco, const double* clo,
-                                             bool colok, bool head, double s23) {
+                                             bool head, double s23) {
   constexpr int S = MF_S;
   double cc[8], bb[8];
 #pragma unroll
@@ -1244,7 +1244,7 @@ __device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], L
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         cc[u] = qco[t + u];
-        bb[u] = colok ? clo[2 * (t + u)] : 0.0;
+        bb[u] = clo[2 * (t + u)];  // a lane of an unused B column reads the zero row
       }
     }
     double pv[2][R];
@@ -1306,7 +1306,9 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   const int nsg = np2 / S;  // staged segments, each padded to MF_QS / MF_CS entries
   double* qcl = reinterpret_cast<double*>(lsy);  // [nsg][MF_QS] scaled recurrence coefficients
   double2* cl = lsy + nsg * (MF_QS / 2);         // [NF][nsg][MF_CS] coefficients times qg
+  double2* zr = cl + NF * nsg * MF_CS;            // [MF_CS] zeros: the B operand of unused columns
   double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
+  if (threadIdx.x < MF_CS) zr[threadIdx.x] = make_double2(0.0, 0.0);
   {
     const int mo0 = d.moff[mord[0]], mo1 = d.moff[mord[1]];
     const int cnt0 = nl - mord[0], cnt1 = nl - mord[1];
@@ -1370,11 +1372,12 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
     const double s23 = o ? s23o1 : s23o0;
     const int ob = (o ? base1 : 0) / S + sg;  // staged segment
     const double* qco = qcl + ob * MF_QS;
-    const double* clo = reinterpret_cast<const double*>(cl + ((colok ? fb : 0) * nsg + ob) * MF_CS) + comp;
+    const double* clo = colok ? reinterpret_cast<const double*>(cl + (fb * nsg + ob) * MF_CS) + comp
+                               : reinterpret_cast<const double*>(zr);
     if (gq == 0)
-      syn_mf_steps<R, true>(aE, aO, st, xr, qco, clo, colok, head, s23);
+      syn_mf_steps<R, true>(aE, aO, st, xr, qco, clo, head, s23);
     else
-      syn_mf_steps<R, false>(aE, aO, st, xr, qco, clo, colok, head, s23);
+      syn_mf_steps<R, false>(aE, aO, st, xr, qco, clo, head, s23);
     if (o == 0) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1447,6 +1450,8 @@ k_leg_syn_mw(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   const int nsg = np2 / S;
   double* qcl = reinterpret_cast<double*>(lsy);  // [nsg][MF_QS] scaled recurrence coefficients
   double2* cl = lsy + nsg * (MF_QS / 2);         // [NF][nsg][MF_CS] coefficients times qg
+  double2* zr = cl + NF * nsg * MF_CS;            // [MF_CS] zeros: the B operand of unused columns
+  if (threadIdx.x < MF_CS) zr[threadIdx.x] = make_double2(0.0, 0.0);
   {
     const int mo0 = d.moff[mord[0]], mo1 = d.moff[mord[1]];
     const int cnt0 = nl - mord[0], cnt1 = nl - mord[1];
@@ -1509,11 +1514,12 @@ k_leg_syn_mw(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
       }
       const int ob = (o ? base1 : 0) / S + sg;  // staged segment
       const double* qco = qcl + ob * MF_QS;
-      const double* clo = reinterpret_cast<const double*>(cl + ((colok ? fb : 0) * nsg + ob) * MF_CS) + comp;
+      const double* clo = colok ? reinterpret_cast<const double*>(cl + (fb * nsg + ob) * MF_CS) + comp
+                                 : reinterpret_cast<const double*>(zr);
       if (gq == 0)
-        syn_mf_steps<R, true>(aE, aO, st, xr, qco, clo, colok, head, s23);
+        syn_mf_steps<R, true>(aE, aO, st, xr, qco, clo, head, s23);
       else
-        syn_mf_steps<R, false>(aE, aO, st, xr, qco, clo, colok, head, s23);
+        syn_mf_steps<R, false>(aE, aO, st, xr, qco, clo, head, s23);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         E[r] += aE[r];

@@ -240,7 +240,8 @@ int launch_leg_syn_mw(const slx_sht* h, const double2* src, long cstride, long o
   const int ny = (h->nh + 64 * R - 1) / (64 * R), np = (h->lmax + 2) / 2;
   const dim3 grid(8 * ny * ((np + 7) / 8));
   const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
-  const size_t lds = (size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2));
+  const size_t lds = (size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2)) +
+                     MF_CS * sizeof(double2);  // + the zero row
   if (int rc = allow_lds<&k_leg_syn_mw<NF, R>>(lds)) return rc;
   k_leg_syn_mw<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();
@@ -254,7 +255,8 @@ int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long o
   // staged entries of an order pair: ceil(n0 / 4S) + ceil(n1 / 4S) groups of 4S degrees with
   // n0 + n1 = L + 2, at most ceil((L + 2) / 4S) + 1 groups
   const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
-  const size_t lds = std::max((size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2)),
+  const size_t lds = std::max((size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2)) +
+                                  MF_CS * sizeof(double2),  // + the zero row
                               (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
   if (int rc = allow_lds<&k_leg_syn_mf<NF, R>>(lds)) return rc;
   k_leg_syn_mf<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);

@@ -195,9 +195,9 @@ def test_kernel_choice_is_a_rule_per_shape(monkeypatch):
     assert torch.equal(a.analysis(field), b.analysis(field))
     c = torch.from_numpy(rand_coeffs(127, rng)).cuda()
     assert torch.equal(a.synthesis(c), b.synthesis(c))
-    # the benchmark grids: the walk analysis, and the synthesis with the waves over rings above 256
-    # ring pairs
+    # the benchmark grids: the walk analysis, and the synthesis with the waves over rings from 256
+    # ring pairs on
     assert SHT(512, 1024, 512).kernels == {"analysis": "k_leg_ana_qw", "analysis_rings_per_lane": 4,
-                                           "synthesis": "k_leg_syn_mf", "synthesis_rings_per_lane": 2}
+                                           "synthesis": "k_leg_syn_mw", "synthesis_rings_per_lane": 2}
     assert SHT(1024, 2048, 1023).kernels == {"analysis": "k_leg_ana_qw", "analysis_rings_per_lane": 4,
                                              "synthesis": "k_leg_syn_mw", "synthesis_rings_per_lane": 2}
