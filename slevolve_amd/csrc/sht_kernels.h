@@ -1280,12 +1280,59 @@ __device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], L
   }
 }
 
+// Staging of an order pair for the synthesis contractions: the scaled-recurrence coefficients qc
+// and the spectral coefficients times qg of order mord[0]'s groups of 4 S degrees, then order
+// mord[1]'s from entry base1 on, zero past each order's last degree.  The coefficients come from
+// the loader cf: CoefPlain reads an assembled array; an assembling loader (eq. B18, the adjoint's
+// scaled coefficients) forms them here, after its block prologue where its index ranges need one
+// (block-uniform), so that no separate fill kernel runs.  Four entries per thread are loaded
+// before any is stored, so that their loads are in flight together.
+template <int NF, class Coef>
+__device__ __forceinline__ void syn_stage(const ShtDev& d, const Coef& cf, const int (&mord)[2], int nord, int np2,
+                                          int base1, int nsg, double* __restrict__ qcl, double2* __restrict__ cl) {
+  constexpr int S = MF_S, U = 4;
+  __shared__ double pro[32];
+  const int nl = d.lmax + 1;
+  const int mo0 = d.moff[mord[0]], mo1 = d.moff[mord[1]];
+  const int cnt0 = nl - mord[0], cnt1 = nl - mord[1];
+  if (cf.prologue_needed(mo0, mo0 + cnt0) || (nord == 2 && cf.prologue_needed(mo1, mo1 + cnt1))) {
+    cf.prologue(pro);
+    __syncthreads();
+  }
+  for (int i0 = threadIdx.x; i0 < np2; i0 += 256 * U) {
+    double qv[U], gv[U];
+    double2 cv[U][NF];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + 256 * u;
+      const bool second = i >= base1;
+      const int k = second ? i - base1 : i;
+      const bool v = i < np2 && k < (second ? cnt1 : cnt0);
+      const int m = second ? mord[1] : mord[0];
+      const int src = (second ? mo1 : mo0) + k;
+      qv[u] = v ? d.qc[src] : 0.0;
+      gv[u] = v ? d.qg[src] : 0.0;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) cv[u][f] = v ? cf(f, src, m + k, m, pro) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + 256 * u;
+      if (i < np2) {
+        const int sgi = i / S, tq = i % S;
+        qcl[sgi * MF_QS + tq] = qv[u];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) cl[(f * nsg + sgi) * MF_CS + tq] = make_double2(cv[u][f].x * gv[u], cv[u][f].y * gv[u]);
+      }
+    }
+  }
+}
+
 // One field with 2 rings per lane is compiled for 5 waves per SIMD (91 VGPRs, no spills; the LDS of
 // 5 blocks fits a CU): 11 % faster than at the 4 waves its natural 98 registers allow.
-template <int NF, int R>
+template <int NF, int R, class Coef>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NF == 1 && R == 2) ? 5 : 1, 8)))
-k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, double2* __restrict__ FN,
-             double2* __restrict__ FS, Pred pr) {
+k_leg_syn_mf(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __restrict__ FS, Pred pr) {
   static_assert(NF == 1 || NF == 2, "one or two fields per launch");
   if (!pr.active()) return;
   constexpr int S = MF_S;
@@ -1309,24 +1356,7 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   double2* zr = cl + NF * nsg * MF_CS;            // [MF_CS] zeros: the B operand of unused columns
   double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
   if (threadIdx.x < MF_CS) zr[threadIdx.x] = make_double2(0.0, 0.0);
-  {
-    const int mo0 = d.moff[mord[0]], mo1 = d.moff[mord[1]];
-    const int cnt0 = nl - mord[0], cnt1 = nl - mord[1];
-    for (int i = threadIdx.x; i < np2; i += 256) {
-      const bool second = i >= base1;
-      const int k = second ? i - base1 : i;
-      const bool v = k < (second ? cnt1 : cnt0);
-      const int src = (second ? mo1 : mo0) + k;
-      const int sgi = i / S, tq = i % S;
-      qcl[sgi * MF_QS + tq] = v ? d.qc[src] : 0.0;
-      const double gq = v ? d.qg[src] : 0.0;
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const double2 c = v ? cf[f * cstride + src] : make_double2(0.0, 0.0);
-        cl[(f * nsg + sgi) * MF_CS + tq] = make_double2(c.x * gq, c.y * gq);
-      }
-    }
-  }
+  syn_stage<NF>(d, cf, mord, nord, np2, base1, nsg, qcl, cl);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1428,10 +1458,9 @@ k_leg_syn_mf(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
 // owns rings 16 R w .. 16 R w + 16 R - 1 of the chunk and walks every segment group of the pair
 // (the same degree steps and staged coefficients as k_leg_syn_mf), so the waves carry equal work
 // whatever the pair's group count, and each writes its own rings' FN, FS with no cross-wave sum.
-template <int NF, int R>
+template <int NF, int R, class Coef>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R <= 2 ? 4 : 1, 8)))
-k_leg_syn_mw(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, double2* __restrict__ FN,
-             double2* __restrict__ FS, Pred pr) {
+k_leg_syn_mw(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __restrict__ FS, Pred pr) {
   static_assert(NF == 1 || NF == 2, "one or two fields per launch");
   if (!pr.active()) return;
   constexpr int S = MF_S;
@@ -1451,26 +1480,6 @@ k_leg_syn_mw(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
   double* qcl = reinterpret_cast<double*>(lsy);  // [nsg][MF_QS] scaled recurrence coefficients
   double2* cl = lsy + nsg * (MF_QS / 2);         // [NF][nsg][MF_CS] coefficients times qg
   double2* zr = cl + NF * nsg * MF_CS;            // [MF_CS] zeros: the B operand of unused columns
-  if (threadIdx.x < MF_CS) zr[threadIdx.x] = make_double2(0.0, 0.0);
-  {
-    const int mo0 = d.moff[mord[0]], mo1 = d.moff[mord[1]];
-    const int cnt0 = nl - mord[0], cnt1 = nl - mord[1];
-    for (int i = threadIdx.x; i < np2; i += 256) {
-      const bool second = i >= base1;
-      const int k = second ? i - base1 : i;
-      const bool v = k < (second ? cnt1 : cnt0);
-      const int src = (second ? mo1 : mo0) + k;
-      const int sgi = i / S, tq = i % S;
-      qcl[sgi * MF_QS + tq] = v ? d.qc[src] : 0.0;
-      const double gq = v ? d.qg[src] : 0.0;
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const double2 c = v ? cf[f * cstride + src] : make_double2(0.0, 0.0);
-        cl[(f * nsg + sgi) * MF_CS + tq] = make_double2(c.x * gq, c.y * gq);
-      }
-    }
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int blk = (lane >> 2) & 3, ri = lane & 3, slot = lane >> 4;
@@ -1486,8 +1495,25 @@ k_leg_syn_mw(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
     okr[r] = jr[r] < nh;
     xr[r] = d.x[okr[r] ? jr[r] : 0];
   }
-  if (jw >= nh) return;  // a wave past the last ring (no barrier follows)
   const int first0 = g.first[mord[0]], first1 = g.first[mord[1]];
+  // start states of segment group gq of order o (zero for lanes past the last ring or the order's
+  // last segment); the next group's are loaded while the current group's degree steps run, the
+  // first group's while the coefficients are staged
+  const int nsm0 = (nl - mord[0] + S - 1) / S, nsm1 = (nl - mord[1] + S - 1) / S;
+  const double2* __restrict__ sab = g.sab;
+  auto seeds = [=](int o, int gq, double2 (&ab)[R]) {
+    const int sg = 4 * gq + slot;
+    const bool sval = sg < (o ? nsm1 : nsm0);
+    const long sx = (long)((o ? first1 : first0) + (sval ? sg : 0)) * nh;
+#pragma unroll
+    for (int r = 0; r < R; ++r) ab[r] = (okr[r] && sval) ? sab[sx + (okr[r] ? jr[r] : 0)] : make_double2(0.0, 0.0);
+  };
+  double2 nab[R];
+  seeds(0, 0, nab);
+  if (threadIdx.x < MF_CS) zr[threadIdx.x] = make_double2(0.0, 0.0);
+  syn_stage<NF>(d, cf, mord, nord, np2, base1, nsg, qcl, cl);
+  __syncthreads();
+  if (jw >= nh) return;  // a wave past the last ring (no barrier follows)
   const double s23o0 = d.s2m3[mord[0]], s23o1 = d.s2m3[mord[1]];
   const long fstride = (long)d.nm * nh;
   const int rowr = lane >> 4;  // ring row of the D element this lane holds
@@ -1502,16 +1528,17 @@ k_leg_syn_mw(ShtDev d, SegDev g, const double2* __restrict__ cf, long cstride, d
       const int sg = 4 * gq + slot;
       const bool sval = sg < nsm;
       const bool head = sval && sg == 0;
-      const long sx = (long)((o ? first1 : first0) + (sval ? sg : 0)) * nh;
       LegState st[R];
       double aE[R], aO[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int jj = okr[r] ? jr[r] : 0;
-        const double2 ab = (okr[r] && sval) ? g.sab[sx + jj] : make_double2(0.0, 0.0);
-        st[r] = LegState{ab.x, ab.y, 0};
+        st[r] = LegState{nab[r].x, nab[r].y, 0};
         aE[r] = aO[r] = 0.0;
       }
+      if (gq + 1 < ng)
+        seeds(o, gq + 1, nab);
+      else if (o + 1 < nord)
+        seeds(o + 1, 0, nab);
       const int ob = (o ? base1 : 0) / S + sg;  // staged segment
       const double* qco = qcl + ob * MF_QS;
       const double* clo = colok ? reinterpret_cast<const double*>(cl + (fb * nsg + ob) * MF_CS) + comp

@@ -235,20 +235,20 @@ int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
 }
 // Synthesis with the waves over rings: 8 XCD lanes x chunks of 64 R rings x ceil(order pairs / 8)
 // blocks; LDS = the pair's staged tables and coefficients.
-template <int NF, int R>
-int launch_leg_syn_mw(const slx_sht* h, const double2* src, long cstride, long off, Pred pr, hipStream_t s) {
+template <int NF, int R, class Coef>
+int launch_leg_syn_mw(const slx_sht* h, Coef cf, long off, Pred pr, hipStream_t s) {
   const int ny = (h->nh + 64 * R - 1) / (64 * R), np = (h->lmax + 2) / 2;
   const dim3 grid(8 * ny * ((np + 7) / 8));
   const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
   const size_t lds = (size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2)) +
                      MF_CS * sizeof(double2);  // + the zero row
-  if (int rc = allow_lds<&k_leg_syn_mw<NF, R>>(lds)) return rc;
-  k_leg_syn_mw<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
+  if (int rc = allow_lds<&k_leg_syn_mw<NF, R, Coef>>(lds)) return rc;
+  k_leg_syn_mw<NF, R, Coef><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), cf, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }
-template <int NF, int R>
-int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long off, Pred pr, hipStream_t s) {
+template <int NF, int R, class Coef>
+int launch_leg_syn_mf(const slx_sht* h, Coef cf, long off, Pred pr, hipStream_t s) {
   // 1-D grid: 8 XCD lanes x ring chunks x ceil(order pairs / 8) (blocks past the last pair exit)
   const int ny = (h->nh + 16 * R - 1) / (16 * R), np = (h->lmax + 2) / 2;
   const dim3 grid(8 * ny * ((np + 7) / 8));
@@ -258,16 +258,25 @@ int launch_leg_syn_mf(const slx_sht* h, const double2* src, long cstride, long o
   const size_t lds = std::max((size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2)) +
                                   MF_CS * sizeof(double2),  // + the zero row
                               (size_t)4 * 2 * 2 * R * 64 * sizeof(double));
-  if (int rc = allow_lds<&k_leg_syn_mf<NF, R>>(lds)) return rc;
-  k_leg_syn_mf<NF, R><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), src, cstride, h->FN + off, h->FSo + off, pr);
+  if (int rc = allow_lds<&k_leg_syn_mf<NF, R, Coef>>(lds)) return rc;
+  k_leg_syn_mf<NF, R, Coef><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), cf, h->FN + off, h->FSo + off, pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }
 
-// Synthesis contraction of NF fields (FN/FS fields field .. field + NF - 1).  A CoefPlain
-// source is read in place; any other loader is first assembled into the handle's CS array.
+// Synthesis contraction of NF fields (FN/FS fields field .. field + NF - 1).  The matrix-core
+// kernels take the coefficient loader itself and form the coefficients while they stage them (a
+// CoefPlain source is read in place); the scalar-fed kernel reads an array, so any other loader is
+// first assembled into the handle's CS array for it.
 template <int NF, class Coef>
 int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field = 0) {
+  const long off = (long)field * h->nm * h->nh;
+  if constexpr (NF <= 2) {
+    if (h->syn_mfma == 2) return h->syn_r == 1 ? launch_leg_syn_mw<NF, 1>(h, cf, off, pr, s)
+                                               : launch_leg_syn_mw<NF, 2>(h, cf, off, pr, s);
+    if (h->syn_mfma) return h->syn_r == 2 ? launch_leg_syn_mf<NF, 2>(h, cf, off, pr, s)
+                                          : launch_leg_syn_mf<NF, 4>(h, cf, off, pr, s);
+  }
   const double2* src;
   long cstride;
   if constexpr (std::is_same<Coef, CoefPlain>::value) {
@@ -279,13 +288,6 @@ int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field 
     slx_mark(h, SLX_STAGE_SPEC, s);
     src = h->CS;
     cstride = h->ncoef;
-  }
-  const long off = (long)field * h->nm * h->nh;
-  if constexpr (NF <= 2) {
-    if (h->syn_mfma == 2) return h->syn_r == 1 ? launch_leg_syn_mw<NF, 1>(h, src, cstride, off, pr, s)
-                                               : launch_leg_syn_mw<NF, 2>(h, src, cstride, off, pr, s);
-    if (h->syn_mfma) return h->syn_r == 2 ? launch_leg_syn_mf<NF, 2>(h, src, cstride, off, pr, s)
-                                          : launch_leg_syn_mf<NF, 4>(h, src, cstride, off, pr, s);
   }
   dim3 grid((h->lmax + 2) / 2, (h->nh + 63) / 64);
   k_leg_syn_s<NF, 1><<<grid, 256, 0, s>>>(dev_of(h), seg_of(h), h->rec, src, cstride, h->FN + off, h->FSo + off, pr);
