@@ -946,6 +946,30 @@ k_leg_ana_ck(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, cons
   }
 }
 
+// Ring chunk ch (16 R listed rings) of the sparse analysis: ring values, start states of this
+// lane's segment (zero past the order's last segment), folded inputs of field 0.
+template <int R>
+__device__ __forceinline__ void sp_chunk_loads(const ShtDev& d, const SegDev& g, const double2* __restrict__ WS,
+                                               const double2* __restrict__ WD, const int* list, int nact, int ch,
+                                               long sx, bool sval, int m, int lane, double (&xr)[R],
+                                               LegState (&st)[R], double (&wv)[2][R]) {
+  const int blk = (lane >> 2) & 3, slot = lane & 3, kk = lane >> 4;
+  const int nh = d.nh;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int jl = ch * 16 * R + r * 16 + 4 * blk + kk;
+    const bool ok = jl < nact;
+    const int jj = ok ? list[jl] : 0;
+    xr[r] = d.x[jj];
+    const double2 ab = (ok && sval) ? g.sab[sx + jj] : make_double2(0.0, 0.0);
+    st[r] = LegState{ab.x, ab.y, 0};
+    const double2 vs = WS[(long)m * nh + jj], vd = WD[(long)m * nh + jj];
+    const bool use = ok && slot < 2;
+    wv[0][r] = use ? ((slot & 1) ? vs.y : vs.x) : 0.0;
+    wv[1][r] = use ? ((slot & 1) ? vd.y : vd.x) : 0.0;
+  }
+}
+
 // Incremental analysis of one field over a sparse set of rings: out += the analysis of the folded
 // spectra in WS/WD field 0 restricted to the rings whose ring_tok equals token (a grounded re-guess
 // changes I* at a few grid points; its ring pass writes those rings' spectra of the change).  One
@@ -953,9 +977,9 @@ k_leg_ana_ck(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, cons
 // rings (ascending) are walked 16 R at a time, their partial sums added in that order, and each
 // coefficient is updated by the one lane that emits it.  Same degree steps as k_leg_ana_ck.
 template <int R>
-__global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS,
-                                                  const double2* __restrict__ WD, double2* __restrict__ out,
-                                                  const int* __restrict__ ring_tok, int token, Pred pr) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, const double2* __restrict__ WD,
+             double2* __restrict__ out, const int* __restrict__ ring_tok, int token, Pred pr) {
   if (!pr.active()) return;
   constexpr int S = MF_S;
   __shared__ double redw[8 * 64];  // the wave's partials of one ring chunk
@@ -979,44 +1003,52 @@ __global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr
   const int per = (gr.ngrp + 7) >> 3;
   const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
   const int gend = min(gr.ngrp, (x + 1) * per);
-  const int blk = (lane >> 2) & 3, slot = lane & 3, kk = lane >> 4;
-  const bool colok = slot < 2;  // B columns of field 0: (re, im)
+  const int slot = lane & 3;
   const int nchunk = (nact + 16 * R - 1) / (16 * R);
-  for (int gi = x * per + (blockIdx.x >> 3); gi < gend; gi += nbx) {
-    const int4 ds = gr.desc[gi];
-    const int m = ds.x, s0 = ds.y, mo = ds.w;
+  const int row = lane >> 4, cj = lane & 3;
+  int gi = x * per + (blockIdx.x >> 3);
+  if (gi >= gend) return;
+  int4 ds = gr.desc[gi];
+  int m = ds.x, s0 = ds.y, sz = ds.z, mo = ds.w;  // carried as scalars (an int4 went to the stack)
+  for (; gi < gend; gi += nbx) {
     const int nsm = (L - m + S) / S;
-    for (int i = lane; i < 4 * S; i += 64) {
-      const int off = (s0 + i / S) * S + i % S;
-      const bool use = off <= L - m;
-      qcs[(i / S) * MF_QS + i % S] = use ? d.qc[mo + off] : 0.0;
-      qgs[(i / S) * MF_QS + i % S] = use ? d.qg[mo + off] : 0.0;
-    }
+    // everything this group reads is requested before the first wait: the tables, the first ring
+    // chunk's values, the coefficients it adds to (no other block writes them) and the next
+    // group's descriptor
+    static_assert(4 * S == 128, "two table entries per lane");
+    const int offa = s0 * S + lane, offb = offa + 64;  // entries lane and lane + 64 of [4][S]
+    const double tqa = offa <= L - m ? d.qc[mo + offa] : 0.0, tga = offa <= L - m ? d.qg[mo + offa] : 0.0;
+    const double tqb = offb <= L - m ? d.qc[mo + offb] : 0.0, tgb = offb <= L - m ? d.qg[mo + offb] : 0.0;
     const int segi = s0 + slot;
     const bool sval = segi < nsm;
     const bool head = sval && segi == 0;
-    const long sx = (long)(ds.z + (sval ? slot : 0)) * nh;
+    const long sx = (long)(sz + (sval ? slot : 0)) * nh;
+    double xr[R], wv[2][R];
+    LegState st[R];
+    sp_chunk_loads<R>(d, g, WS, WD, list, nact, 0, sx, sval, m, lane, xr, st, wv);
+    // coefficient of emitted value v of this lane: l - m = (s0 + row) S + t_v, written when it exists
+    const int t0l = 4 * ((lane >> 3) & 1) + 2 * ((lane >> 2) & 1);
+    const int offl = (s0 + row) * S + t0l;
+    const bool rowok = (s0 + row) < nsm && cj < 2;
+    double* const ob = reinterpret_cast<double*>(out) + 2 * (long)(mo + offl) + cj;
+    double ov[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int dt = 8 * (v >> 1) + (v & 1);
+      ov[v] = (rowok && offl + dt <= L - m) ? ob[2 * dt] : 0.0;
+    }
+    const int4 dn = gr.desc[gi + nbx < gend ? gi + nbx : gi];
     const double s23 = d.s2m3[m];
+    qcs[(lane / S) * MF_QS + lane % S] = tqa;
+    qgs[(lane / S) * MF_QS + lane % S] = tga;
+    qcs[(lane / S + 2) * MF_QS + lane % S] = tqb;
+    qgs[(lane / S + 2) * MF_QS + lane % S] = tgb;
     __syncthreads();
     double accv[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) accv[v] = 0.0;
     for (int ch = 0; ch < nchunk; ++ch) {
-      double xr[R], wv[2][R];
-      LegState st[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int jl = ch * 16 * R + r * 16 + 4 * blk + kk;
-        const bool ok = jl < nact;
-        const int jj = ok ? list[jl] : 0;
-        xr[r] = d.x[jj];
-        const double2 ab = (ok && sval) ? g.sab[sx + jj] : make_double2(0.0, 0.0);
-        st[r] = LegState{ab.x, ab.y, 0};
-        const double2 vs = WS[(long)m * nh + jj], vd = WD[(long)m * nh + jj];
-        const bool use = ok && colok;
-        wv[0][r] = use ? ((slot & 1) ? vs.y : vs.x) : 0.0;
-        wv[1][r] = use ? ((slot & 1) ? vd.y : vd.x) : 0.0;
-      }
+      if (ch > 0) sp_chunk_loads<R>(d, g, WS, WD, list, nact, ch, sx, sval, m, lane, xr, st, wv);
       if (s0 == 0)
         ana_ck_steps<R, true>(st, xr, wv, qcs, slot, head, s23, redw, lane);
       else
@@ -1026,15 +1058,14 @@ __global__ void __launch_bounds__(64) k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr
     }
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-      const int t = 8 * (v >> 1) + 4 * ((lane >> 3) & 1) + 2 * ((lane >> 2) & 1) + (v & 1);
-      const int row = lane >> 4, cj = lane & 3;
-      const int off = (s0 + row) * S + t;
-      if ((s0 + row) < nsm && off <= L - m && cj < 2) {
-        double* o = reinterpret_cast<double*>(out) + 2 * (long)(mo + off) + cj;
-        *o = *o + (accv[v] * qgs[row * MF_QS + t]) / SLX_SQRT2_LEGACY;
-      }
+      const int dt = 8 * (v >> 1) + (v & 1);
+      if (rowok && offl + dt <= L - m) ob[2 * dt] = ov[v] + (accv[v] * qgs[row * MF_QS + t0l + dt]) / SLX_SQRT2_LEGACY;
     }
     __syncthreads();  // the tables are refilled by the next group
+    m = dn.x;
+    s0 = dn.y;
+    sz = dn.z;
+    mo = dn.w;
   }
 }
 
