@@ -490,30 +490,36 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
     // only with the matrix-core segments).  By default a fixed rule: matrix cores with 2 rings per
     // lane, the waves of a block over rings from 256 ring pairs on (k_leg_syn_mw: every wave walks
     // all of the pair's segment groups, so 5-10 groups per pair leave no wave idle; with the ring
-    // pass 26.4 -> 24.7 us at lmax 512, 111 -> 89-93 us at lmax 1023) and over the pair's segment
-    // groups below (its 16 KB of partial sums leave the packed staging area the LDS limit, 5 blocks
-    // per CU); scalar without the matrix-core segments;
-    // SLX_SYN_VARIANT = scalar | mfma2 | mfma4 | mw1 | mw2 (matrix cores with the waves over rings)
-    // fixes the choice, mfma times the two segmented ring counts and auto all five.
+    // pass 26.4 -> 24.7 us at lmax 512, 111 -> 89-93 us at lmax 1023), in blocks of 8 waves from
+    // 512 ring pairs on (half the blocks stage each order pair's coefficients: 573.6 -> 582.7
+    // steps/s at lmax 1023; 1 818 -> 1 768 at lmax 512, kept at 4 waves there), and over the pair's
+    // segment groups below (its 16 KB of partial sums leave the packed staging area the LDS limit,
+    // 5 blocks per CU); scalar without the matrix-core segments;
+    // SLX_SYN_VARIANT = scalar | mfma2 | mfma4 | mw1 | mw2 | mw1x8 | mw2x8 (matrix cores with the
+    // waves over rings, 4 or 8 waves per block) fixes the choice, mfma times the two segmented ring
+    // counts and auto all seven.
     {
       const char* v = std::getenv("SLX_SYN_VARIANT");
       const std::string want = v ? v : "";
-      struct Cand { int mf, r; };
+      struct Cand { int mf, r, nw = 4; };
       std::vector<Cand> cand;
       if (want == "scalar" || !h->ana_mfma) cand = {{0, 4}};
       else if (want == "mfma2") cand = {{1, 2}};
       else if (want == "mfma4") cand = {{1, 4}};
       else if (want == "mw1") cand = {{2, 1}};
       else if (want == "mw2") cand = {{2, 2}};
+      else if (want == "mw1x8") cand = {{2, 1, 8}};
+      else if (want == "mw2x8") cand = {{2, 2, 8}};
       else if (want == "mfma") cand = {{1, 2}, {1, 4}};
-      else if (want == "auto") cand = {{0, 4}, {1, 2}, {1, 4}, {2, 1}, {2, 2}};
-      else cand = {{nh >= 256 ? 2 : 1, 2}};  // the shape rule: waves over rings from 256 ring pairs on
+      else if (want == "auto") cand = {{0, 4}, {1, 2}, {1, 4}, {2, 1}, {2, 2}, {2, 1, 8}, {2, 2, 8}};
+      else cand = {{nh >= 256 ? 2 : 1, 2, nh >= 512 ? 8 : 4}};  // the shape rule
       float bs = 1e30f;
       Cand bc = cand[0];
       for (const Cand& c : cand) {
         if (cand.size() == 1) break;
         h->syn_mfma = c.mf;
         h->syn_r = c.r;
+        h->syn_nw = c.nw;
         int rc = launch_leg_syn<1>(h, CoefPlain{tmp, 0}, Pred{}, nullptr);
         if (rc) return rc;
         SLX_HIP(hipEventRecord(e0, nullptr));
@@ -532,6 +538,7 @@ extern "C" int slx_sht_create(int nlat, int nlon, int lmax, int precision, int m
       }
       h->syn_mfma = bc.mf;
       h->syn_r = bc.r;
+      h->syn_nw = bc.nw;
     }
     SLX_HIP(hipEventDestroy(e0));
     SLX_HIP(hipEventDestroy(e1));

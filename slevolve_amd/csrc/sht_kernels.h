@@ -1318,7 +1318,7 @@ __device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], L
 // scaled coefficients) forms them here, after its block prologue where its index ranges need one
 // (block-uniform), so that no separate fill kernel runs.  Four entries per thread are loaded
 // before any is stored, so that their loads are in flight together.
-template <int NF, class Coef>
+template <int NF, int NT, class Coef>
 __device__ __forceinline__ void syn_stage(const ShtDev& d, const Coef& cf, const int (&mord)[2], int nord, int np2,
                                           int base1, int nsg, double* __restrict__ qcl, double2* __restrict__ cl) {
   constexpr int S = MF_S, U = 4;
@@ -1330,12 +1330,13 @@ __device__ __forceinline__ void syn_stage(const ShtDev& d, const Coef& cf, const
     cf.prologue(pro);
     __syncthreads();
   }
-  for (int i0 = threadIdx.x; i0 < np2; i0 += 256 * U) {
+  constexpr int nt = NT;  // threads of the block
+  for (int i0 = threadIdx.x; i0 < np2; i0 += nt * U) {
     double qv[U], gv[U];
     double2 cv[U][NF];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int i = i0 + 256 * u;
+      const int i = i0 + nt * u;
       const bool second = i >= base1;
       const int k = second ? i - base1 : i;
       const bool v = i < np2 && k < (second ? cnt1 : cnt0);
@@ -1348,7 +1349,7 @@ __device__ __forceinline__ void syn_stage(const ShtDev& d, const Coef& cf, const
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int i = i0 + 256 * u;
+      const int i = i0 + nt * u;
       if (i < np2) {
         const int sgi = i / S, tq = i % S;
         qcl[sgi * MF_QS + tq] = qv[u];
@@ -1387,7 +1388,7 @@ k_leg_syn_mf(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __r
   double2* zr = cl + NF * nsg * MF_CS;            // [MF_CS] zeros: the B operand of unused columns
   double* part = reinterpret_cast<double*>(lsy);  // [4][2][2][R][64], reuses the staging area
   if (threadIdx.x < MF_CS) zr[threadIdx.x] = make_double2(0.0, 0.0);
-  syn_stage<NF>(d, cf, mord, nord, np2, base1, nsg, qcl, cl);
+  syn_stage<NF, 256>(d, cf, mord, nord, np2, base1, nsg, qcl, cl);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1485,12 +1486,12 @@ k_leg_syn_mf(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __r
 }
 
 
-// Synthesis contraction with the waves over rings: block = (order pair, chunk of 64 R rings), wave w
-// owns rings 16 R w .. 16 R w + 16 R - 1 of the chunk and walks every segment group of the pair
+// Synthesis contraction with the waves over rings: block = (order pair, chunk of 16 NWV R rings), wave
+// w owns rings 16 R w .. 16 R w + 16 R - 1 of the chunk and walks every segment group of the pair
 // (the same degree steps and staged coefficients as k_leg_syn_mf), so the waves carry equal work
 // whatever the pair's group count, and each writes its own rings' FN, FS with no cross-wave sum.
-template <int NF, int R, class Coef>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R <= 2 ? 4 : 1, 8)))
+template <int NF, int R, int NWV, class Coef>
+__global__ void __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(R <= 2 ? 4 : 1, 8)))
 k_leg_syn_mw(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __restrict__ FS, Pred pr) {
   static_assert(NF == 1 || NF == 2, "one or two fields per launch");
   if (!pr.active()) return;
@@ -1499,7 +1500,7 @@ k_leg_syn_mw(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __r
   const int L = d.lmax, nl = L + 1, nh = d.nh;
   // XCD-aware block order: block b runs on XCD b % 8; the ny ring chunks of one order pair go to
   // the same XCD, so that its L2 fetches the pair's staged coefficients once
-  const int ny = (nh + 64 * R - 1) / (64 * R), bt = blockIdx.x >> 3;
+  const int ny = (nh + 16 * NWV * R - 1) / (16 * NWV * R), bt = blockIdx.x >> 3;
   const int yb = bt % ny, p = (blockIdx.x & 7) + 8 * (bt / ny);
   if (p >= (L + 2) / 2) return;
   const int mord[2] = {p, L - p};
@@ -1516,7 +1517,7 @@ k_leg_syn_mw(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __r
   const int blk = (lane >> 2) & 3, ri = lane & 3, slot = lane >> 4;
   const int col = lane & 3, fb = col >> 1, comp = col & 1;
   const bool colok = fb < NF;
-  const int jw = yb * 64 * R + wave * 16 * R;  // this wave's first ring
+  const int jw = yb * 16 * NWV * R + wave * 16 * R;  // this wave's first ring
   int jr[R];
   bool okr[R];
   double xr[R];
@@ -1542,7 +1543,7 @@ k_leg_syn_mw(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __r
   double2 nab[R];
   seeds(0, 0, nab);
   if (threadIdx.x < MF_CS) zr[threadIdx.x] = make_double2(0.0, 0.0);
-  syn_stage<NF>(d, cf, mord, nord, np2, base1, nsg, qcl, cl);
+  syn_stage<NF, 64 * NWV>(d, cf, mord, nord, np2, base1, nsg, qcl, cl);
   __syncthreads();
   if (jw >= nh) return;  // a wave past the last ring (no barrier follows)
   const double s23o0 = d.s2m3[mord[0]], s23o1 = d.s2m3[mord[1]];

@@ -233,17 +233,18 @@ template <class Store>
 int launch_fft_syn(const slx_sht* h, Store st, int nf, Pred pr, hipStream_t s) {
   return ring_points(h) == 4 ? launch_fft_syn_p<4>(h, st, nf, pr, s) : launch_fft_syn_p<8>(h, st, nf, pr, s);
 }
-// Synthesis with the waves over rings: 8 XCD lanes x chunks of 64 R rings x ceil(order pairs / 8)
-// blocks; LDS = the pair's staged tables and coefficients.
-template <int NF, int R, class Coef>
+// Synthesis with the waves over rings: 8 XCD lanes x chunks of 16 NWV R rings x ceil(order pairs /
+// 8) blocks of NWV waves; LDS = the pair's staged tables and coefficients.
+template <int NF, int R, int NWV, class Coef>
 int launch_leg_syn_mw(const slx_sht* h, Coef cf, long off, Pred pr, hipStream_t s) {
-  const int ny = (h->nh + 64 * R - 1) / (64 * R), np = (h->lmax + 2) / 2;
+  const int ny = (h->nh + 16 * NWV * R - 1) / (16 * NWV * R), np = (h->lmax + 2) / 2;
   const dim3 grid(8 * ny * ((np + 7) / 8));
   const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
   const size_t lds = (size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2)) +
                      MF_CS * sizeof(double2);  // + the zero row
-  if (int rc = allow_lds<&k_leg_syn_mw<NF, R, Coef>>(lds)) return rc;
-  k_leg_syn_mw<NF, R, Coef><<<grid, 256, lds, s>>>(dev_of(h), seg_of(h), cf, h->FN + off, h->FSo + off, pr);
+  if (int rc = allow_lds<&k_leg_syn_mw<NF, R, NWV, Coef>>(lds)) return rc;
+  k_leg_syn_mw<NF, R, NWV, Coef><<<grid, 64 * NWV, lds, s>>>(dev_of(h), seg_of(h), cf, h->FN + off, h->FSo + off,
+                                                             pr);
   SLX_LAUNCH_CHECK();
   return 0;
 }
@@ -272,8 +273,13 @@ template <int NF, class Coef>
 int launch_leg_syn(const slx_sht* h, Coef cf, Pred pr, hipStream_t s, int field = 0) {
   const long off = (long)field * h->nm * h->nh;
   if constexpr (NF <= 2) {
-    if (h->syn_mfma == 2) return h->syn_r == 1 ? launch_leg_syn_mw<NF, 1>(h, cf, off, pr, s)
-                                               : launch_leg_syn_mw<NF, 2>(h, cf, off, pr, s);
+    if (h->syn_mfma == 2) {
+      if (h->syn_nw == 8)
+        return h->syn_r == 1 ? launch_leg_syn_mw<NF, 1, 8>(h, cf, off, pr, s)
+                             : launch_leg_syn_mw<NF, 2, 8>(h, cf, off, pr, s);
+      return h->syn_r == 1 ? launch_leg_syn_mw<NF, 1, 4>(h, cf, off, pr, s)
+                           : launch_leg_syn_mw<NF, 2, 4>(h, cf, off, pr, s);
+    }
     if (h->syn_mfma) return h->syn_r == 2 ? launch_leg_syn_mf<NF, 2>(h, cf, off, pr, s)
                                           : launch_leg_syn_mf<NF, 4>(h, cf, off, pr, s);
   }

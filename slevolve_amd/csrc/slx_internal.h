@@ -84,7 +84,8 @@ struct slx_sht {
   int4* qw_blk = nullptr;   // [qw_nblk] units of a block, -1 padded
   int syn_mfma = 0;       // synthesis: 0 wave-uniform scalar-fed VALU, 1 matrix cores with the waves over
                           // segment groups (k_leg_syn_mf), 2 with the waves over rings (k_leg_syn_mw)
-  int syn_r = 4;          // rings per lane of the matrix-core synthesis (2 or 4)
+  int syn_r = 4;          // rings per lane of the matrix-core synthesis (1, 2 or 4)
+  int syn_nw = 4;         // waves per block of the ring-split synthesis (4 or 8)
   int *grp_m = nullptr, *grp_s0 = nullptr;
   double2* CS = nullptr;  // [max_fields][ncoef] synthesis coefficients assembled by a loader
   double2* WS = nullptr;  // [max_fields][nm][nh] analysis: folded even parts
