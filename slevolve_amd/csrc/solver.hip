@@ -641,15 +641,38 @@ __global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it, double2
 // update's block partials in a fixed order (sum|old| = the previous iteration's partials, or the
 // guess's), so that all blocks take the same decision; block 0 publishes it.
 // sh: 2 x 256 doubles; both sums are reduced together, each by the tree of block_sum
+// Any block of 64 or more threads (a multiple of 64) forms the same sums in the same order: 256
+// virtual lanes each add every 256th partial, then the tree of block_sums.
 __device__ double convergence_xi(const Pts& p, int it, int nbu, int nbg, double* sh, double& sa, double& so) {
-  double v[2] = {0.0, 0.0};
-  for (int b = threadIdx.x; b < nbu; b += blockDim.x) v[0] += p.partial[(it & 1) * PSLOT + b];
-  if (it == 0) {
-    for (int b = threadIdx.x; b < nbg; b += blockDim.x) v[1] += p.partial[2 * PSLOT + b];
-  } else {
-    for (int b = threadIdx.x; b < nbu; b += blockDim.x) v[1] += p.partial[((it - 1) & 1) * PSLOT + b];
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int vt = t; vt < 256; vt += nt) {
+    double a = 0.0, o = 0.0;
+    for (int b = vt; b < nbu; b += 256) a += p.partial[(it & 1) * PSLOT + b];
+    if (it == 0) {
+      for (int b = vt; b < nbg; b += 256) o += p.partial[2 * PSLOT + b];
+    } else {
+      for (int b = vt; b < nbu; b += 256) o += p.partial[((it - 1) & 1) * PSLOT + b];
+    }
+    sh[vt] = a;
+    sh[256 + vt] = o;
   }
-  block_sums<2>(v, sh);
+  __syncthreads();
+  for (int vt = t; vt < 128; vt += nt) {
+    sh[vt] += sh[vt + 128];
+    sh[256 + vt] += sh[256 + vt + 128];
+  }
+  __syncthreads();
+  if (t < 64) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      double x = sh[q * 256 + t] + sh[q * 256 + t + 64];
+#pragma unroll
+      for (int s = 32; s > 0; s >>= 1) x += __shfl_down(x, s, 64);
+      if (t == 0) sh[q * 256] = x;
+    }
+  }
+  __syncthreads();
+  double v[2] = {sh[0], sh[256]};
   sa = v[0];
   so = v[1];
   if (fabs(sa - so) < EPS0 && fabs(so) < EPS0) return 0.0;
@@ -657,19 +680,16 @@ __device__ double convergence_xi(const Pts& p, int it, int nbu, int nbg, double*
   return fabs((sa - so) / so);
 }
 
-__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nbu, int nbg) {
-  __shared__ double sh[2 * 256];
-  __shared__ int shf, shm;
+// The iteration's decision from its convergence ratio (one thread): posted to the host, the next
+// sweep's precision, the status block; returns whether the step ends here.
+__device__ bool refresh_decide(const Pts& p, const Consts& c, int it, double xi, double sa, double so, double conserv,
+                               bool post) {
   DevState* s = p.st;
-  if (*(volatile int*)&s->st.done) return;
-  double sa, so;
-  const double xi = convergence_xi(p, it, nbu, nbg, sh, sa, so);
-  const double conserv = conserv_of(p, c);
   // a step converges on an FP64 sweep only; FP32-product sweeps run while xi > eps_sw
   const int fp64 = *(volatile const int*)&s->sweep64[it & 1];
   const bool conv = xi <= c.eps1 && fp64;
   const bool fail = !conv && it + 1 >= c.max_inner;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (post) {
     post_decision(p, it, conv || fail, fail);
     s->sweep64[(it + 1) & 1] = (fp64 || xi <= c.eps_sw) ? 1 : 0;
     if (!fp64) s->st.sweeps32 += 1;
@@ -681,15 +701,21 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
     if (conv || fail) s->st.done = 1;
     if (fail) s->st.failed = 1;
   }
-  if (conv || fail) return;
-  if (threadIdx.x == 0) shf = shm = 0;
-  if (c.mixed && blockIdx.x == 0 && threadIdx.x == 0) s->mchg[(it + 1) & 1] = 0;
+  return conv || fail;
+}
+
+// Topography, ocean function and grounded-flip count of the refresh over the grid points of blocks
+// b0 .. b0 + nb - 1 (block b = blockIdx.x - first block of the grid part).
+__device__ void refresh_grid(const Pts& p, const Consts& c, int it, double conserv, int b, int nb, int* shf,
+                             int* shm) {
+  DevState* s = p.st;
+  if (threadIdx.x == 0) *shf = *shm = 0;
   __syncthreads();
   const long n = c.npts;
   const double cadd = conserv * c.K00;
   const bool regs = c.checkmarine && s->nacc2[it & 1] < GR_MAXACC;
   int flips = 0, mflips = 0;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  for (long i = b * (long)blockDim.x + threadIdx.x; i < n; i += (long)nb * blockDim.x) {
     const double t = p.tinit[i] - (p.dsl[i] + cadd);
     const double cn = t < 0.0 ? 1.0 : 0.0;
     if (c.mixed && cn != p.cxy[i]) mflips += 1;
@@ -702,11 +728,56 @@ __global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nb
       if (bnew != (p.beta[i] != 0.0)) flips += 1;
     }
   }
-  if (flips) atomicAdd(&shf, flips);
-  if (mflips) atomicAdd(&shm, mflips);
+  if (flips) atomicAdd(shf, flips);
+  if (mflips) atomicAdd(shm, mflips);
   __syncthreads();
-  if (threadIdx.x == 0 && shf) atomicAdd(&s->nflip[it & 1], shf);
-  if (threadIdx.x == 0 && shm) atomicAdd(&s->mchg[it & 1], shm);
+  if (threadIdx.x == 0 && *shf) atomicAdd(&s->nflip[it & 1], *shf);
+  if (threadIdx.x == 0 && *shm) atomicAdd(&s->mchg[it & 1], *shm);
+}
+
+__global__ void __launch_bounds__(256) k_refresh(Pts p, Consts c, int it, int nbu, int nbg) {
+  __shared__ double sh[2 * 256];
+  __shared__ int shf, shm;
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done) return;
+  double sa, so;
+  const double xi = convergence_xi(p, it, nbu, nbg, sh, sa, so);
+  const double conserv = conserv_of(p, c);
+  if (refresh_decide(p, c, it, xi, sa, so, conserv, blockIdx.x == 0 && threadIdx.x == 0)) return;
+  if (c.mixed && blockIdx.x == 0 && threadIdx.x == 0) s->mchg[(it + 1) & 1] = 0;
+  refresh_grid(p, c, it, conserv, blockIdx.x, gridDim.x, &shf, &shm);
+}
+
+// The ocean-load update and the refresh in one launch (solver.py:1002-1040), for the loop whose
+// grounded re-guess decides the iteration (k_reguess_ring with decide set).  Blocks 0 .. nbu - 1
+// update the coefficients and post their partial sums of |load_inc| (as k_update); the other
+// blocks refresh the grids with this iteration's sea level and conservation constant without
+// waiting for the convergence test: when the step converges here, the post-loop pass (k_post_grid)
+// rewrites topo and the ocean function from the same sea level and constant, as the reference
+// recomputes both after its loop (solver.py:1053-1058), so this refresh leaves no trace; its flip
+// counts only feed the re-guess, which does not run once the step is done.
+__global__ void __launch_bounds__(256) k_update_refresh(Pts p, Consts c, int it, int nbu, double2* out) {
+  __shared__ double sh[256];
+  __shared__ int shf, shm;
+  DevState* s = p.st;
+  if (*(volatile int*)&s->st.done) return;
+  const double conserv = conserv_of(p, c);
+  if (blockIdx.x >= nbu) {
+    if (c.mixed && blockIdx.x == nbu && threadIdx.x == 0) s->mchg[(it + 1) & 1] = 0;
+    refresh_grid(p, c, it, conserv, blockIdx.x - nbu, gridDim.x - nbu, &shf, &shm);
+    return;
+  }
+  const int n = c.ncoef;
+  double a = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nbu * blockDim.x) {
+    const double2 sp = p.p_ocean[i], rt = p.rt_lm[i], cs = p.cstar_lm[i];
+    const double2 nw = make_double2((-sp.x + rt.x) + cs.x * conserv, (-sp.y + rt.y) + cs.y * conserv);
+    out[i] = nw;
+    p.dS_cur[i] = make_double2(sp.x + nw.x, sp.y + nw.y);
+    a += cabs_d(nw);
+  }
+  const double bs = block_sum(a, sh);
+  if (threadIdx.x == 0) p.partial[(it & 1) * PSLOT + blockIdx.x] = bs;
 }
 
 // The grounded-ice re-guess of an inner iteration, applied by the ice ring pass
@@ -755,8 +826,16 @@ struct LoadReguess {
 template <int P>
 __global__ void __launch_bounds__(RING_MAXNT) k_reguess_ring(ShtDev d, LoadReguess lr, double2* __restrict__ WS,
                                                              double2* __restrict__ WD, int* __restrict__ ring_tok,
-                                                             int token, Pred pr) {
-  if (!pr.active() || !lr.gate()) return;
+                                                             int token, Pred pr, Pts p, Consts c, int nbu,
+                                                             int decide) {
+  if (!pr.active()) return;
+  if (decide) {  // the iteration's convergence test (k_update_refresh left it to this pass)
+    __shared__ double shx[2 * 256];
+    double sa, so;
+    const double xi = convergence_xi(p, lr.it, nbu, nbu, shx, sa, so);
+    if (refresh_decide(p, c, lr.it, xi, sa, so, conserv_of(p, c), blockIdx.x == 0 && threadIdx.x == 0)) return;
+  }
+  if (!lr.gate()) return;
   extern __shared__ double2 smem[];
   const int N = d.nlon, nt = N / P, tid = threadIdx.x;
   double2* a = smem;
@@ -1923,19 +2002,23 @@ extern "C" int slx_step_begin_warm(slx_solver* s, const double* ice_cur, double 
 // incremental grounded re-guess: the ring pass of the change, then the sparse contraction (a
 // persistent grid of one-wave blocks, sized once to what is resident)
 template <int P>
-static int launch_reguess_ring_p(slx_solver* s, const LoadReguess& ld, int token, Pred pr, hipStream_t st) {
+static int launch_reguess_ring_p(slx_solver* s, const LoadReguess& ld, int token, Pred pr, const Pts& p, int decide,
+                                 hipStream_t st) {
   const slx_sht* h = s->h;
   const size_t lds = ring_lds(h);
   if (int rc = allow_lds<&k_reguess_ring<P>>(lds)) return rc;
-  k_reguess_ring<P><<<h->nh, h->nlon / P, lds, st>>>(dev_of(h), ld, h->WS, h->WD, s->ring_tok, token, pr);
+  k_reguess_ring<P><<<h->nh, h->nlon / P, lds, st>>>(dev_of(h), ld, h->WS, h->WD, s->ring_tok, token, pr, p, s->c,
+                                                     red_blocks(s->c.ncoef), decide);
   SLX_LAUNCH_CHECK();
   return 0;
 }
-static int launch_reguess_incremental(slx_solver* s, const LoadReguess& ld, Pred pr_ring, Pred pr_ana, hipStream_t st) {
+// decide: the ring pass first takes the iteration's convergence decision (after k_update_refresh)
+static int launch_reguess_incremental(slx_solver* s, const LoadReguess& ld, Pred pr_ring, Pred pr_ana, const Pts& p,
+                                      int decide, hipStream_t st) {
   const slx_sht* h = s->h;
   const int token = ++s->next_tok;
-  int rc = ring_points(h) == 4 ? launch_reguess_ring_p<4>(s, ld, token, pr_ring, st)
-                               : launch_reguess_ring_p<8>(s, ld, token, pr_ring, st);
+  int rc = ring_points(h) == 4 ? launch_reguess_ring_p<4>(s, ld, token, pr_ring, p, decide, st)
+                               : launch_reguess_ring_p<8>(s, ld, token, pr_ring, p, decide, st);
   if (rc) return rc;
   slx_mark(h, SLX_STAGE_GRID, st);
   return launch_ice_sparse(s, token, pr_ana, st);
@@ -2069,16 +2152,24 @@ static int iter_tail(slx_solver* s, hipStream_t st) {
   }
   if (rc) return rc;
   slx_mark(s->h, SLX_STAGE_SHT, st);
-  k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step, s->load_inc);
-  SLX_LAUNCH_CHECK();
-  slx_mark(s->h, SLX_STAGE_SPEC, st);
-  k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
+  // With the incremental re-guess (and no stage timing, which keeps the launches apart) the update
+  // and the refresh share one launch and the re-guess ring pass takes the convergence decision.
+  const bool fused = c.checkmarine && s->ice_incr && s->h->marks != &s->marks && s->h->nlon / ring_points(s->h) >= 64;
+  if (fused) {
+    const int nbu = red_blocks(c.ncoef);
+    k_update_refresh<<<nbu + grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, nbu, s->load_inc);
+  } else {
+    k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step, s->load_inc);
+    SLX_LAUNCH_CHECK();
+    slx_mark(s->h, SLX_STAGE_SPEC, st);
+    k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
+  }
   SLX_LAUNCH_CHECK();
   slx_mark(s->h, SLX_STAGE_GRID, st);
   if (c.checkmarine) {
     const LoadReguess ld{s->ice_cur, s->topo, s->icestar, s->beta, c.rhow, c.rhoi, s->st, it_step, c.checkmarine};
     if (s->ice_incr) {  // the re-guess changes I* at a few points: analyse the change only
-      rc = launch_reguess_incremental(s, ld, Pred{done, 0}, Pred{done, 0, changed, 1}, st);
+      rc = launch_reguess_incremental(s, ld, Pred{done, 0}, Pred{done, 0, changed, 1}, p, fused ? 1 : 0, st);
     } else {
       rc = launch_fft_ana(s->h, ld, 1, Pred{done, 0}, st);
       slx_mark(s->h, SLX_STAGE_GRID, st);
