@@ -537,7 +537,9 @@ struct LoadPtrs {
 // C* and reduced by tO (both formed from C, beta, tinit, C*0 as they are read) in LDS (rO - tO,
 // solver.py:993-996), forward-transformed and folded into
 // analysis field 0; with nf == 2 (or a set cst_flag) the ocean function C* of the same rows is
-// transformed into analysis field 1 as well.  Replaces separate synthesis and analysis ring passes.
+// transformed into analysis field 1 as well, after it or (up to nlon 1024) by a second block per
+// ring pair (blockIdx.y = 1) that runs beside the rO - tO path.  Replaces separate synthesis and
+// analysis ring passes.
 struct OceanFn {  // C* = C beta and tO = tinit (C* - C*0) of a grid point (solver.py:881-884)
   const double *cxy, *beta, *tinit, *cstar0;
   __device__ __forceinline__ double cstar(long i) const { return cxy[i] * beta[i]; }
@@ -561,10 +563,27 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const doubl
   double2* b = smem + N;
   const int j = xcd_ring(blockIdx.x, d.nh), js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
+  const double wn = d.wn[j], ws = d.ws[j];
+  const long fstride = (long)d.nm * d.nh;
   // every global read of the pass first: the FFT twiddles, the ring pair's Fourier coefficients,
   // then C* and tO of the thread's points of both rows
   TwOf<P, REG> tw;
   tw.load(d.stw, N, d.log2n);
+  if (gridDim.y == 2) {  // C* has blocks of its own (blockIdx.y = 1), concurrent with the rO - tO path
+    if (blockIdx.y == 1) {
+      if (nf < 2) return;
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const long i = tid + k * nt;
+        a[i] = make_double2(oc.cstar(rn + i), oc.cstar(rs + i));
+      }
+      __syncthreads();
+      double2* Z = fft_lds<P>(a, b, tw, N, d.log2n, false);
+      ring_fold(Z, WS, WD, fstride, d.nh, j, L, N, wn, ws, nt);
+      return;
+    }
+    nf = 1;
+  }
   RingCoef<P> rc;
   rc.load(FN, FS, 0, d.nh, j, L, nt);
   double cn[P], cs[P], tn[P], ts[P];
@@ -589,8 +608,6 @@ __global__ void __launch_bounds__(RING_MAXNT) k_fft_synana(ShtDev d, const doubl
     z[i] = make_double2(v.x * cn[k] - tn[k], v.y * cs[k] - ts[k]);
   }
   __syncthreads();
-  const double wn = d.wn[j], ws = d.ws[j];
-  const long fstride = (long)d.nm * d.nh;
   for (int f = 0; f < nf; ++f) {
     if (f == 1) {
 #pragma unroll
@@ -2030,7 +2047,10 @@ static int launch_fft_synana_p(slx_solver* s, int nf, const int* cst, Pred pr, h
   const slx_sht* h = s->h;
   const size_t lds = ring_lds(h);
   if (int rc = allow_lds<&k_fft_synana<P, REG>>(lds)) return rc;
-  k_fft_synana<P, REG><<<h->nh, h->nlon / P, lds, st>>>(dev_of(h), h->FN, h->FSo, s->dsl,
+  // C* transformed by blocks of its own where a ring pair per CU leaves the CUs idle (1 872 -> 1 885
+  // steps/s at lmax 512; at nlon 2048 two blocks per CU are resident already: 585 -> 577 at lmax 1023)
+  const int ny = ((nf == 2 || cst != nullptr) && h->nlon <= 1024) ? 2 : 1;
+  k_fft_synana<P, REG><<<dim3(h->nh, ny), h->nlon / P, lds, st>>>(dev_of(h), h->FN, h->FSo, s->dsl,
                                                          OceanFn{s->cxy, s->beta, s->tinit, s->cstar0}, h->WS, h->WD,
                                                          nf, cst, pr);
   SLX_LAUNCH_CHECK();
