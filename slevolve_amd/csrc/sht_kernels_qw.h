@@ -60,8 +60,10 @@ template <int CA, int CB, int BANKS>
 __device__ __forceinline__ double dpp_merge(double a, double b) {
   const int2 ia = __builtin_bit_cast(int2, a), ib = __builtin_bit_cast(int2, b);
   int2 r;
-  r.x = dpp32<CB, 0xF & ~BANKS>(0, ib.x);
-  r.y = dpp32<CB, 0xF & ~BANKS>(0, ib.y);
+  // the first move leaves the lanes of BANKS as they were (undefined: no register is cleared for
+  // them), the second writes exactly those lanes
+  r.x = __builtin_amdgcn_mov_dpp(ib.x, CB, 0xF, 0xF & ~BANKS, false);
+  r.y = __builtin_amdgcn_mov_dpp(ib.y, CB, 0xF, 0xF & ~BANKS, false);
   r.x = dpp32<CA, BANKS>(r.x, ia.x);
   r.y = dpp32<CA, BANKS>(r.y, ia.y);
   return __builtin_bit_cast(double, r);
