@@ -6,8 +6,11 @@
 //   * rO - tO is analysed as one field (its analysis is all the update uses, solver.py:1002-1015);
 //   * the ocean function C* is analysed in the same sweep as rO - tO of the next iteration;
 //   * deltasl = dslxy + conserv K00 (one synthesis less per iteration, solver_opt.py:28-32);
-//   * eq. B18 is assembled by one coefficient pass (k_coef_fill with the CoefB18 loader, which also
-//     evaluates the rotation) straight into the synthesis contraction's input.
+//   * eq. B18 is formed by the synthesis contraction as it stages each order pair's coefficients
+//     (the CoefB18 loader, which also evaluates the rotation; the mixed policy assembles them once
+//     with k_coef_fill for whichever of its two contraction forms runs);
+//   * with the marine check, the load update and the refresh of topography and ocean function
+//     share one launch and the re-guess ring pass takes the convergence decision.
 // The inner iterations are launched ahead and predicated on a device convergence flag.
 #include <cmath>
 #include <cstring>
