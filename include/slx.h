@@ -233,9 +233,11 @@ int slx_solver_fields(slx_solver* s, slx_fields* out);
 /* Stage timing of profiled steps, the per-stage keys of SeaLevelSolver.step(profile=True)
  * (python/slevolve/solver.py:829-831,954-956; solver_opt.py:382-384).  With on = 1, the launches
  * of slx_step_begin / _iterate / _finish on this solver's stream are followed by timing events
- * ("memory": modal advance and commit; "sht": ring passes and contractions; "spec": coefficient
- * fills, guess and load update; "grid": ocean-function refresh, re-guess ring pass, post-step grids;
- * "rot": degree-2 states and the rotation vector).  slx_solver_stage_times waits for the last event
+ * ("memory": modal advance and commit; "sht": ring passes and contractions, with the synthesis's
+ * B18 coefficients, which it forms while staging them; "spec": the remaining coefficient fills,
+ * guess and load update; "grid": ocean-function refresh, re-guess ring pass, post-step grids;
+ * "rot": degree-2 states and the rotation vector; profiled steps launch the load update and the
+ * refresh separately, which unprofiled steps with the marine check fuse, with the same results).  slx_solver_stage_times waits for the last event
  * and returns the device milliseconds per stage, ms[5] = memory, sht, spec, grid, rot, since the
  * previous call (or since profiling was switched on).  Intended for step-by-step use; the native
  * loop (slx_run_steps) is not profiled.  One solver per transform handle may profile at a time. */
