@@ -1255,7 +1255,7 @@ __global__ void __launch_bounds__(256) k_coef_fill(ShtDev d, Coef cf, double2* _
 // segments 4g .. 4g + 3 of one order), columns j = field x (re, im).  A lane walks the
 // recurrence of its ring through its slot; at step t the K index runs over the degrees
 // l0_k + t, all of the parity of t, so even and odd steps accumulate into separate E and O
-// outputs.  Block = (order pair p, L - p) x 16 R rings (R rings per lane, 4 waves; a 1-D grid dealt
+// outputs.  Block = (order pair of syn_pair) x 16 R rings (R rings per lane, 4 waves; a 1-D grid dealt
 // so that the ring chunks of one order pair share an XCD); the segment groups of both orders are
 // dealt to the waves, whose E/O partials are summed in LDS in a fixed
 // order.  Recurrence and spectral coefficients of both orders are staged in LDS, one order after
@@ -1309,6 +1309,25 @@ __device__ __forceinline__ void syn_mf_steps(double (&aE)[R], double (&aO)[R], L
       aO[r] = __builtin_amdgcn_mfma_f64_4x4x4f64(pv[1][r], bb[(t + 1) & 7], aO[r], 0, 0, 0);
     }
   }
+}
+
+// Order pairs of the synthesis blocks, p < (L + 2) / 2: (p, L - p) for odd L; for even L, (p, L - 1 - p)
+// for p < L / 2 and order L alone (one degree) for p = L / 2, so that every pair but that one spans
+// L + 2 or L + 3 degrees (pairing p with L - p would leave order L / 2, L / 2 + 1 degrees, alone in a
+// block of half the others' work).  Returns the number of orders.
+__device__ __forceinline__ int syn_pair(int L, int p, int (&mord)[2]) {
+  if (L & 1) {
+    mord[0] = p;
+    mord[1] = L - p;
+    return 2;
+  }
+  if (p == L / 2) {
+    mord[0] = mord[1] = L;
+    return 1;
+  }
+  mord[0] = p;
+  mord[1] = L - 1 - p;
+  return 2;
 }
 
 // Staging of an order pair for the synthesis contractions: the scaled-recurrence coefficients qc
@@ -1375,12 +1394,12 @@ k_leg_syn_mf(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __r
   const int ny = (nh + 16 * R - 1) / (16 * R), bt = blockIdx.x >> 3;
   const int yb = bt % ny, p = (blockIdx.x & 7) + 8 * (bt / ny);
   if (p >= (L + 2) / 2) return;
-  const int mord[2] = {p, L - p};
-  const int nord = (L - p == p) ? 1 : 2;
+  int mord[2];
+  const int nord = syn_pair(L, p, mord);
   const int ng0 = ((nl - mord[0] + S - 1) / S + 3) / 4;
   const int ng1 = nord == 2 ? ((nl - mord[1] + S - 1) / S + 3) / 4 : 0;
-  // staging, packed: order p's ng0 groups of 4 S degrees, then order L - p's ng1 groups, zero past
-  // each order's last degree; np2 = 4 S (ng0 + ng1) <= 4 S (ceil((L + 2) / (4 S)) + 1)
+  // staging, packed: order mord[0]'s ng0 groups of 4 S degrees, then order mord[1]'s ng1 groups,
+  // zero past each order's last degree; np2 = 4 S (ng0 + ng1) <= 4 S (ceil((L + 3) / (4 S)) + 1)
   const int np2 = 4 * S * (ng0 + ng1), base1 = 4 * S * ng0;
   const int nsg = np2 / S;  // staged segments, each padded to MF_QS / MF_CS entries
   double* qcl = reinterpret_cast<double*>(lsy);  // [nsg][MF_QS] scaled recurrence coefficients
@@ -1503,8 +1522,8 @@ k_leg_syn_mw(ShtDev d, SegDev g, Coef cf, double2* __restrict__ FN, double2* __r
   const int ny = (nh + 16 * NWV * R - 1) / (16 * NWV * R), bt = blockIdx.x >> 3;
   const int yb = bt % ny, p = (blockIdx.x & 7) + 8 * (bt / ny);
   if (p >= (L + 2) / 2) return;
-  const int mord[2] = {p, L - p};
-  const int nord = (L - p == p) ? 1 : 2;
+  int mord[2];
+  const int nord = syn_pair(L, p, mord);
   const int ng0 = ((nl - mord[0] + S - 1) / S + 3) / 4;
   const int ng1 = nord == 2 ? ((nl - mord[1] + S - 1) / S + 3) / 4 : 0;
   const int np2 = 4 * S * (ng0 + ng1), base1 = 4 * S * ng0;

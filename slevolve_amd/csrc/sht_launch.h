@@ -239,7 +239,7 @@ template <int NF, int R, int NWV, class Coef>
 int launch_leg_syn_mw(const slx_sht* h, Coef cf, long off, Pred pr, hipStream_t s) {
   const int ny = (h->nh + 16 * NWV * R - 1) / (16 * NWV * R), np = (h->lmax + 2) / 2;
   const dim3 grid(8 * ny * ((np + 7) / 8));
-  const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
+  const int np2 = 4 * MF_S * ((h->lmax + 3 + 4 * MF_S - 1) / (4 * MF_S) + 1);
   const size_t lds = (size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2)) +
                      MF_CS * sizeof(double2);  // + the zero row
   if (int rc = allow_lds<&k_leg_syn_mw<NF, R, NWV, Coef>>(lds)) return rc;
@@ -254,8 +254,8 @@ int launch_leg_syn_mf(const slx_sht* h, Coef cf, long off, Pred pr, hipStream_t 
   const int ny = (h->nh + 16 * R - 1) / (16 * R), np = (h->lmax + 2) / 2;
   const dim3 grid(8 * ny * ((np + 7) / 8));
   // staged entries of an order pair: ceil(n0 / 4S) + ceil(n1 / 4S) groups of 4S degrees with
-  // n0 + n1 = L + 2, at most ceil((L + 2) / 4S) + 1 groups
-  const int np2 = 4 * MF_S * ((h->lmax + 2 + 4 * MF_S - 1) / (4 * MF_S) + 1);
+  // n0 + n1 <= L + 3 (syn_pair), at most ceil((L + 3) / 4S) + 1 groups
+  const int np2 = 4 * MF_S * ((h->lmax + 3 + 4 * MF_S - 1) / (4 * MF_S) + 1);
   const size_t lds = std::max((size_t)(np2 / MF_S) * (MF_QS * sizeof(double) + NF * MF_CS * sizeof(double2)) +
                                   MF_CS * sizeof(double2),  // + the zero row
                               (size_t)4 * 2 * 2 * R * 64 * sizeof(double));

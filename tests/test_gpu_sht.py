@@ -16,7 +16,7 @@ def rand_coeffs(L, rng):
 
 
 @pytest.mark.parametrize("variant", ["mfma2", "mfma4", "mw1", "mw2", "mw1x8", "mw2x8", "scalar"])
-@pytest.mark.parametrize("nlat,L", [(64, 63), (128, 63), (64, 31), (512, 511), (512, 512), (1024, 1023)])
+@pytest.mark.parametrize("nlat,L", [(64, 63), (128, 63), (64, 31), (64, 62), (16, 4), (512, 511), (512, 512), (1024, 1023)])
 def test_synthesis_and_analysis_match_oracle(nlat, L, variant, monkeypatch):
     """Every synthesis kernel (matrix cores with the waves over segment groups or over rings, and
     scalar-fed VALU; SLX_SYN_VARIANT is read when a transform handle is created) and the analysis
