@@ -665,13 +665,24 @@ __global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it, double2
 // virtual lanes each add every 256th partial, then the tree of block_sums.
 __device__ double convergence_xi(const Pts& p, int it, int nbu, int nbg, double* sh, double& sa, double& so) {
   const int t = threadIdx.x, nt = blockDim.x;
+  // a virtual lane's partials (at most PSLOT / 256 of each sum) are loaded together, then added in
+  // index order; absent ones read as +0.0, which leaves the sums of non-negative partials unchanged
+  constexpr int NQ = PSLOT / 256;
+  const double* pa = p.partial + (it & 1) * PSLOT;
+  const double* po = p.partial + (it == 0 ? 2 : ((it - 1) & 1)) * PSLOT;
+  const int nbo = it == 0 ? nbg : nbu;
   for (int vt = t; vt < 256; vt += nt) {
+    double av[NQ], ov[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      av[q] = vt + 256 * q < nbu ? pa[vt + 256 * q] : 0.0;
+      ov[q] = vt + 256 * q < nbo ? po[vt + 256 * q] : 0.0;
+    }
     double a = 0.0, o = 0.0;
-    for (int b = vt; b < nbu; b += 256) a += p.partial[(it & 1) * PSLOT + b];
-    if (it == 0) {
-      for (int b = vt; b < nbg; b += 256) o += p.partial[2 * PSLOT + b];
-    } else {
-      for (int b = vt; b < nbu; b += 256) o += p.partial[((it - 1) & 1) * PSLOT + b];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      a += av[q];
+      o += ov[q];
     }
     sh[vt] = a;
     sh[256 + vt] = o;
@@ -811,9 +822,15 @@ struct LoadReguess {
   double rhow, rhoi;
   DevState* st;
   int it, checkmarine;
-  __device__ bool gate() const {
+  // the gate's counters (nacc2, nflip of this iteration's slot), loaded together
+  __device__ int2 counters() const {
     const int slot = it & 1;
-    const int nacc = st->nacc2[slot], nflip = st->nflip[slot];  // loaded together
+    return make_int2(st->nacc2[slot], st->nflip[slot]);
+  }
+  __device__ bool gate() const { return gate(counters()); }
+  __device__ bool gate(int2 cnt) const {
+    const int slot = it & 1;
+    const int nacc = cnt.x, nflip = cnt.y;
     const bool regs = checkmarine && nacc < GR_MAXACC;
     const bool changed = regs && nflip > 0;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
@@ -842,20 +859,23 @@ struct LoadReguess {
 // LoadReguess), every block stores I* and beta of its ring pair where they change (as LoadReguess
 // does everywhere) and forms the change dI* = I*new - I*old; a ring pair whose rows did not change ends there, the
 // others transform the change, fold it into WS/WD field 0 (their own columns) and tag the ring
-// with token for k_leg_ana_sp, which adds the analysis of the change to I*_lm.
+// with token for k_leg_ana_sp, which adds the analysis of the change to I*_lm.  The gate's counters
+// are loaded with the convergence test's partials (the ring pair's reads wait for the gate: about
+// half the iterations flip no grounded flag and read nothing more).
 template <int P>
 __global__ void __launch_bounds__(RING_MAXNT) k_reguess_ring(ShtDev d, LoadReguess lr, double2* __restrict__ WS,
                                                              double2* __restrict__ WD, int* __restrict__ ring_tok,
                                                              int token, Pred pr, Pts p, Consts c, int nbu,
                                                              int decide) {
   if (!pr.active()) return;
+  const int2 gc = lr.counters();
   if (decide) {  // the iteration's convergence test (k_update_refresh left it to this pass)
     __shared__ double shx[2 * 256];
     double sa, so;
     const double xi = convergence_xi(p, lr.it, nbu, nbu, shx, sa, so);
     if (refresh_decide(p, c, lr.it, xi, sa, so, conserv_of(p, c), blockIdx.x == 0 && threadIdx.x == 0)) return;
   }
-  if (!lr.gate()) return;
+  if (!lr.gate(gc)) return;
   extern __shared__ double2 smem[];
   const int N = d.nlon, nt = N / P, tid = threadIdx.x;
   double2* a = smem;
