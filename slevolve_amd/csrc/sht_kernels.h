@@ -985,30 +985,33 @@ k_leg_ana_sp(ShtDev d, SegDev g, GrpDev gr, const double2* __restrict__ WS, cons
   __shared__ double redw[8 * 64];  // the wave's partials of one ring chunk
   __shared__ double qcs[4 * MF_QS];
   __shared__ double qgs[4 * MF_QS];
-  __shared__ int list[512];  // nh <= 512 on the matrix-core path
+  __shared__ int list[513];  // nh <= 512 on the matrix-core path, + a slot for unlisted lanes
   const int L = d.lmax, nh = d.nh, lane = threadIdx.x;
-  int tk[8];  // every ring's token loaded first (one round trip), then listed
+  // every ring's token and this wave's first group descriptor are loaded first (one round trip),
+  // then the tagged rings are listed
+  const int per = (gr.ngrp + 7) >> 3;
+  const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
+  const int gend = min(gr.ngrp, (x + 1) * per);
+  int gi = x * per + (blockIdx.x >> 3);
+  if (gi >= gend) return;
+  int4 ds = gr.desc[gi];
+  int tk[8];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) tk[u] = (u * 64 + lane < nh) ? ring_tok[u * 64 + lane] : 0;
+  for (int u = 0; u < 8; ++u) tk[u] = ring_tok[min(u * 64 + lane, nh - 1)];
   int nact = 0;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const bool on = u * 64 + lane < nh && tk[u] == token;
     const unsigned long long bal = __ballot(on);
-    if (on) list[nact + __popcll(bal & ((1ull << lane) - 1ull))] = u * 64 + lane;
+    list[on ? nact + __popcll(bal & ((1ull << lane) - 1ull)) : 512] = u * 64 + lane;  // no branch
     nact += __popcll(bal);
   }
-  if (nact == 0) return;
+  // (the descriptor takes part in the test, m <= L always, so that its load is not sunk below it)
+  if (nact == 0 || ds.x > L) return;
   __syncthreads();
-  const int per = (gr.ngrp + 7) >> 3;
-  const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
-  const int gend = min(gr.ngrp, (x + 1) * per);
   const int slot = lane & 3;
   const int nchunk = (nact + 16 * R - 1) / (16 * R);
   const int row = lane >> 4, cj = lane & 3;
-  int gi = x * per + (blockIdx.x >> 3);
-  if (gi >= gend) return;
-  int4 ds = gr.desc[gi];
   int m = ds.x, s0 = ds.y, sz = ds.z, mo = ds.w;  // carried as scalars (an int4 went to the stack)
   for (; gi < gend; gi += nbx) {
     const int nsm = (L - m + S) / S;
