@@ -746,17 +746,40 @@ __device__ void refresh_grid(const Pts& p, const Consts& c, int it, double conse
   const double cadd = conserv * c.K00;
   const bool regs = c.checkmarine && s->nacc2[it & 1] < GR_MAXACC;
   int flips = 0, mflips = 0;
-  for (long i = b * (long)blockDim.x + threadIdx.x; i < n; i += (long)nb * blockDim.x) {
-    const double t = p.tinit[i] - (p.dsl[i] + cadd);
-    const double cn = t < 0.0 ? 1.0 : 0.0;
-    if (c.mixed && cn != p.cxy[i]) mflips += 1;
-    p.topo[i] = t;
-    p.cxy[i] = cn;
+  // one grid point: topography and ocean function from the sea level, flip counts
+  auto point = [&](double ti, double ds, double cx, double ice, double be, double& t, double& cn) {
+    t = ti - (ds + cadd);
+    cn = t < 0.0 ? 1.0 : 0.0;
+    if (c.mixed && cn != cx) mflips += 1;
     if (regs) {
-      const double ice = p.ice_cur[i];
       const double cand = ice * keep_factor(ice, t, c.rhow, c.rhoi);
       const bool bnew = cand < EPS0;
-      if (bnew != (p.beta[i] != 0.0)) flips += 1;
+      if (bnew != (be != 0.0)) flips += 1;
+    }
+  };
+  const uintptr_t al = reinterpret_cast<uintptr_t>(p.tinit) | reinterpret_cast<uintptr_t>(p.dsl) |
+                       reinterpret_cast<uintptr_t>(p.topo) | reinterpret_cast<uintptr_t>(p.cxy) |
+                       reinterpret_cast<uintptr_t>(p.ice_cur) | reinterpret_cast<uintptr_t>(p.beta);
+  if ((n & 1) == 0 && (al & 15) == 0) {  // two points per thread, 16-byte accesses
+    const long n2 = n >> 1;
+    for (long i = b * (long)blockDim.x + threadIdx.x; i < n2; i += (long)nb * blockDim.x) {
+      const double2 ti = reinterpret_cast<const double2*>(p.tinit)[i];
+      const double2 ds = reinterpret_cast<const double2*>(p.dsl)[i];
+      const double2 cx = c.mixed ? reinterpret_cast<const double2*>(p.cxy)[i] : make_double2(0.0, 0.0);
+      const double2 ice = regs ? reinterpret_cast<const double2*>(p.ice_cur)[i] : make_double2(0.0, 0.0);
+      const double2 be = regs ? reinterpret_cast<const double2*>(p.beta)[i] : make_double2(0.0, 0.0);
+      double2 t, cn;
+      point(ti.x, ds.x, cx.x, ice.x, be.x, t.x, cn.x);
+      point(ti.y, ds.y, cx.y, ice.y, be.y, t.y, cn.y);
+      reinterpret_cast<double2*>(p.topo)[i] = t;
+      reinterpret_cast<double2*>(p.cxy)[i] = cn;
+    }
+  } else {
+    for (long i = b * (long)blockDim.x + threadIdx.x; i < n; i += (long)nb * blockDim.x) {
+      double t, cn;
+      point(p.tinit[i], p.dsl[i], c.mixed ? p.cxy[i] : 0.0, regs ? p.ice_cur[i] : 0.0, regs ? p.beta[i] : 0.0, t, cn);
+      p.topo[i] = t;
+      p.cxy[i] = cn;
     }
   }
   if (flips) atomicAdd(shf, flips);
@@ -1630,6 +1653,8 @@ static int grid_blocks(long n) {
   long b = (n + 255) / 256;
   return (int)(b > 2048 ? 2048 : b);
 }
+// blocks of the grid part of the refresh (two points per thread)
+static int refresh_blocks(long n) { return grid_blocks((n + 1) / 2); }
 // blocks of the spectral kernels that post block partials (<= PSLOT)
 static int red_blocks(long n) {
   long b = (n + 511) / 512;
@@ -2200,12 +2225,12 @@ static int iter_tail(slx_solver* s, hipStream_t st) {
   const bool fused = c.checkmarine && s->ice_incr && s->h->marks != &s->marks && s->h->nlon / ring_points(s->h) >= 64;
   if (fused) {
     const int nbu = red_blocks(c.ncoef);
-    k_update_refresh<<<nbu + grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, nbu, s->load_inc);
+    k_update_refresh<<<nbu + refresh_blocks(c.npts), 256, 0, st>>>(p, c, it_step, nbu, s->load_inc);
   } else {
     k_update<<<red_blocks(c.ncoef), 256, 0, st>>>(p, c, it_step, s->load_inc);
     SLX_LAUNCH_CHECK();
     slx_mark(s->h, SLX_STAGE_SPEC, st);
-    k_refresh<<<grid_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
+    k_refresh<<<refresh_blocks(c.npts), 256, 0, st>>>(p, c, it_step, red_blocks(c.ncoef), red_blocks(c.ncoef));
   }
   SLX_LAUNCH_CHECK();
   slx_mark(s->h, SLX_STAGE_GRID, st);
