@@ -407,8 +407,12 @@ struct LoadIceGrid {
   int checkmarine;
   __device__ double operator()(int f, long i) const {
     const double v = ice[i];
-    const double ist = checkmarine ? v * keep_factor(v, topo_prev[i], rhow, rhoi) : v;
+    const double ist = ist_of(v, checkmarine ? topo_prev[i] : 0.0);
     return f == 0 ? ist : cxy[i] * (ist < EPS0 ? 1.0 : 0.0);
+  }
+  // I* of a point from its ice and previous topography (the topography is not read unchecked)
+  __device__ double ist_of(double v, double t) const {
+    return checkmarine ? v * keep_factor(v, t, rhow, rhoi) : v;
   }
   __device__ void commit(int f, long i, double val) const {
     if (f == 0) {
@@ -665,8 +669,9 @@ __global__ void __launch_bounds__(256) k_update(Pts p, Consts c, int it, double2
 // virtual lanes each add every 256th partial, then the tree of block_sums.
 __device__ double convergence_xi(const Pts& p, int it, int nbu, int nbg, double* sh, double& sa, double& so) {
   const int t = threadIdx.x, nt = blockDim.x;
-  // a virtual lane's partials (at most PSLOT / 256 of each sum) are loaded together, then added in
-  // index order; absent ones read as +0.0, which leaves the sums of non-negative partials unchanged
+  // a virtual lane's partials (at most PSLOT / 256 of each sum) are loaded together, without a
+  // branch (every index < PSLOT lies in its slot), then added in index order with the absent ones
+  // as +0.0, which leaves the sums of non-negative partials unchanged
   constexpr int NQ = PSLOT / 256;
   const double* pa = p.partial + (it & 1) * PSLOT;
   const double* po = p.partial + (it == 0 ? 2 : ((it - 1) & 1)) * PSLOT;
@@ -675,14 +680,14 @@ __device__ double convergence_xi(const Pts& p, int it, int nbu, int nbg, double*
     double av[NQ], ov[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      av[q] = vt + 256 * q < nbu ? pa[vt + 256 * q] : 0.0;
-      ov[q] = vt + 256 * q < nbo ? po[vt + 256 * q] : 0.0;
+      av[q] = pa[vt + 256 * q];
+      ov[q] = po[vt + 256 * q];
     }
     double a = 0.0, o = 0.0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      a += av[q];
-      o += ov[q];
+      a += vt + 256 * q < nbu ? av[q] : 0.0;
+      o += vt + 256 * q < nbo ? ov[q] : 0.0;
     }
     sh[vt] = a;
     sh[256 + vt] = o;
@@ -868,10 +873,8 @@ struct LoadReguess {
     }
     return changed;
   }
-  __device__ double operator()(int, long i) const {
-    const double v = ice[i];
-    return v * keep_factor(v, topo[i], rhow, rhoi);
-  }
+  __device__ double operator()(int, long i) const { return of(ice[i], topo[i]); }
+  __device__ double of(double v, double t) const { return v * keep_factor(v, t, rhow, rhoi); }
   __device__ void commit(int, long i, double cand) const {
     icestar[i] = cand;
     beta[i] = cand < EPS0 ? 1.0 : 0.0;
@@ -905,18 +908,30 @@ __global__ void __launch_bounds__(RING_MAXNT) k_reguess_ring(ShtDev d, LoadRegue
   double2* b = smem + N;
   const int j = xcd_ring(blockIdx.x, d.nh), js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
+  // every point's reads before any store (the stores could alias them, so interleaved they would
+  // take a round trip per point)
+  double on[P], os[P], vn[P], vs[P], tn[P], ts[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const long in = rn + tid + k * nt, is = rs + tid + k * nt;
+    on[k] = lr.icestar[in];
+    os[k] = lr.icestar[is];
+    vn[k] = lr.ice[in];
+    vs[k] = lr.ice[is];
+    tn[k] = lr.topo[in];
+    ts[k] = lr.topo[is];
+  }
   double dn[P], dv[P];
   int any = 0;
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     const long in = rn + tid + k * nt, is = rs + tid + k * nt;
-    const double on = lr.icestar[in], os = lr.icestar[is];
-    const double cn = lr(0, in), cs = lr(0, is);
+    const double cn = lr.of(vn[k], tn[k]), cs = lr.of(vs[k], ts[k]);
     // I* and beta = (I* < eps) are always stored together, so only changed points need storing
-    if (cn != on) lr.commit(0, in, cn);
-    if (cs != os) lr.commit(0, is, cs);
-    dn[k] = cn - on;
-    dv[k] = cs - os;
+    if (cn != on[k]) lr.commit(0, in, cn);
+    if (cs != os[k]) lr.commit(0, is, cs);
+    dn[k] = cn - on[k];
+    dv[k] = cs - os[k];
     any |= (dn[k] != 0.0 || dv[k] != 0.0) ? 1 : 0;
   }
   if (!__syncthreads_or(any)) return;
@@ -947,19 +962,33 @@ __global__ void __launch_bounds__(RING_MAXNT) k_begin_ring(ShtDev d, LoadIceGrid
   double2* b = smem + N;
   const int j = xcd_ring(blockIdx.x, d.nh), js = d.nlat - 1 - j;
   const long rn = (long)j * N, rs = (long)js * N;
+  // every point's reads before any store (the stores could alias them, so interleaved they would
+  // take a round trip per point)
+  double on[P], os[P], vn[P], vs[P], tn[P], ts[P], xn[P], xs[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const long in = rn + tid + k * nt, is = rs + tid + k * nt;
+    on[k] = lg.icestar[in];
+    os[k] = lg.icestar[is];
+    vn[k] = lg.ice[in];
+    vs[k] = lg.ice[is];
+    tn[k] = lg.checkmarine ? lg.topo_prev[in] : 0.0;
+    ts[k] = lg.checkmarine ? lg.topo_prev[is] : 0.0;
+    xn[k] = lg.cxy[in];
+    xs[k] = lg.cxy[is];
+  }
   double dn[P], dv[P];
   int any = 0;
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     const long in = rn + tid + k * nt, is = rs + tid + k * nt;
-    const double on = lg.icestar[in], os = lg.icestar[is];
-    const double cn = lg(0, in), cs = lg(0, is);
-    if (cn != on) lg.commit(0, in, cn);
-    if (cs != os) lg.commit(0, is, cs);
-    lg.cstar[in] = lg.cxy[in] * (cn < EPS0 ? 1.0 : 0.0);
-    lg.cstar[is] = lg.cxy[is] * (cs < EPS0 ? 1.0 : 0.0);
-    dn[k] = cn - on;
-    dv[k] = cs - os;
+    const double cn = lg.ist_of(vn[k], tn[k]), cs = lg.ist_of(vs[k], ts[k]);
+    if (cn != on[k]) lg.commit(0, in, cn);
+    if (cs != os[k]) lg.commit(0, is, cs);
+    lg.cstar[in] = xn[k] * (cn < EPS0 ? 1.0 : 0.0);
+    lg.cstar[is] = xs[k] * (cs < EPS0 ? 1.0 : 0.0);
+    dn[k] = cn - on[k];
+    dv[k] = cs - os[k];
     any |= (dn[k] != 0.0 || dv[k] != 0.0) ? 1 : 0;
   }
   if (!__syncthreads_or(any)) return;
