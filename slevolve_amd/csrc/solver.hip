@@ -828,12 +828,24 @@ __global__ void __launch_bounds__(256) k_update_refresh(Pts p, Consts c, int it,
   }
   const int n = c.ncoef;
   double a = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nbu * blockDim.x) {
+  // two strides per pass, both coefficients' reads before either's stores (which may alias them);
+  // |load_inc| is added in the same order as one stride at a time
+  const int st = nbu * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * st) {
+    const int i2 = i + st;
+    const bool two = i2 < n;
     const double2 sp = p.p_ocean[i], rt = p.rt_lm[i], cs = p.cstar_lm[i];
+    const double2 sp2 = p.p_ocean[two ? i2 : i], rt2 = p.rt_lm[two ? i2 : i], cs2 = p.cstar_lm[two ? i2 : i];
     const double2 nw = make_double2((-sp.x + rt.x) + cs.x * conserv, (-sp.y + rt.y) + cs.y * conserv);
+    const double2 nw2 = make_double2((-sp2.x + rt2.x) + cs2.x * conserv, (-sp2.y + rt2.y) + cs2.y * conserv);
     out[i] = nw;
     p.dS_cur[i] = make_double2(sp.x + nw.x, sp.y + nw.y);
     a += cabs_d(nw);
+    if (two) {
+      out[i2] = nw2;
+      p.dS_cur[i2] = make_double2(sp2.x + nw2.x, sp2.y + nw2.y);
+      a += cabs_d(nw2);
+    }
   }
   const double bs = block_sum(a, sh);
   if (threadIdx.x == 0) p.partial[(it & 1) * PSLOT + blockIdx.x] = bs;
